@@ -1,0 +1,241 @@
+"""gudni_amd -- MI355X (gfx950) raster path of Gudni behind its own C ABI.
+
+Python mirror of the reference's host interface for this path:
+
+  Rasterizer(spec)          setupOpenCL (OpenCL/Setup.hs:102-147)
+  RasterSpec                RasterSpec (OpenCL/Rasterizer.hs:37-51)
+  Geometry(scene)           buildOverScene/onShape serialisation
+                            (Raster/Serialize.hs:148-177)
+  RasterJobs(spec, geo)     buildTileTree + addShapeToTree + buildRasterJobs
+                            (Raster/TileTree.hs:81-190, CallKernels.hs:244-255)
+  Rasterizer.raster_job     generateCall (OpenCL/CallKernels.hs:88-179)
+  Rasterizer.queue_jobs     queueRasterJobs (OpenCL/CallKernels.hs:218-242)
+
+Every device call goes through libgudni_hip.so; if it is missing the
+import of the native library raises -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import Spec as RasterSpec
+
+__all__ = ["RasterSpec", "default_spec", "Geometry", "RasterJobs", "Rasterizer", "GudniError",
+           "random_field"]
+
+ERRORS = {-1: "bad argument", -2: "HIP runtime error", -3: "threshold queue overflow",
+          -4: "section guard exceeded", -5: "no HIP device"}
+
+
+class GudniError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"{ERRORS.get(code, 'error')} ({code}): {message}")
+        self.code = code
+
+
+def _ptr(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def default_spec(**overrides) -> RasterSpec:
+    s = RasterSpec()
+    _lib.load().gudni_default_spec(C.byref(s))
+    for k, v in overrides.items():
+        setattr(s, k, v)
+    return s
+
+
+def random_field(n: int = 4096) -> np.ndarray:
+    """The random field handed to the kernels (Util/RandomField.hs).  Its
+    values only matter when stochastic_factor != 0; a fixed golden-ratio
+    sequence keeps frames reproducible."""
+    x = np.arange(n, dtype=np.float64)
+    return ((x * 0.6180339887498949) % 1.0).astype(np.float32)
+
+
+def _check_host(rc):
+    if rc != 0:
+        msg = _lib.load().gudni_last_error(None)
+        raise GudniError(rc, msg.decode() if msg else "")
+
+
+class Geometry:
+    """Serialised scene: strand heap, 16-byte shape records, bounding boxes and
+    strand counts of the shapes that survive excludeBox."""
+
+    def __init__(self, scene):
+        L = _lib.load()
+        self._arrays = [np.ascontiguousarray(scene.tags, np.uint64),
+                        np.ascontiguousarray(scene.shape_outline_offsets, np.int32),
+                        np.ascontiguousarray(scene.outline_pair_offsets, np.int32),
+                        np.ascontiguousarray(scene.pairs, np.float32)]
+        o = _lib.Outlines(scene.n_shapes, *[a.ctypes.data for a in self._arrays])
+        self.handle = C.c_void_p()
+        _check_host(L.gudni_geometry_build(C.byref(o), scene.width, scene.height,
+                                           C.byref(self.handle)))
+        v = _lib.GeometryView()
+        L.gudni_geometry_view_get(self.handle, C.byref(v))
+        n = v.n_shapes
+        self.n_shapes = n
+        self.geo = np.ctypeslib.as_array(v.geo, (v.geo_bytes,)).copy() if v.geo_bytes else np.zeros(0, np.uint8)
+        self.shapes = np.ctypeslib.as_array(v.shapes, (n * 16,)).copy() if n else np.zeros(0, np.uint8)
+        self.bbox = np.ctypeslib.as_array(v.bbox, (n * 4,)).reshape(n, 4).copy() if n else np.zeros((0, 4), np.float32)
+        self.strand_count = np.ctypeslib.as_array(v.strand_count, (n,)).copy() if n else np.zeros(0, np.uint32)
+        self.source_index = np.ctypeslib.as_array(v.source_index, (n,)).copy() if n else np.zeros(0, np.int32)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                _lib.load().gudni_geometry_free(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class RasterJobs:
+    """The tile tree flattened into raster jobs, in traversal order."""
+
+    def __init__(self, spec: RasterSpec, geometry: Geometry, width: int, height: int):
+        L = _lib.load()
+        self.handle = C.c_void_p()
+        _check_host(L.gudni_jobs_build(C.byref(spec), geometry.handle, width, height,
+                                       C.byref(self.handle)))
+        v = _lib.JobsView()
+        L.gudni_jobs_view_get(self.handle, C.byref(v))
+        self.n_jobs = v.n_jobs
+        self.n_tiles = v.n_tiles
+        self.n_refs = v.n_refs
+        self.job_tile_offset = np.ctypeslib.as_array(v.job_tile_offset, (v.n_jobs + 1,)).copy()
+        self.job_shape_offset = np.ctypeslib.as_array(v.job_shape_offset, (v.n_jobs + 1,)).copy()
+        self.tiles = np.ctypeslib.as_array(v.tiles, (v.n_tiles * 32,)).copy()
+        self.shape_refs = np.ctypeslib.as_array(v.shape_refs, (v.n_refs * 16,)).copy() \
+            if v.n_refs else np.zeros(0, np.uint8)
+
+    def job(self, i):
+        t0, t1 = self.job_tile_offset[i], self.job_tile_offset[i + 1]
+        s0, s1 = self.job_shape_offset[i], self.job_shape_offset[i + 1]
+        return self.tiles[t0 * 32: t1 * 32], self.shape_refs[s0 * 16: s1 * 16]
+
+    def __del__(self):
+        try:
+            if self.handle:
+                _lib.load().gudni_jobs_free(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class Rasterizer:
+    """One device context (the reference's Rasterizer)."""
+
+    def __init__(self, spec: RasterSpec | None = None, device: int = 0):
+        self.L = _lib.load()
+        self.spec = spec or default_spec()
+        self.ctx = C.c_void_p()
+        rc = self.L.gudni_create(C.byref(self.spec), device, C.byref(self.ctx))
+        if rc != 0:
+            msg = self.L.gudni_last_error(None)
+            raise GudniError(rc, msg.decode() if msg else "")
+        self._keep = []
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = self.L.gudni_last_error(self.ctx)
+            raise GudniError(rc, msg.decode() if msg else "")
+
+    def close(self):
+        if self.ctx:
+            self.L.gudni_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream_handle: int | None):
+        self._check(self.L.gudni_set_stream(self.ctx, C.c_void_p(stream_handle or 0)))
+
+    def upload_common(self, geo: np.ndarray, scene, rnd: np.ndarray | None = None):
+        rnd = random_field(self.spec.random_field_size) if rnd is None else rnd
+        geo = np.ascontiguousarray(geo, np.uint8)
+        subs = np.ascontiguousarray(scene.substances, np.float32)
+        pict = np.ascontiguousarray(scene.pict_data, np.uint8)
+        uses = np.ascontiguousarray(scene.pict_uses, np.uint8)
+        rnd = np.ascontiguousarray(rnd, np.float32)
+        self._check(self.L.gudni_upload_common(
+            self.ctx, _ptr(geo), geo.size, _ptr(subs), subs.shape[0], _ptr(pict), pict.size,
+            _ptr(uses) if uses.size else None, uses.shape[0], _ptr(rnd), rnd.size))
+
+    def raster_job(self, shapes: np.ndarray, tiles: np.ndarray, background, width, height,
+                   frame=0, job_index=0, out: np.ndarray | None = None) -> np.ndarray:
+        out = np.zeros(width * height, np.uint32) if out is None else out
+        bg = np.ascontiguousarray(background, np.float32)
+        shapes = np.ascontiguousarray(shapes, np.uint8)
+        tiles = np.ascontiguousarray(tiles, np.uint8)
+        self._check(self.L.gudni_raster_job(self.ctx, _ptr(shapes) if shapes.size else None,
+                                            shapes.size // 16, _ptr(tiles) if tiles.size else None,
+                                            tiles.size // 32, _ptr(bg), width, height, frame,
+                                            job_index, _ptr(out)))
+        return out
+
+    def queue_jobs(self, jobs: RasterJobs, scene, out: np.ndarray | None = None) -> np.ndarray:
+        """queueRasterJobs: every job through the Level-1 entry point, numbered
+        like the reference's reversed job list."""
+        out = np.zeros(scene.width * scene.height, np.uint32) if out is None else out
+        for j in range(jobs.n_jobs):
+            tiles, shapes = jobs.job(j)
+            self.raster_job(shapes, tiles, scene.background, scene.width, scene.height,
+                            0, jobs.n_jobs - 1 - j, out)
+        return out
+
+    def dump_job(self, shapes, tiles, width, height, job_index=0, cap=64):
+        n_tiles = tiles.size // 32
+        slots = n_tiles * self.spec.threads_per_tile
+        d = {"pre_thr": np.zeros((slots, cap, 4), np.float32), "pre_hdr": np.zeros((slots, cap), np.uint32),
+             "thr": np.zeros((slots, cap, 4), np.float32), "hdr": np.zeros((slots, cap), np.uint32),
+             "qlen": np.full(slots, -1, np.int32), "shape_bits": np.zeros(slots, np.uint32),
+             "stack": np.zeros((slots, 2), np.uint64),
+             "shape_indices": np.zeros((slots, self.spec.max_shapes), np.int32)}
+        dump = _lib.Dump(cap, *[d[k].ctypes.data for k in
+                                ("pre_thr", "pre_hdr", "thr", "hdr", "qlen", "shape_bits", "stack",
+                                 "shape_indices")])
+        shapes = np.ascontiguousarray(shapes, np.uint8)
+        tiles = np.ascontiguousarray(tiles, np.uint8)
+        self._check(self.L.gudni_dump_job(self.ctx, _ptr(shapes) if shapes.size else None,
+                                          shapes.size // 16, _ptr(tiles), n_tiles, width, height,
+                                          job_index, C.byref(dump)))
+        return d
+
+    # Level 2 ---------------------------------------------------------------
+    def frame_upload(self, jobs: RasterJobs, scene, frame=0):
+        bg = np.ascontiguousarray(scene.background, np.float32)
+        self._check(self.L.gudni_frame_upload(self.ctx, jobs.handle, scene.width, scene.height,
+                                              _ptr(bg), frame))
+
+    def frame_render(self, out_dev_ptr: int, out_f32_dev_ptr: int | None = None):
+        self._check(self.L.gudni_frame_render(self.ctx, C.c_void_p(out_dev_ptr),
+                                              C.c_void_p(out_f32_dev_ptr) if out_f32_dev_ptr else None))
+
+    def frame_check(self):
+        self._check(self.L.gudni_frame_check(self.ctx))
+
+    def enable_timing(self, on=True):
+        self._check(self.L.gudni_enable_timing(self.ctx, 1 if on else 0))
+
+    def last_timings(self):
+        ms = np.zeros(2, np.float32)
+        self._check(self.L.gudni_last_timings(self.ctx, _ptr(ms), 2))
+        return ms
+
+
+def prepare(scene, spec: RasterSpec | None = None):
+    """Host stages for a scene: (spec, Geometry, RasterJobs)."""
+    spec = spec or default_spec()
+    geo = Geometry(scene)
+    jobs = RasterJobs(spec, geo, scene.width, scene.height)
+    return spec, geo, jobs

@@ -1,0 +1,272 @@
+"""Seeded synthetic scenes for the raster path.
+
+Every scene is the flat form of what ``buildOverScene`` walks
+(Raster/Serialize.hs:148-177, 266-273): a sequence of shapes, each with its
+ShapeTag (Raster/ShapeInfo.hs:89-99) and closed outlines of curve pairs
+(Figure/Outline.hs:55-107), plus the substance table, picture data and
+picture usages the kernels read.
+
+Randomness is SplitMix64 (Steele, Lea & Flood 2014) so scenes are identical on
+every machine; floats are ``(u >> 40) * 2**-24``.  All geometry is FP32 pixel
+units.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+MASK64 = (1 << 64) - 1
+
+TAG_SOLID = 0x8000000000000000
+TAG_PICTURE = 0x4000000000000000
+TAG_ADD = 0x2000000000000000
+TAG_SUBTRACT = 0x3000000000000000
+TAG_ID_MASK = 0x0FFFFFFFFFFFFFFF
+
+SEEDS = {1: 0x67756431, 2: 0x67756432, 3: 0x67756433, 4: 0x67756434, 5: 0x67756435}
+
+
+class SplitMix64:
+    """Vectorised SplitMix64: ``next(n)`` returns n consecutive outputs."""
+
+    def __init__(self, seed: int):
+        self.state = np.uint64(seed & MASK64)
+
+    def next(self, n: int) -> np.ndarray:
+        with np.errstate(over="ignore"):
+            inc = np.uint64(0x9E3779B97F4A7C15)
+            z = self.state + inc * np.arange(1, n + 1, dtype=np.uint64)
+            self.state = np.uint64(self.state + inc * np.uint64(n))
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            return z ^ (z >> np.uint64(31))
+
+    def uniform(self, n: int) -> np.ndarray:
+        """Floats in [0, 1) with 24 random bits, as float64."""
+        return (self.next(n) >> np.uint64(40)).astype(np.float64) * (2.0 ** -24)
+
+    def integers(self, lo: int, hi: int, n: int) -> np.ndarray:
+        """Integers uniform over [lo, hi] (inclusive)."""
+        span = hi - lo + 1
+        return lo + (self.next(n) % np.uint64(span)).astype(np.int64)
+
+    def normal(self, n: int) -> np.ndarray:
+        """Box-Muller standard normals."""
+        u1 = self.uniform(n)
+        u2 = self.uniform(n)
+        return np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * math.pi * u2)
+
+
+@dataclasses.dataclass
+class Scene:
+    width: int
+    height: int
+    tags: np.ndarray                    # uint64 [n_shapes]
+    shape_outline_offsets: np.ndarray   # int32 [n_shapes + 1]
+    outline_pair_offsets: np.ndarray    # int32 [n_outlines + 1]
+    pairs: np.ndarray                   # float32 [n_pairs, 4]
+    substances: np.ndarray              # float32 [n_substances, 4] (picture: u32 id in .x bits)
+    pict_data: np.ndarray               # uint8 RGBA pile
+    pict_uses: np.ndarray               # uint8 [n_uses, 24] PictureUse records
+    background: tuple = (1.0, 1.0, 1.0, 1.0)
+    name: str = ""
+
+    @property
+    def n_shapes(self) -> int:
+        return int(self.tags.shape[0])
+
+    @property
+    def n_curves(self) -> int:
+        return int(self.pairs.shape[0])
+
+
+def pict_use_records(uses) -> np.ndarray:
+    """PictureUse records, 24 bytes each (Figure/Picture.hs:183-199):
+    float2 translate, int2 size, i32 memOffset, f32 scale."""
+    dt = np.dtype([("tx", "<f4"), ("ty", "<f4"), ("w", "<i4"), ("h", "<i4"),
+                   ("off", "<i4"), ("scale", "<f4")])
+    arr = np.zeros(len(uses), dtype=dt)
+    for i, (tx, ty, w, h, off, scale) in enumerate(uses):
+        arr[i] = (tx, ty, w, h, off, scale)
+    return arr.view(np.uint8).reshape(len(uses), 24) if len(uses) else np.zeros((0, 24), np.uint8)
+
+
+def picture_substance(pict_id: int) -> np.ndarray:
+    """Substance record of a picture (ShapeInfo.hs:144-148): u32 id, u32 0,
+    8 undefined bytes (zero here)."""
+    rec = np.zeros(4, dtype=np.uint32)
+    rec[0] = pict_id
+    return rec.view(np.float32)
+
+
+def build_scene(width, height, shapes, substances, pict_data=None, pict_uses=(),
+                background=(1.0, 1.0, 1.0, 1.0), name=""):
+    """shapes: list of (tag, [outline, ...]) with outline an (k, 4) array of
+    curve pairs (on.x, on.y, off.x, off.y)."""
+    tags = np.array([t for t, _ in shapes], dtype=np.uint64)
+    so = [0]
+    oo = [0]
+    chunks = []
+    for _, outlines in shapes:
+        for o in outlines:
+            o = np.asarray(o, dtype=np.float32).reshape(-1, 4)
+            chunks.append(o)
+            oo.append(oo[-1] + o.shape[0])
+        so.append(len(oo) - 1)
+    pairs = np.concatenate(chunks) if chunks else np.zeros((0, 4), np.float32)
+    subs = np.asarray(substances, dtype=np.float32).reshape(-1, 4)
+    pd = np.zeros(4, np.uint8) if pict_data is None else np.asarray(pict_data, np.uint8).ravel()
+    return Scene(width, height, tags, np.array(so, np.int32), np.array(oo, np.int32),
+                 np.ascontiguousarray(pairs), np.ascontiguousarray(subs), pd,
+                 pict_use_records(list(pict_uses)), tuple(background), name)
+
+
+def straight_pairs(points) -> np.ndarray:
+    """Closed polygon -> curve pairs; each straight segment's control point is
+    its midpoint (segmentsToCurvePairs, Outline.hs:91-103, with linear's
+    lerp 0.5 = 0.5*a + 0.5*b)."""
+    p = np.asarray(points, dtype=np.float32).reshape(-1, 2)
+    q = np.roll(p, -1, axis=0)
+    mid = (np.float32(0.5) * p + np.float32(0.5) * q).astype(np.float32)
+    return np.concatenate([p, mid], axis=1).astype(np.float32)
+
+
+def rectangle_pairs(x0, y0, x1, y1) -> np.ndarray:
+    """rectangleCurve (Layout/Draw.hs:71-78) translated to (x0, y0)."""
+    return straight_pairs([(x0, y0), (x1, y0), (x1, y1), (x0, y1)])
+
+
+def solid_tag(sid: int, subtract: bool = False) -> int:
+    return TAG_SOLID | (TAG_SUBTRACT if subtract else TAG_ADD) | (sid & TAG_ID_MASK)
+
+
+def picture_tag(sid: int) -> int:
+    return TAG_PICTURE | TAG_ADD | (sid & TAG_ID_MASK)
+
+
+def _assemble(width, height, tags, k, pairs, substances, pict_data, pict_uses, name,
+              background=(1.0, 1.0, 1.0, 1.0)):
+    """One outline per shape with k[i] pairs each."""
+    n = len(k)
+    oo = np.zeros(n + 1, np.int32)
+    oo[1:] = np.cumsum(k)
+    so = np.arange(n + 1, dtype=np.int32)
+    return Scene(width, height, np.asarray(tags, np.uint64), so, oo,
+                 np.ascontiguousarray(pairs, dtype=np.float32),
+                 np.ascontiguousarray(substances, dtype=np.float32),
+                 pict_data, pict_uses, background, name)
+
+
+def random_colors(rng: SplitMix64, n: int) -> np.ndarray:
+    """Random RGBA8 colours as floats byte/255 (Figure/Color.hs stores floats)."""
+    b = (rng.next(n * 4) & np.uint64(0xFF)).astype(np.float32).reshape(n, 4)
+    return (b / np.float32(255.0)).astype(np.float32)
+
+
+def config1(n_shapes: int = 1000, size: int = 512, seed: int = SEEDS[1]) -> Scene:
+    """512x512, 1000 closed shapes of 3-12 quadratics whose on- and off-curve
+    points are uniform over the canvas; solid random RGBA8 colours.  The
+    reference fills even-odd only (Kernels.cl:298-304), so every shape here is
+    even-odd."""
+    rng = SplitMix64(seed)
+    k = rng.integers(3, 12, n_shapes)
+    total = int(k.sum())
+    pts = (rng.uniform(total * 4) * size).astype(np.float32).reshape(total, 4)
+    colors = random_colors(rng, n_shapes)
+    tags = [solid_tag(i) for i in range(n_shapes)]
+    return _assemble(size, size, tags, k, pts, colors, np.zeros(4, np.uint8),
+                     np.zeros((0, 24), np.uint8), "cfg1")
+
+
+def noise_texture(rng: SplitMix64, side: int = 1024) -> np.ndarray:
+    return (rng.next(side * side) & np.uint64(0xFFFFFFFF)).astype(np.uint32).view(np.uint8)
+
+
+def _blob_pairs(rng, cx, cy, r, k):
+    """Closed blobs: k on-curve points at angles 2*pi*j/k, radius
+    r*(1 + 0.25*U[-1,1]); controls at the half angles, pushed out by
+    1/cos(pi/k) so the outline bulges like a circle."""
+    total = int(k.sum())
+    owner = np.repeat(np.arange(len(k)), k)
+    start = np.concatenate([[0], np.cumsum(k)[:-1]])
+    j = np.arange(total) - np.repeat(start, k)
+    kk = np.repeat(k, k).astype(np.float64)
+    ang = 2.0 * math.pi * j / kk
+    rr = np.repeat(r, k) * (1.0 + 0.25 * (2.0 * rng.uniform(total) - 1.0))
+    rc = np.repeat(r, k) * (1.0 + 0.25 * (2.0 * rng.uniform(total) - 1.0)) / np.cos(math.pi / kk)
+    angc = ang + math.pi / kk
+    x = np.repeat(cx, k) + rr * np.cos(ang)
+    y = np.repeat(cy, k) + rr * np.sin(ang)
+    xc = np.repeat(cx, k) + rc * np.cos(angc)
+    yc = np.repeat(cy, k) + rc * np.sin(angc)
+    return np.stack([x, y, xc, yc], axis=1).astype(np.float32), owner
+
+
+def config2(n_blobs: int = 180_000, n_glyphs: int = 20_000, size: int = 4096,
+            seed: int = SEEDS[2], texture_side: int = 1024) -> Scene:
+    """4096x4096, 200k shapes, ~2.1M quadratics.  Blobs: centre uniform,
+    radius 8*exp(1.2*N(0,1)) (median 8 px), 4-16 quadratics.  Glyph-like
+    outlines: 4-30 random quadratics inside a 12-48 px em box.  Substances:
+    90% solid random RGBA8, 10% a seeded 1024^2 RGBA8 noise texture sampled
+    nearest (the reference's only picture mode).  The gradient share of the
+    workload is drawn as solid (first stop): the reference has no gradient
+    substance."""
+    rng = SplitMix64(seed)
+    cx = rng.uniform(n_blobs) * size
+    cy = rng.uniform(n_blobs) * size
+    r = 8.0 * np.exp(1.2 * rng.normal(n_blobs))
+    kb = rng.integers(4, 16, n_blobs)
+    blob_pairs, _ = _blob_pairs(rng, cx, cy, r, kb)
+    kg = rng.integers(4, 30, n_glyphs)
+    em = 12.0 + rng.uniform(n_glyphs) * 36.0
+    gx = rng.uniform(n_glyphs) * size
+    gy = rng.uniform(n_glyphs) * size
+    tg = int(kg.sum())
+    u = rng.uniform(tg * 4).reshape(tg, 4)
+    gem = np.repeat(em, kg)
+    glyph_pairs = np.stack([np.repeat(gx, kg) + u[:, 0] * gem, np.repeat(gy, kg) + u[:, 1] * gem,
+                            np.repeat(gx, kg) + u[:, 2] * gem, np.repeat(gy, kg) + u[:, 3] * gem],
+                           axis=1).astype(np.float32)
+    # seeded shuffle of whole shapes so blobs and glyphs interleave
+    k_all = np.concatenate([kb, kg])
+    pairs_all = np.concatenate([blob_pairs, glyph_pairs])
+    start_all = np.concatenate([[0], np.cumsum(k_all)[:-1]])
+    n = n_blobs + n_glyphs
+    perm = np.argsort(rng.next(n), kind="stable")
+    k = k_all[perm]
+    first = np.concatenate([[0], np.cumsum(k)[:-1]])
+    within = np.arange(int(k.sum())) - np.repeat(first, k)
+    pairs = pairs_all[np.repeat(start_all[perm], k) + within]
+    # substances
+    kind = rng.next(n) % np.uint64(10)
+    picture = kind == np.uint64(9)
+    colors = random_colors(rng, n)
+    tex = noise_texture(rng, texture_side)
+    subs = colors.copy()
+    uses = []
+    tags = np.empty(n, np.uint64)
+    pic_idx = np.nonzero(picture)[0]
+    pid = np.full(n, -1, np.int64)
+    pid[pic_idx] = np.arange(len(pic_idx))
+    ids = np.arange(n, dtype=np.uint64)
+    tags[:] = np.uint64(TAG_SOLID | TAG_ADD) | ids
+    tags[picture] = np.uint64(TAG_PICTURE | TAG_ADD) | ids[picture]
+    sub_u32 = subs.view(np.uint32)
+    sub_u32[picture, 0] = pid[picture].astype(np.uint32)
+    sub_u32[picture, 1:] = 0
+    # each picture usage maps the texture's origin onto a random point of the canvas
+    ox = rng.uniform(len(pic_idx)) * size
+    oy = rng.uniform(len(pic_idx)) * size
+    for j in range(len(pic_idx)):
+        uses.append((-float(np.float32(ox[j])), -float(np.float32(oy[j])), texture_side,
+                     texture_side, 0, 1.0))
+    return _assemble(size, size, tags, k, pairs, subs, tex, pict_use_records(uses), "cfg2")
+
+
+def scaled_config2(size: int, n_shapes: int, seed: int, glyph_share: float = 0.1) -> Scene:
+    """The config-2 generator at another canvas size and shape count."""
+    n_g = int(round(n_shapes * glyph_share))
+    return config2(n_blobs=n_shapes - n_g, n_glyphs=max(n_g, 1), size=size, seed=seed,
+                   texture_side=256)

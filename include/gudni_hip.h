@@ -1,0 +1,200 @@
+/*
+ * gudni_hip.h -- C ABI of libgudni_hip, the MI355X (gfx950) raster path of
+ * Gudni.  Plain C99: opaque handles, plain pointers and sizes, int status
+ * codes (0 = ok, < 0 = error, message in gudni_last_error), no exceptions
+ * across the boundary.  One gudni_ctx per thread; a ctx is not thread-safe.
+ *
+ * Record formats are the reference's own Storable layouts, so Haskell encoders
+ * feed this library unchanged:
+ *   strand heap    Raster/Strand.hs:180-206   (16*(m+1) bytes per strand)
+ *   Shape          Raster/Types.hs:159-173    (u64 tag, u32 geoStart in
+ *                                              16-byte units, u32 numStrands)
+ *   TileInfo       Raster/Types.hs:176-198    (int4 box, i16 hDepth,
+ *                                              i16 vDepth, i32 columnAllocation,
+ *                                              i32 shapeStart, i32 shapeCount)
+ *   Substance      Raster/ShapeInfo.hs:136-148 (float4 colour | u32 pictId)
+ *   PictureUse     Figure/Picture.hs:183-199  (float2 translate, int2 size,
+ *                                              i32 memOffset, f32 scale)
+ * Output pixels are the reference's BGRA8 words (Kernels.cl:842-844).
+ */
+#ifndef GUDNI_HIP_H
+#define GUDNI_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GUDNI_OK              0
+#define GUDNI_ERR_ARG        -1   /* bad argument / size                      */
+#define GUDNI_ERR_HIP        -2   /* HIP runtime error                        */
+#define GUDNI_ERR_OVERFLOW   -3   /* a column's threshold queue exceeded
+                                     maxThresholds (Kernels.cl:375-380 would
+                                     corrupt the neighbouring column)         */
+#define GUDNI_ERR_GUARD      -4   /* a column exceeded the section guard      */
+#define GUDNI_ERR_NO_DEVICE  -5
+
+/* RasterSpec (OpenCL/Rasterizer.hs:37-51) with the values determineRasterSpec
+ * (OpenCL/Setup.hs:71-87) yields for a device reporting a maximum work-group
+ * size of 256 and a 4 GiB maximum allocation. */
+typedef struct gudni_spec {
+    int32_t max_tile_size;        /* specMaxTileSize      = 256               */
+    int32_t threads_per_tile;     /* specThreadsPerTile   = 256               */
+    int32_t max_tiles_per_call;   /* specMaxTilesPerCall  = 256               */
+    int32_t max_thresholds;       /* specMaxThresholds    = 4096              */
+    int32_t max_strands_per_tile; /* specMaxStrandsPerTile = 4094             */
+    int32_t max_shapes;           /* specMaxShapes        = 127               */
+    int32_t random_field_size;    /* RANDOMFIELDSIZE      = 4096              */
+    float   stochastic_factor;    /* STOCHASTIC_FACTOR    = 0.0               */
+    int32_t coverage_mode;        /* GUDNI_COVERAGE_ACCURATE (default) or
+                                     GUDNI_COVERAGE_REFERENCE                 */
+} gudni_spec;
+
+/* Section mid-x evaluation (thresholdMidXLow, Kernels.cl:916-926).  Records,
+ * their order and the section structure are identical in both modes.
+ * REFERENCE evaluates yMid = y0 + (y1 - y0)/2 and then x(yMid) exactly as the
+ * reference writes it; in tall column segments yMid's rounding (ulp(200) =
+ * 1.5e-5) times a shallow edge's inverse slope costs up to ~1e-5 of
+ * coverage.  ACCURATE evaluates the same point as
+ * ((y0 - start.y) + (y1 - y0)/2) * invSlope + left, whose intermediates are
+ * bounded by the threshold's own height (coverage within ~1e-6 of an FP64
+ * integration of the same sections). */
+#define GUDNI_COVERAGE_ACCURATE  0
+#define GUDNI_COVERAGE_REFERENCE 1
+
+typedef struct gudni_ctx gudni_ctx;
+
+/* Fills the pinned default spec above. */
+void        gudni_default_spec(gudni_spec *spec);
+/* Replaces setupOpenCL (OpenCL/Setup.hs:102-147): selects HIP device
+ * `device`, records the spec; kernels are compiled ahead of time. */
+int         gudni_create(const gudni_spec *spec, int device, gudni_ctx **out);
+void        gudni_destroy(gudni_ctx *ctx);
+const char *gudni_last_error(const gudni_ctx *ctx);
+/* Launch stream (a hipStream_t); NULL = the null stream. */
+int         gudni_set_stream(gudni_ctx *ctx, void *hip_stream);
+
+/* ---- Level 1: byte-compatible with generateCall -------------------------- */
+/* Replaces the common-buffer upload in queueRasterJobs
+ * (OpenCL/CallKernels.hs:224-240): geometry pile, substance pile, picture
+ * data, picture usages and the random field.  Copies to device memory. */
+int gudni_upload_common(gudni_ctx *ctx,
+                        const void *geo, size_t geo_bytes,
+                        const void *substances, int32_t n_substances,
+                        const void *pict_data, size_t pict_bytes,
+                        const void *pict_uses, int32_t n_pict_uses,
+                        const float *random_field, int32_t n_random);
+
+/* Replaces generateCall (OpenCL/CallKernels.hs:88-179): runs the
+ * generateThresholds / sortThresholds / renderThresholds work of one
+ * RasterJob (Raster/Job.hs:72-77) and writes the job's pixels into `out`
+ * (host memory, width*height words; other pixels are left untouched).
+ * Synchronous, like the reference's KernelSync launches. */
+int gudni_raster_job(gudni_ctx *ctx,
+                     const void *shapes, int32_t n_shapes,
+                     const void *tiles, int32_t n_tiles,
+                     const float background[4],
+                     int32_t width, int32_t height,
+                     int32_t frame, int32_t job_index,
+                     uint32_t *out);
+
+/* ---- Host scene preparation (the Haskell host stages) -------------------- */
+/* A scene as the sequence of onShape calls buildOverScene makes
+ * (Raster/Serialize.hs:148-177, 266-273): per shape its ShapeTag and its
+ * transformed outlines, each a closed loop of curve pairs
+ * (Figure/Outline.hs:55-107; straight segments carry their midpoint). */
+typedef struct gudni_outlines {
+    int32_t         n_shapes;
+    const uint64_t *tags;                  /* [n_shapes]                      */
+    const int32_t  *shape_outline_offsets; /* [n_shapes+1]                    */
+    const int32_t  *outline_pair_offsets;  /* [n_outlines+1]                  */
+    const float    *pairs;                 /* [n_pairs][4] on.x on.y off.x off.y */
+} gudni_outlines;
+
+/* Serialised geometry: what onShape leaves in the geometry pile plus the
+ * ShapeEntry fields (Raster/Types.hs:140-145) addShapeToTree reads. */
+typedef struct gudni_geometry_view {
+    const uint8_t  *geo;      size_t geo_bytes;
+    const uint8_t  *shapes;   /* 16-byte Shape records, kept shapes only      */
+    const float    *bbox;     /* [n][4] left top right bottom                 */
+    const uint32_t *strand_count;
+    const int32_t  *source_index;          /* index in gudni_outlines         */
+    int32_t         n_shapes;
+} gudni_geometry_view;
+
+typedef struct gudni_geometry gudni_geometry;
+/* Replaces onShape's serialisation (Serialize.hs:148-177, Enclosure.hs:62-73,
+ * Strand.hs:153-206): culls off-canvas shapes (excludeBox, Serialize.hs:97-104)
+ * and writes strands. */
+int  gudni_geometry_build(const gudni_outlines *scene, int32_t canvas_w,
+                          int32_t canvas_h, gudni_geometry **out);
+int  gudni_geometry_view_get(const gudni_geometry *g, gudni_geometry_view *v);
+void gudni_geometry_free(gudni_geometry *g);
+
+/* Raster jobs: the tile tree (Raster/TileTree.hs:81-204) flattened into jobs
+ * of <= max_tiles_per_call tiles (Raster/Job.hs:132-179), in the order
+ * buildRasterJobs traverses them (jobs are listed in traversal order; the
+ * reference's list is the reverse, CallKernels.hs:255). */
+typedef struct gudni_jobs_view {
+    int32_t         n_jobs;
+    const int32_t  *job_tile_offset;   /* [n_jobs+1]                          */
+    const int32_t  *job_shape_offset;  /* [n_jobs+1]                          */
+    const uint8_t  *tiles;             /* 32-byte TileInfo records            */
+    const uint8_t  *shape_refs;        /* 16-byte Shape records               */
+    int32_t         n_tiles, n_refs;
+} gudni_jobs_view;
+
+typedef struct gudni_jobs gudni_jobs;
+/* Replaces buildTileTree + addShapeToTree + buildRasterJobs
+ * (TileTree.hs:81-190, CallKernels.hs:244-255). */
+int  gudni_jobs_build(const gudni_spec *spec, const gudni_geometry *g,
+                      int32_t canvas_w, int32_t canvas_h, gudni_jobs **out);
+int  gudni_jobs_view_get(const gudni_jobs *j, gudni_jobs_view *v);
+void gudni_jobs_free(gudni_jobs *j);
+
+/* ---- Level 2: whole frame with every buffer resident in HBM --------------- */
+/* Replaces queueRasterJobs (CallKernels.hs:218-242) for a whole frame: the
+ * jobs' tiles and shape references are uploaded once, then every frame render
+ * runs all tiles of all jobs from device memory in one persistent launch. */
+int gudni_frame_upload(gudni_ctx *ctx, const gudni_jobs *jobs,
+                       int32_t width, int32_t height,
+                       const float background[4], int32_t frame);
+/* Renders the uploaded frame into device memory (stream-ordered, returns
+ * without waiting).  out_dev: width*height BGRA words; out_f32_dev (may be
+ * NULL): width*height*4 floats, the pre-quantisation RGBA of each pixel. */
+int gudni_frame_render(gudni_ctx *ctx, uint32_t *out_dev, float *out_f32_dev);
+/* Waits for the stream and reports a queue overflow or guard hit seen by the
+ * last render. */
+int gudni_frame_check(gudni_ctx *ctx);
+/* Pass timings (ms) of the last render, recorded with hipEvents on the
+ * launch stream when enabled and read by gudni_frame_check: [0] counter
+ * reset + fast pass, [1] retry pass. */
+int gudni_enable_timing(gudni_ctx *ctx, int enable);
+int gudni_last_timings(gudni_ctx *ctx, float *ms, int32_t n);
+
+/* Test hook: per-column threshold records after generate (queue order) and
+ * after sort, plus the initial shape state, for one RasterJob.  Slot =
+ * tile_in_job * threads_per_tile + column; `cap` records are kept per slot,
+ * front first.  All arrays are host memory sized by the caller. */
+typedef struct gudni_dump {
+    int32_t   cap;
+    float    *pre_thresholds;   /* [slots][cap][4]                            */
+    uint32_t *pre_headers;      /* [slots][cap]                               */
+    float    *thresholds;       /* [slots][cap][4]                            */
+    uint32_t *headers;          /* [slots][cap]                               */
+    int32_t  *queue_len;        /* [slots] (-1: inactive column)              */
+    uint32_t *shape_bits;       /* [slots]                                    */
+    uint64_t *shape_stack;      /* [slots][2]                                 */
+    int32_t  *shape_indices;    /* [slots][max_shapes]                        */
+} gudni_dump;
+int gudni_dump_job(gudni_ctx *ctx,
+                   const void *shapes, int32_t n_shapes,
+                   const void *tiles, int32_t n_tiles,
+                   int32_t width, int32_t height, int32_t job_index,
+                   gudni_dump *dump);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

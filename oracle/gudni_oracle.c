@@ -1,0 +1,1308 @@
+/*
+ * gudni_oracle.c -- CPU restatement of Gudni's raster path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg load this library, and only as the checker.
+ * The gudni_amd product never links, loads or calls it.
+ *
+ * Built with gcc -O2 -ffp-contract=off (IEEE binary32 via SSE, no FMA
+ * contraction, correctly rounded division), so every FP32 operation below
+ * happens in the order written -- the "strict" reading of Kernels.cl, which
+ * the reference itself compiles with -cl-fast-relaxed-math (Setup.hs:124).
+ *
+ * min/max follow the OpenCL definitions literally: max(x,y) = x < y ? y : x,
+ * min(x,y) = y < x ? y : x (OpenCL 1.2 s6.12.4), and fmin/fmax are taken the
+ * same way (they agree on every non-NaN input apart from the sign of zero).
+ *
+ * Pinning status of each part (how the tests check this file):
+ *   reorder table ........ closed form: equals the Eytzinger in-order fill
+ *                          (tests/test_oracle_host.py)
+ *   knob split ........... split parameter within 2e-4 of the analytic
+ *                          vertical-tangent parameter; no knobs remain
+ *   strands .............. invariants + in-order decode round trip
+ *   tile tree / jobs ..... agrees byte-for-byte with a second, independent
+ *                          formulation (recursive partition by counting) in
+ *                          the product host code, plus partition invariants
+ *   generate/sort/render . convex polygons with integer vertex x give the
+ *                          exact FP64 polygon-pixel area (Sutherland-Hodgman),
+ *                          axis-aligned rectangles give closed-form pixels,
+ *                          stacked translucent rectangles give closed-form
+ *                          Porter-Duff colours, the sorted queue equals
+ *                          numpy's stable lexsort of the unsorted one.
+ *   quadratic flattening . bounded against the exact Bezier area; the
+ *                          bit-exact FP32 trajectory is parity unpinned
+ *                          beyond that (no runnable reference here).
+ */
+#include "gudni_oracle.h"
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+
+/* ------------------------------------------------------------------------ */
+/* constants (Kernels.cl:42-250, Constants.hs:47-89)                          */
+/* ------------------------------------------------------------------------ */
+#define O_POSITIVE_SLOPE   0x80000000u
+#define O_PERSIST          0x40000000u
+#define O_PERSIST_AND_SLOPE 0xC0000000u
+#define O_PERSIST_TOP      0xC0000000u
+#define O_PERSIST_BOTTOM   0x40000000u
+#define O_UNPERSIST        0xBFFFFFFFu
+#define O_SHAPEBIT_MASK    0x0FFFFFFFu
+#define O_MINCROP          0.2f
+#define O_FLATNESS         0.25f
+#define O_STACK_SECTIONS   8
+/* A column whose section sweep runs longer than this is reported as an
+ * error instead of spinning; the GPU kernels use the same bound. */
+#define GO_SECTION_GUARD   (1L << 22)
+#define GO_MAX_BISECT      64
+
+#define TAG_SUBSTANCETYPE_MASK 0xC000000000000000ull
+#define TAG_SOLID              0x8000000000000000ull
+#define TAG_COMPOUND_MASK      0x3000000000000000ull
+#define TAG_CONTINUE           0x1000000000000000ull
+#define TAG_ADD                0x2000000000000000ull
+#define TAG_SUBSTANCEID_MASK   0x0FFFFFFFFFFFFFFFull
+
+static inline float cl_max(float x, float y) { return x < y ? y : x; }
+static inline float cl_min(float x, float y) { return y < x ? y : x; }
+static inline int   imax(int a, int b) { return a > b ? a : b; }
+static inline int   imin(int a, int b) { return a < b ? a : b; }
+
+void go_default_spec(go_spec *s)
+{
+    s->max_thresholds = 4096;
+    s->max_shapes = 127;
+    s->compute_depth = 8;
+    s->threads_per_tile = 256;
+    s->tiles_per_call = 256;
+    s->max_tile_size = 256;
+    s->max_strands_per_tile = 4094;
+    s->random_field_size = 4096;
+    s->stochastic_factor = 0.0f;
+    s->coverage_mode = 0;
+}
+
+/* ======================================================================== */
+/* Reorder table (ReorderTable.hs:44-111)                                     */
+/* ======================================================================== */
+/* perfectTreePartition (ReorderTable.hs:44-56) */
+static int tree_partition(int n)
+{
+    int x = 1;
+    while (x <= n / 2) x *= 2;
+    return ((x / 2) - 1 <= n - x) ? x - 1 : n - (x / 2);
+}
+
+/* buildITree + breadth: BFS order of the partition tree over [lo, lo+n). */
+static int bfs_partition(int n, int *out)
+{
+    /* queue of (lo, len) ranges */
+    int qlo[64], qn[64];
+    int head = 0, tail = 0, k = 0;
+    if (n <= 0) return 0;
+    qlo[tail] = 0; qn[tail] = n; tail++;
+    while (head < tail) {
+        int lo = qlo[head], len = qn[head]; head++;
+        if (len <= 0) continue;
+        int half = tree_partition(len);
+        out[k++] = lo + half;
+        qlo[tail] = lo;            qn[tail] = half;           tail++;
+        qlo[tail] = lo + half + 1; qn[tail] = len - half - 1; tail++;
+    }
+    return k;
+}
+
+/* reorderForExtents maxSize size, entry i (ReorderTable.hs:92-99). */
+int go_reorder_entry(int size, int i)
+{
+    if (size < 3 || i < 0 || i >= size) return -1;
+    if (i == 0) return size - 1;
+    if (i == 1) return 0;
+    if (i == 2) return 1;
+    {
+        int inner = size - 3;              /* makeTreeRow (size - 3) */
+        int half_tree[32];
+        int nodes = bfs_partition(inner / 2, half_tree);
+        int j = i - 3;
+        (void)nodes;
+        return 2 * half_tree[j / 2] + (j & 1) + 2;
+    }
+}
+
+/* ======================================================================== */
+/* Knob removal (Deknob.hs:40-108)                                            */
+/* ======================================================================== */
+typedef struct { float x, y; } v2;
+typedef struct { v2 a, c, b; } bez_t;   /* Bez v0 control v1 */
+
+static inline v2 between(float t, v2 p, v2 q)
+{   /* (p ^* (1-t)) ^+^ (q ^* t) */
+    float u = 1.0f - t;
+    v2 r; r.x = p.x * u + q.x * t; r.y = p.y * u + q.y * t; return r;
+}
+
+/* findSplit (Deknob.hs:61-79).  left != 0: staysRelativeTo = isLeftOf. */
+static void find_split(int left, bez_t z, v2 *mid0, v2 *on, v2 *mid1)
+{
+    float bottom = 0.0f, top = 1.0f, t = 0.5f;
+    for (;;) {
+        v2 m0 = between(t, z.a, z.c);
+        v2 m1 = between(t, z.c, z.b);
+        v2 oc = between(t, m0, m1);
+        int m1_stays = left ? (m1.x < oc.x) : (m1.x > oc.x);
+        int m0_stays = left ? (m0.x < oc.x) : (m0.x > oc.x);
+        if (top - bottom <= 0.0001f) { *mid0 = m0; *on = oc; *mid1 = m1; return; }
+        if (m1_stays) {
+            float nt = t + ((top - t) / 2.0f);
+            bottom = t; t = nt;
+        } else if (m0_stays) {
+            float nt = bottom + ((t - bottom) / 2.0f);
+            top = t; t = nt;
+        } else { *mid0 = m0; *on = oc; *mid1 = m1; return; }
+    }
+}
+
+/* fixKnob (Deknob.hs:85-102). */
+static int fix_knob(bez_t z, bez_t *out)
+{
+    int left  = (z.c.x < z.a.x) && (z.c.x < z.b.x);
+    int right = (z.c.x > z.a.x) && (z.c.x > z.b.x);
+    if (left || right) {
+        v2 m0, oc, m1;
+        find_split(left, z, &m0, &oc, &m1);
+        out[0].a = z.a; out[0].c = m0; out[0].b = oc;
+        out[1].a = oc;  out[1].c = m1; out[1].b = z.b;
+        return 2;
+    }
+    out[0] = z;
+    return 1;
+}
+
+int go_fix_knob(const float *b6, float *o12)
+{
+    bez_t z, o[2];
+    z.a.x = b6[0]; z.a.y = b6[1]; z.c.x = b6[2]; z.c.y = b6[3]; z.b.x = b6[4]; z.b.y = b6[5];
+    int n = fix_knob(z, o);
+    for (int i = 0; i < n; i++) {
+        o12[6*i+0] = o[i].a.x; o12[6*i+1] = o[i].a.y;
+        o12[6*i+2] = o[i].c.x; o12[6*i+3] = o[i].c.y;
+        o12[6*i+4] = o[i].b.x; o12[6*i+5] = o[i].b.y;
+    }
+    return n;
+}
+
+/* ======================================================================== */
+/* Strands (Strand.hs:77-206)                                                 */
+/* ======================================================================== */
+static int hcompare(const bez_t *z)
+{   /* compareHorizontal: compare a.x b.x */
+    return (z->a.x < z->b.x) ? -1 : (z->a.x > z->b.x) ? 1 : 0;
+}
+
+/* write one strand of m curves (already oriented) at out; returns bytes */
+static long write_strand(const bez_t *zs, int m, uint8_t *out, long cap)
+{
+    int npts = 2 * m + 1;
+    long bytes = 8 + 8L * npts;
+    v2 pts[33];
+    if (bytes > cap) return -1;
+    for (int i = 0; i < m; i++) { pts[2*i] = zs[i].a; pts[2*i+1] = zs[i].c; }
+    pts[2*m] = zs[m-1].b;
+    {
+        uint16_t size = (uint16_t)(bytes / 8);
+        uint16_t z16 = 0; uint32_t z32 = 0;
+        memcpy(out, &size, 2); memcpy(out + 2, &z16, 2); memcpy(out + 4, &z32, 4);
+    }
+    for (int i = 0; i < npts; i++) {
+        v2 p = pts[go_reorder_entry(npts, i)];
+        memcpy(out + 8 + 8 * i, &p.x, 4);
+        memcpy(out + 12 + 8 * i, &p.y, 4);
+    }
+    return bytes;
+}
+
+/* splitShape for one outline (Strand.hs:153-178). */
+static long outline_to_strands(const float *pairs, int k, uint8_t *out, long cap, int32_t *nstr)
+{
+    long used = 0;
+    if (k < 2) return 0;
+    bez_t *beziers = (bez_t *)malloc(sizeof(bez_t) * (size_t)(2 * k));
+    int nb = 0;
+    for (int i = 0; i < k; i++) {           /* pairsToBeziers + replaceKnobs */
+        int j = (i + 1) % k;
+        bez_t z;
+        z.a.x = pairs[4*i+0]; z.a.y = pairs[4*i+1];
+        z.c.x = pairs[4*i+2]; z.c.y = pairs[4*i+3];
+        z.b.x = pairs[4*j+0]; z.b.y = pairs[4*j+1];
+        nb += fix_knob(z, beziers + nb);
+    }
+    /* splitIntoStrands: runs of connectable curves; the final run first. */
+    int *run_start = (int *)malloc(sizeof(int) * (size_t)(nb + 1));
+    int nruns = 0;
+    run_start[nruns++] = 0;
+    for (int i = 1; i < nb; i++) {
+        int ha = hcompare(&beziers[i-1]), hb = hcompare(&beziers[i]);
+        if (!(ha == hb && ha != 0)) run_start[nruns++] = i;
+    }
+    run_start[nruns] = nb;
+    for (int rr = 0; rr < nruns; rr++) {
+        int r = (rr == 0) ? nruns - 1 : rr - 1;   /* acc `cons` strands */
+        int s = run_start[r], e = run_start[r+1];
+        for (int c = s; c < e; c += 16) {         /* splitTooLarge 16 */
+            int m = imin(16, e - c);
+            bez_t zs[16];
+            memcpy(zs, beziers + c, sizeof(bez_t) * (size_t)m);
+            if (zs[0].a.x > zs[m-1].b.x) {         /* reverseIfBackwards */
+                bez_t tmp[16];
+                for (int q = 0; q < m; q++) {
+                    tmp[q].a = zs[m-1-q].b; tmp[q].c = zs[m-1-q].c; tmp[q].b = zs[m-1-q].a;
+                }
+                memcpy(zs, tmp, sizeof(bez_t) * (size_t)m);
+            }
+            long w = write_strand(zs, m, out + used, cap - used);
+            if (w < 0) { free(beziers); free(run_start); return -1; }
+            used += w; (*nstr)++;
+        }
+    }
+    free(beziers); free(run_start);
+    return used;
+}
+
+long go_serialize_outlines(const float *pairs, const int32_t *ooff, int32_t n_outlines,
+                           uint8_t *out, long cap, int32_t *n_strands)
+{
+    long used = 0;
+    *n_strands = 0;
+    for (int o = 0; o < n_outlines; o++) {
+        long w = outline_to_strands(pairs + 4L * ooff[o], ooff[o+1] - ooff[o],
+                                    out + used, cap - used, n_strands);
+        if (w < 0) return -1;
+        used += w;
+    }
+    return used;
+}
+
+int go_serialize_scene(int32_t n_shapes, const uint64_t *tags, const int32_t *soff,
+                       const int32_t *ooff, const float *pairs, float cw, float ch,
+                       go_serialized *s)
+{
+    long cap = 0;
+    for (int i = 0; i < n_shapes; i++) {
+        int np = ooff[soff[i+1]] - ooff[soff[i]];
+        cap += 64L * np + 16;   /* <= 2 curves per pair, <= 1 strand per curve */
+    }
+    memset(s, 0, sizeof(*s));
+    s->geo = (uint8_t *)malloc((size_t)cap + 16);
+    s->shapes = (uint8_t *)malloc(16u * (size_t)(n_shapes + 1));
+    s->bbox = (float *)malloc(16u * (size_t)(n_shapes + 1));
+    s->strand_count = (uint32_t *)malloc(4u * (size_t)(n_shapes + 1));
+    s->source_index = (int32_t *)malloc(4u * (size_t)(n_shapes + 1));
+    long used = 0;
+    int kept = 0;
+    for (int i = 0; i < n_shapes; i++) {
+        int p0 = ooff[soff[i]], p1 = ooff[soff[i+1]];
+        if (p1 <= p0) continue;                  /* no points: nothing to bin */
+        float l = pairs[4*p0], t = pairs[4*p0+1], r = l, b = t;
+        for (int p = p0; p < p1; p++) {          /* boxOf over on+off points */
+            for (int h = 0; h < 2; h++) {
+                float x = pairs[4*p + 2*h], y = pairs[4*p + 2*h + 1];
+                l = (l <= x) ? l : x;  r = (r <= x) ? x : r;
+                t = (t <= y) ? t : y;  b = (b <= y) ? y : b;
+            }
+        }
+        /* excludeBox (Serialize.hs:97-104) */
+        if (l >= cw || t >= ch || r <= 0.0f || b <= 0.0f) continue;
+        int32_t ns = 0;
+        long w = go_serialize_outlines(pairs, ooff + soff[i], soff[i+1] - soff[i],
+                                       s->geo + used, cap - used, &ns);
+        if (w < 0) return -1;
+        uint32_t start = (uint32_t)(used / 16);  /* GeoRef in 16-byte units */
+        memcpy(s->shapes + 16 * kept, &tags[i], 8);
+        memcpy(s->shapes + 16 * kept + 8, &start, 4);
+        memcpy(s->shapes + 16 * kept + 12, &ns, 4);
+        s->bbox[4*kept+0] = l; s->bbox[4*kept+1] = t;
+        s->bbox[4*kept+2] = r; s->bbox[4*kept+3] = b;
+        s->strand_count[kept] = (uint32_t)ns;
+        s->source_index[kept] = i;
+        used += w;
+        kept++;
+    }
+    s->geo_bytes = used;
+    s->n_shapes = kept;
+    return 0;
+}
+
+void go_free_serialized(go_serialized *s)
+{
+    free(s->geo); free(s->shapes); free(s->bbox); free(s->strand_count); free(s->source_index);
+    memset(s, 0, sizeof(*s));
+}
+
+/* ======================================================================== */
+/* Tile tree: literal insert / split (TileTree.hs:81-204)                      */
+/* ======================================================================== */
+enum { N_VTREE = 0, N_HTREE = 1, N_VLEAF = 2, N_HLEAF = 3 };
+typedef struct {
+    int kind;
+    float cut; int child[2];
+    int box[4]; int hd, vd;            /* leaves: l t r b */
+    int *list; int n, cap;             /* leaf shapes, oldest first */
+    uint32_t strands;
+} tnode;
+
+typedef struct {
+    tnode *nodes; int n, cap;
+    const go_serialized *sc;
+    uint32_t max_strands; int max_shapes;
+} ttree;
+
+static int tt_new(ttree *t, int kind, const int *box, int hd, int vd)
+{
+    if (t->n == t->cap) {
+        t->cap = t->cap ? 2 * t->cap : 1024;
+        t->nodes = (tnode *)realloc(t->nodes, sizeof(tnode) * (size_t)t->cap);
+    }
+    tnode *nd = &t->nodes[t->n];
+    memset(nd, 0, sizeof(*nd));
+    nd->kind = kind; nd->hd = hd; nd->vd = vd;
+    if (box) memcpy(nd->box, box, sizeof(nd->box));
+    return t->n++;
+}
+
+static void tt_push(tnode *nd, int s, uint32_t strands)
+{
+    if (nd->n == nd->cap) {
+        nd->cap = nd->cap ? 2 * nd->cap : 8;
+        nd->list = (int *)realloc(nd->list, sizeof(int) * (size_t)nd->cap);
+    }
+    nd->list[nd->n++] = s;
+    nd->strands += strands;
+}
+
+static void tt_insert_v(ttree *t, int id, int s);
+static void tt_insert_h(ttree *t, int id, int s);
+
+/* checkTileSpace (TileTree.hs:167-172) */
+static int tt_fits(ttree *t, tnode *nd, int s)
+{
+    return nd->n < t->max_shapes - 1 &&
+           nd->strands + t->sc->strand_count[s] < t->max_strands;
+}
+
+static void tt_insert_v(ttree *t, int id, int s)
+{
+    const float *bb = &t->sc->bbox[4 * s];
+    tnode *nd = &t->nodes[id];
+    if (nd->kind == N_VTREE) {
+        float cut = nd->cut; int c0 = nd->child[0], c1 = nd->child[1];
+        if (bb[1] < cut) tt_insert_h(t, c0, s);
+        if (bb[3] > cut) tt_insert_h(t, c1, s);
+        return;
+    }
+    /* VLeaf */
+    if (tt_fits(t, nd, s) || (nd->box[3] - nd->box[1]) <= 8) {
+        tt_push(nd, s, t->sc->strand_count[s]);
+        return;
+    }
+    /* vSplit (TileTree.hs:184-190) */
+    {
+        int box[4]; memcpy(box, nd->box, sizeof(box));
+        int hd = nd->hd, vd = nd->vd;
+        int cut = box[1] + (box[3] - box[1]) / 2;
+        int *old = nd->list; int nold = nd->n;
+        int tb[4] = { box[0], box[1], box[2], cut };
+        int bt[4] = { box[0], cut, box[2], box[3] };
+        int a = tt_new(t, N_HLEAF, tb, hd, vd - 1);
+        int b = tt_new(t, N_HLEAF, bt, hd, vd - 1);
+        nd = &t->nodes[id];
+        nd->kind = N_VTREE; nd->cut = (float)cut; nd->child[0] = a; nd->child[1] = b;
+        nd->list = NULL; nd->n = nd->cap = 0; nd->strands = 0;
+        for (int i = 0; i < nold; i++) tt_insert_v(t, id, old[i]);
+        free(old);
+        tt_insert_v(t, id, s);
+    }
+}
+
+static void tt_insert_h(ttree *t, int id, int s)
+{
+    const float *bb = &t->sc->bbox[4 * s];
+    tnode *nd = &t->nodes[id];
+    if (nd->kind == N_HTREE) {
+        float cut = nd->cut; int c0 = nd->child[0], c1 = nd->child[1];
+        if (bb[0] < cut) tt_insert_v(t, c0, s);
+        if (bb[2] > cut) tt_insert_v(t, c1, s);
+        return;
+    }
+    if (tt_fits(t, nd, s) || (nd->box[2] - nd->box[0]) <= 8) {
+        tt_push(nd, s, t->sc->strand_count[s]);
+        return;
+    }
+    /* hSplit (TileTree.hs:175-182) */
+    {
+        int box[4]; memcpy(box, nd->box, sizeof(box));
+        int hd = nd->hd, vd = nd->vd;
+        int cut = box[0] + (box[2] - box[0]) / 2;
+        int *old = nd->list; int nold = nd->n;
+        int lb[4] = { box[0], box[1], cut, box[3] };
+        int rb[4] = { cut, box[1], box[2], box[3] };
+        int a = tt_new(t, N_VLEAF, lb, hd - 1, vd);
+        int b = tt_new(t, N_VLEAF, rb, hd - 1, vd);
+        nd = &t->nodes[id];
+        nd->kind = N_HTREE; nd->cut = (float)cut; nd->child[0] = a; nd->child[1] = b;
+        nd->list = NULL; nd->n = nd->cap = 0; nd->strands = 0;
+        for (int i = 0; i < nold; i++) tt_insert_h(t, id, old[i]);
+        free(old);
+        tt_insert_h(t, id, s);
+    }
+}
+
+static int adjusted_log(int x)
+{   /* adjustedLog (TileTree.hs:74-75): ceiling (logBase 2 x), 0 for x < 1 */
+    int k = 0;
+    if (x < 1) return 0;
+    while ((1L << k) < (long)x) k++;
+    return k;
+}
+
+/* buildTileTree goV / goH (TileTree.hs:81-109) */
+static int tt_build_v(ttree *t, int depth, int tile_depth, const int *box);
+static int tt_build_h(ttree *t, int depth, int tile_depth, const int *box)
+{
+    int cut = box[0] + (1 << (depth - 1));
+    if (depth > tile_depth) {
+        int id = tt_new(t, N_HTREE, NULL, 0, 0);
+        int lb[4] = { box[0], box[1], cut, box[3] };
+        int rb[4] = { cut, box[1], box[2], box[3] };
+        int a = tt_build_v(t, depth - 1, tile_depth, lb);
+        int b = tt_build_v(t, depth - 1, tile_depth, rb);
+        t->nodes[id].cut = (float)cut; t->nodes[id].child[0] = a; t->nodes[id].child[1] = b;
+        return id;
+    }
+    return tt_new(t, N_HLEAF, box, depth, depth);
+}
+static int tt_build_v(ttree *t, int depth, int tile_depth, const int *box)
+{
+    int cut = box[1] + (1 << (depth - 1));
+    if (depth > tile_depth) {
+        int id = tt_new(t, N_VTREE, NULL, 0, 0);
+        int tb[4] = { box[0], box[1], box[2], cut };
+        int bb[4] = { box[0], cut, box[2], box[3] };
+        int a = tt_build_h(t, depth, tile_depth, tb);
+        int b = tt_build_h(t, depth, tile_depth, bb);
+        t->nodes[id].cut = (float)cut; t->nodes[id].child[0] = a; t->nodes[id].child[1] = b;
+        return id;
+    }
+    return tt_new(t, N_VLEAF, box, depth, depth);
+}
+
+typedef struct {
+    go_jobs *j; const go_serialized *sc; const go_spec *spec;
+    int tiles_cap, refs_cap, jobs_cap;
+    int count; int col_alloc;
+} jobacc;
+
+static void jobs_emit_tile(jobacc *a, const tnode *nd)
+{
+    go_jobs *j = a->j;
+    /* accumulateRasterJobs (Job.hs:151-179).  The call site passes
+     * (threadsPerTile, tilesPerCall) into (maxTilesPerJob, threadsPerTile)
+     * (CallKernels.hs:254); both are 256 in every spec used here. */
+    int max_tiles_per_job = a->spec->threads_per_tile;
+    int col_step = a->spec->tiles_per_call;
+    if (a->count >= max_tiles_per_job) {
+        j->n_jobs++;
+        if (j->n_jobs + 1 >= a->jobs_cap) {
+            a->jobs_cap *= 2;
+            j->job_tile_offset = (int32_t *)realloc(j->job_tile_offset, 4u * (size_t)a->jobs_cap);
+            j->job_shape_offset = (int32_t *)realloc(j->job_shape_offset, 4u * (size_t)a->jobs_cap);
+        }
+        j->job_tile_offset[j->n_jobs] = j->n_tiles;
+        j->job_shape_offset[j->n_jobs] = j->n_refs;
+        a->count = 0; a->col_alloc = 0;
+    }
+    if (j->n_refs + nd->n > a->refs_cap) {
+        while (j->n_refs + nd->n > a->refs_cap) a->refs_cap *= 2;
+        j->shape_refs = (uint8_t *)realloc(j->shape_refs, 16u * (size_t)a->refs_cap);
+    }
+    if (j->n_tiles + 1 > a->tiles_cap) {
+        a->tiles_cap *= 2;
+        j->tiles = (uint8_t *)realloc(j->tiles, 32u * (size_t)a->tiles_cap);
+    }
+    int start = j->n_refs - j->job_shape_offset[j->n_jobs];
+    for (int i = nd->n - 1; i >= 0; i--) {      /* newest first */
+        memcpy(j->shape_refs + 16L * j->n_refs, a->sc->shapes + 16L * nd->list[i], 16);
+        j->n_refs++;
+    }
+    uint8_t *rec = j->tiles + 32L * j->n_tiles;
+    int32_t box[4]; for (int q = 0; q < 4; q++) box[q] = nd->box[q];
+    int16_t hd = (int16_t)nd->hd, vd = (int16_t)nd->vd;
+    int32_t ca = a->col_alloc, ss = start, sl = nd->n;
+    memcpy(rec, box, 16); memcpy(rec + 16, &hd, 2); memcpy(rec + 18, &vd, 2);
+    memcpy(rec + 20, &ca, 4); memcpy(rec + 24, &ss, 4); memcpy(rec + 28, &sl, 4);
+    j->n_tiles++;
+    a->count++;
+    a->col_alloc += col_step;
+}
+
+static void tt_traverse(ttree *t, int id, jobacc *a)
+{   /* traverseTileTree (TileTree.hs:193-204): V top->bottom, H left->right */
+    tnode *nd = &t->nodes[id];
+    if (nd->kind == N_VTREE || nd->kind == N_HTREE) {
+        tt_traverse(t, nd->child[0], a);
+        tt_traverse(t, t->nodes[id].child[1], a);
+    } else {
+        jobs_emit_tile(a, nd);
+    }
+}
+
+int go_build_jobs(const go_spec *spec, int32_t cw, int32_t ch, const go_serialized *sc, go_jobs *out)
+{
+    ttree t; memset(&t, 0, sizeof(t));
+    t.sc = sc; t.max_strands = (uint32_t)spec->max_strands_per_tile; t.max_shapes = spec->max_shapes;
+    int canvas_depth = adjusted_log(cw > ch ? cw : ch);
+    int tile_depth = adjusted_log(spec->max_tile_size);
+    int root_box[4] = { 0, 0, 1 << canvas_depth, 1 << canvas_depth };
+    int root = tt_build_v(&t, canvas_depth, tile_depth, root_box);
+    for (int s = 0; s < sc->n_shapes; s++) tt_insert_v(&t, root, s);   /* onShape order */
+
+    memset(out, 0, sizeof(*out));
+    jobacc a; memset(&a, 0, sizeof(a));
+    a.j = out; a.sc = sc; a.spec = spec;
+    a.tiles_cap = 256; a.refs_cap = 1024; a.jobs_cap = 16;
+    out->tiles = (uint8_t *)malloc(32u * (size_t)a.tiles_cap);
+    out->shape_refs = (uint8_t *)malloc(16u * (size_t)a.refs_cap);
+    out->job_tile_offset = (int32_t *)calloc((size_t)a.jobs_cap, 4);
+    out->job_shape_offset = (int32_t *)calloc((size_t)a.jobs_cap, 4);
+    tt_traverse(&t, root, &a);
+    out->n_jobs++;
+    out->job_tile_offset[out->n_jobs] = out->n_tiles;
+    out->job_shape_offset[out->n_jobs] = out->n_refs;
+    for (int i = 0; i < t.n; i++) free(t.nodes[i].list);
+    free(t.nodes);
+    return 0;
+}
+
+void go_free_jobs(go_jobs *j)
+{
+    free(j->job_tile_offset); free(j->job_shape_offset); free(j->tiles); free(j->shape_refs);
+    memset(j, 0, sizeof(*j));
+}
+
+/* ======================================================================== */
+/* Kernels (Kernels.cl)                                                       */
+/* ======================================================================== */
+typedef struct { float top, bottom, left, right; } thr_t;      /* THRESHOLD */
+typedef struct { v2 right, left, ctl; float xpos; int index; } trav_t;
+
+typedef struct {
+    const go_spec *spec;
+    /* TileState (Kernels.cl:500-513) */
+    int shape_start, num_shapes;
+    int dx, dy, ix, iy, tile_w, tile_h, bw, bh;
+    int int_height; float float_height; uint32_t thread_unique;
+    /* ThresholdQueue: the reference keeps the live queue in the suffix
+     * [start, MAXT) of the column's array (initQueueSlice, pushTopSlot) */
+    thr_t *q; uint32_t *qh; int qstart, qlen; int overflow;
+    long guard;                     /* remaining section iterations */
+    /* ShapeState */
+    uint32_t shape_bits; int32_t *shape_indices; uint64_t stack[O_STACK_SECTIONS];
+} col_t;
+
+static inline int    hdr_pos(uint32_t h)        { return (h & O_POSITIVE_SLOPE) != 0; }
+static inline int    hdr_persist_top(uint32_t h){ return (h & O_PERSIST_AND_SLOPE) == O_PERSIST_TOP; }
+static inline int    hdr_persist_bot(uint32_t h){ return (h & O_PERSIST_AND_SLOPE) == O_PERSIST_BOTTOM; }
+static inline int    hdr_persist(uint32_t h)    { return (h & O_PERSIST) != 0; }
+
+static inline thr_t  q_get(col_t *c, int i)      { return c->q[c->qstart + i]; }
+static inline uint32_t qh_get(col_t *c, int i)   { return c->qh[c->qstart + i]; }
+static inline void   q_set(col_t *c, int i, uint32_t h, thr_t t) { c->q[c->qstart + i] = t; c->qh[c->qstart + i] = h; }
+static inline void   q_push_slot(col_t *c)
+{
+    if (c->qstart == 0) { c->overflow = 1; return; }
+    c->qstart--; c->qlen++;
+}
+static inline void   q_pop(col_t *c) { c->qstart++; c->qlen--; }
+
+/* thresholdInvertedSlope (Kernels.cl:240-246) */
+static inline float inv_slope(uint32_t h, thr_t t)
+{
+    float sgn = hdr_pos(h) ? 1.0f : -1.0f;
+    return (t.top == t.bottom) ? FLT_MAX : (t.right - t.left) / (t.bottom - t.top) * sgn;
+}
+static inline float top_x(uint32_t h, thr_t t) { return hdr_pos(h) ? t.left : t.right; }
+
+/* thresholdIntersectX (Kernels.cl:900-913) */
+static inline float intersect_x(uint32_t h, thr_t t, float y)
+{
+    if (t.left == t.right) return t.left;
+    float sy = hdr_pos(h) ? t.top : t.bottom;
+    return ((y - sy) * inv_slope(h, t)) + t.left;
+}
+static inline double intersect_x_f64(uint32_t h, thr_t t, double y)
+{
+    if (t.left == t.right) return t.left;
+    double sy = hdr_pos(h) ? t.top : t.bottom;
+    double inv = (t.top == t.bottom) ? (double)FLT_MAX
+               : ((double)t.right - (double)t.left) / ((double)t.bottom - (double)t.top) * (hdr_pos(h) ? 1.0 : -1.0);
+    return ((y - sy) * inv) + (double)t.left;
+}
+
+/* thresholdIsBelow (Kernels.cl:1079-1094) */
+static inline int is_below(uint32_t ah, thr_t a, uint32_t bh, thr_t b)
+{
+    if (a.top > b.top) return 1;
+    if (a.top == b.top) {
+        float ax = top_x(ah, a), bx = top_x(bh, b);
+        if (ax > bx) return 1;
+        if (ax == bx && inv_slope(ah, a) > inv_slope(bh, b)) return 1;
+    }
+    return 0;
+}
+
+/* trimThresholdTop (Kernels.cl:982-1005) */
+static void trim_top(uint32_t *h, thr_t *t, float split_y)
+{
+    float sx = intersect_x(*h, *t, split_y);
+    if (hdr_pos(*h)) {
+        thr_t n = { split_y, t->bottom, sx, t->right }; *t = n; *h &= O_UNPERSIST;
+    } else {
+        thr_t n = { split_y, t->bottom, t->left, sx }; *t = n;
+    }
+}
+
+/* addThreshold (Kernels.cl:1175-1220) */
+static void add_threshold(col_t *c, uint32_t h, thr_t t, int *added, int *enclosed)
+{
+    int keep = hdr_persist(h) || ((t.bottom - t.top) >= O_MINCROP);
+    if (!keep) return;
+    *enclosed = *enclosed || ((t.top <= 0.0f) && hdr_persist_top(h))
+                          || ((t.bottom <= 0.0f) && hdr_persist_bot(h));
+    if (t.top < c->float_height && t.bottom > 0.0f && t.left < 1.0f) {
+        if (t.top <= 0.0f) trim_top(&h, &t, 0.0f);
+        if (t.right <= 0.0f) {
+            *enclosed = 1;
+        } else {
+            *added = 1;
+            q_push_slot(c);
+            if (!c->overflow) q_set(c, 0, h, t);
+        }
+    }
+}
+
+/* addLineSegment / lineToThreshold / lineToHeader (Kernels.cl:1131-1173) */
+static void add_line(col_t *c, v2 l, v2 r, uint32_t bit, int *added, int *enclosed)
+{
+    thr_t t;
+    t.top = cl_min(l.y, r.y);     /* fmin */
+    t.bottom = cl_max(l.y, r.y);  /* fmax */
+    t.left = l.x; t.right = r.x;
+    int pos = l.y <= r.y;
+    int persistent = (l.x != r.x) && (l.x == 0.0f);
+    uint32_t h = (pos ? O_POSITIVE_SLOPE : 0u) | (persistent ? O_PERSIST : 0u) | bit;
+    add_threshold(c, h, t, added, enclosed);
+}
+
+static inline v2 mid_point(v2 a, v2 b)
+{   /* (1-T)*a + T*b with T = 0.5f */
+    v2 r; r.x = 0.5f * a.x + 0.5f * b.x; r.y = 0.5f * a.y + 0.5f * b.y; return r;
+}
+
+/* bifurcateCurve (Kernels.cl:1233-1258) + intersectCurve (:1226-1230) */
+static float intersect_curve(trav_t t)
+{
+    if (!(t.left.x == t.ctl.x && t.left.y == t.ctl.y)) {
+        /* at most GO_MAX_BISECT halvings: a curve inside a 2^30-pixel canvas
+         * is flat after ~35; beyond that FP32 can reach a fixed point where
+         * the reference's loop never ends */
+        for (int it = 0; it < GO_MAX_BISECT; it++) {
+            v2 lm = mid_point(t.left, t.ctl);
+            v2 rm = mid_point(t.ctl, t.right);
+            v2 on = mid_point(lm, rm);
+            float flat = fabsf(t.ctl.x - on.x) + fabsf(t.ctl.y - on.y);
+            if (flat > O_FLATNESS) {
+                if (t.xpos < on.x) { t.ctl = lm; t.right = on; }
+                else               { t.left = on; t.ctl = rm; }
+            } else break;
+        }
+    }
+    if (t.left.x == t.right.x) return cl_min(t.left.y, t.right.y);
+    /* yIntercept (Kernels.cl:818-820) */
+    return (((t.right.y - t.left.y) / (t.right.x - t.left.x)) * (t.xpos - t.left.x)) + t.left.y;
+}
+
+/* spawnThresholds (Kernels.cl:1264-1333) */
+static void spawn(col_t *c, uint32_t bit, trav_t *l, trav_t *r, int *added, int *enclosed)
+{
+    float yl, yr;
+    int lw, rw;
+    yl = (l->left.x >= 0.0f) ? l->left.y : intersect_curve(*l);
+    if ((l->right.x < 1.0f) && (l->right.x > 0.0f)) {
+        v2 a = { l->xpos, yl };
+        add_line(c, a, l->right, bit, added, enclosed);
+        lw = 1;
+    } else lw = 0;
+    yr = (r->right.x <= 1.0f) ? r->right.y : intersect_curve(*r);
+    if ((r->left.x > 0.0f) && (r->left.x < 1.0f) && (l->index != r->index)) {
+        v2 b = { r->xpos, yr };
+        add_line(c, r->left, b, bit, added, enclosed);
+        rw = 1;
+    } else rw = 0;
+    if (l->right.x < r->left.x || (!lw && !rw)) {
+        v2 bl, br;
+        if (lw || (l->left.x == l->right.x)) bl = l->right; else { bl.x = l->xpos; bl.y = yl; }
+        if (rw || (r->left.x == r->right.x)) br = r->left;  else { br.x = r->xpos; br.y = yr; }
+        add_line(c, bl, br, bit, added, enclosed);
+    }
+}
+
+static inline v2 ld2(const uint8_t *p) { v2 r; memcpy(&r.x, p, 4); memcpy(&r.y, p + 4, 4); return r; }
+
+/* searchTree (Kernels.cl:1337-1357) */
+static void search_tree(trav_t *tr, const uint8_t *tree, int tree_size, float dx, float dy, int is_left)
+{
+    tr->index = 0;
+    while (tr->index < tree_size) {
+        const uint8_t *n = tree + 16L * tr->index;
+        v2 p = ld2(n), q = ld2(n + 8);
+        p.x = p.x - dx; p.y = p.y - dy; q.x = q.x - dx; q.y = q.y - dy;
+        if ((tr->xpos < p.x) || (is_left && tr->xpos == p.x)) {
+            tr->right = p;
+            tr->index = (tr->index << 1) + 1;
+        } else {
+            tr->left = p; tr->ctl = q;
+            tr->index = (tr->index << 1) + 2;
+        }
+    }
+}
+
+/* buildThresholdArray (Kernels.cl:1540-1595) */
+static void build_threshold_array(col_t *c, const uint8_t *geo, const uint8_t *shapes)
+{
+    float dx = (float)c->dx, dy = (float)c->dy;
+    for (int n = 0; n < c->num_shapes && c->shape_bits < (uint32_t)c->spec->max_shapes; n++) {
+        int shape_index = c->shape_start + n;
+        const uint8_t *srec = shapes + 16L * shape_index;
+        uint32_t geo_start, num_strands;
+        memcpy(&geo_start, srec + 8, 4); memcpy(&num_strands, srec + 12, 4);
+        const uint8_t *strand = geo + 16L * geo_start;
+        int added = 0, enclosed_shape = 0;
+        for (uint32_t s = 0; s < num_strands; s++) {
+            uint16_t size; memcpy(&size, strand, 2);
+            int enclosed_strand = 0;
+            trav_t l, r;
+            /* traverseTree (Kernels.cl:1378-1408) */
+            int tree_size = ((int)size - 4) / 2;
+            l.right = ld2(strand + 8);  l.right.x -= dx; l.right.y -= dy;
+            l.left = ld2(strand + 16);  l.left.x -= dx;  l.left.y -= dy;
+            l.ctl = ld2(strand + 24);   l.ctl.x -= dx;   l.ctl.y -= dy;
+            l.xpos = 0.0f; l.index = 0;
+            if (l.left.x <= 1.0f && l.right.x > 0.0f) {       /* checkInRange */
+                r = l;
+                l.xpos = cl_max(0.0f, l.left.x);
+                r.xpos = cl_min(1.0f, l.right.x);
+                search_tree(&l, strand + 32, tree_size, dx, dy, 1);
+                search_tree(&r, strand + 32, tree_size, dx, dy, 0);
+                spawn(c, c->shape_bits, &l, &r, &added, &enclosed_strand);
+            }
+            strand += 8L * size;
+            enclosed_shape = enclosed_shape != enclosed_strand;
+        }
+        if (enclosed_shape) {
+            uint32_t b = c->shape_bits;
+            c->stack[b >> 6] ^= (1ull << (b & 63));
+        }
+        if (added || enclosed_shape) {
+            if (c->shape_bits < (uint32_t)c->spec->max_shapes)
+                c->shape_indices[c->shape_bits] = shape_index;
+            c->shape_bits += 1;
+        }
+    }
+}
+
+/* sortThresholdArray (Kernels.cl:1962-1976): bubble sort with swapIfAbove */
+static void sort_thresholds(col_t *c)
+{
+    int done = 0;
+    int k = c->qlen;
+    while (!done) {
+        done = 1;
+        for (int i = 0; i < k - 1; i++) {
+            uint32_t ah = qh_get(c, i), bh = qh_get(c, i + 1);
+            thr_t a = q_get(c, i), b = q_get(c, i + 1);
+            if (is_below(ah, a, bh, b)) {
+                q_set(c, i, bh, b); q_set(c, i + 1, ah, a);
+                done = 0;
+            }
+        }
+        k--;
+    }
+}
+
+/* ---- render ------------------------------------------------------------ */
+typedef struct {
+    int current, num_active;
+    float sx0, sy0, sx1, sy1;   /* sectionStart, sectionEnd */
+    float pixel_y;
+    float acc[8];
+    double dsx0, dsx1;          /* FP64 shadow of the section's x extent */
+    double dacc[5];
+    uint32_t rnd_cursor;
+} parse_t;
+
+typedef struct {
+    const float *substances; const uint8_t *pict; const uint8_t *uses;
+    const uint8_t *shapes; const float *random;
+    float bg[4]; int absx, absy;
+} colorctx_t;
+
+static void pass_header(col_t *c, uint32_t h)
+{
+    uint32_t b = h & O_SHAPEBIT_MASK;
+    c->stack[b >> 6] ^= (1ull << (b & 63));
+}
+
+/* findTop (Kernels.cl:281-292) */
+static int find_top(const uint64_t *stack, int ignore_above)
+{
+    int sec = ignore_above >> 6, bits = ignore_above & 63;
+    uint64_t s = stack[sec] & ~(~0ull << bits);
+    while (s == 0 && sec > 0) { sec--; s = stack[sec]; }
+    int top = (s == 0) ? 0 : 64 - __builtin_clzll(s);
+    return (sec << 6) + top - 1;
+}
+
+/* composite (Kernels.cl:878-887) */
+static void composite(const float *fg, const float *bg, float *out)
+{
+    float a = fg[3] + bg[3] * (1.0f - fg[3]);
+    if (a > 0.0f) {
+        for (int i = 0; i < 3; i++)
+            out[i] = ((fg[i] * fg[3]) + (bg[i] * bg[3] * (1.0f - fg[3]))) / a;
+        out[3] = a;
+    } else { out[0] = out[1] = out[2] = out[3] = 0.0f; }
+}
+
+static int f2i_rtz(float v)
+{   /* convert_int (round toward zero); out-of-range saturates */
+    if (!(v == v)) return 0;
+    if (v >= 2147483647.0f) return 2147483647;
+    if (v <= -2147483648.0f) return (-2147483647 - 1);
+    return (int)v;
+}
+
+/* readColor (Kernels.cl:1411-1442) */
+static void read_color(const colorctx_t *cs, uint64_t sid, int solid, float *out)
+{
+    const float *sub = cs->substances + 4 * sid;
+    if (solid) { memcpy(out, sub, 16); return; }
+    uint32_t pict_id; memcpy(&pict_id, sub, 4);
+    const uint8_t *u = cs->uses + 24L * pict_id;
+    float tx, ty, scale; int32_t w, h, off;
+    memcpy(&tx, u, 4); memcpy(&ty, u + 4, 4); memcpy(&w, u + 8, 4); memcpy(&h, u + 12, 4);
+    memcpy(&off, u + 16, 4); memcpy(&scale, u + 20, 4);
+    scale = scale < 0.0000001f ? 0.0000001f : scale;
+    int rx = f2i_rtz(((float)cs->absx / scale) - tx);
+    int ry = f2i_rtz(((float)cs->absy / scale) - ty);
+    if (rx >= 0 && ry >= 0 && rx < w && ry < h) {
+        const uint8_t *px = cs->pict + off + 4L * ((long)ry * w + rx);
+        for (int i = 0; i < 4; i++) out[i] = (float)px[i] / 255.0f;
+    } else { out[0] = out[1] = out[2] = out[3] = 0.0f; }
+}
+
+/* determineColor (Kernels.cl:1447-1513) */
+static void determine_color(col_t *c, const colorctx_t *cs, float *base)
+{
+    int top_bit = c->spec->max_shapes;
+    float next[4] = {0, 0, 0, 0};
+    uint64_t last_id = ~0ull;
+    int last_is_continue = 1, last_is_set = 0, done = 0;
+    base[0] = base[1] = base[2] = base[3] = 0.0f;
+    while (!done) {
+        int should_composite = 1;
+        top_bit = find_top(c->stack, top_bit);
+        if (top_bit < 0) {
+            memcpy(next, cs->bg, 16);
+            done = 1; should_composite = 1; last_is_set = 1;
+        } else {
+            int ref = c->shape_indices[top_bit];
+            uint64_t tag; memcpy(&tag, cs->shapes + 16L * ref, 8);
+            uint64_t sid = tag & TAG_SUBSTANCEID_MASK;
+            int is_cont = (tag & TAG_COMPOUND_MASK) == TAG_CONTINUE;
+            int is_add = (tag & TAG_COMPOUND_MASK) == TAG_ADD;
+            if (sid == last_id) {
+                if (last_is_continue) {
+                    if (!is_cont) { last_is_set = is_add; last_is_continue = 0; }
+                    else last_is_set = !last_is_set;
+                }
+                should_composite = 0;
+            }
+            if (sid != last_id) {
+                read_color(cs, sid, (tag & TAG_SUBSTANCETYPE_MASK) == TAG_SOLID, next);
+                should_composite = 1;
+                last_is_set = is_add || is_cont;
+            }
+            last_id = sid;
+        }
+        if (should_composite && last_is_set) {
+            float o[4];
+            composite(base, next, o);
+            memcpy(base, o, 16);
+            if (base[3] == 1.0f) done = 1;
+        }
+    }
+}
+
+static int count_active(col_t *c, float *next_top)
+{
+    float top = q_get(c, 0).top;
+    int n = 1;
+    while (n < c->qlen) {
+        thr_t nx = q_get(c, n);
+        if (nx.top > top) { *next_top = nx.top; break; }
+        n++;
+    }
+    return n;
+}
+
+/* insertThreshold (Kernels.cl:1105-1124) */
+static void insert_threshold(col_t *c, uint32_t h, thr_t t)
+{
+    q_push_slot(c);
+    if (c->overflow) return;
+    int cur = 0;
+    while (cur < c->qlen - 1) {
+        uint32_t oh = qh_get(c, cur + 1); thr_t o = q_get(c, cur + 1);
+        if (!is_below(h, t, oh, o)) break;
+        q_set(c, cur, oh, o);
+        cur++;
+    }
+    q_set(c, cur, h, t);
+}
+
+/* splitNext = countActive + nextSlicePoint + sliceActive (Kernels.cl:1007-1077) */
+static float split_next(col_t *c, parse_t *p)
+{
+    float slice = FLT_MAX;
+    p->num_active = count_active(c, &slice);
+    {
+        float top = q_get(c, 0).top;
+        float sp = slice;
+        for (int i = 0; i < p->num_active; i++) {
+            float b = q_get(c, i).bottom;
+            if (top < b) sp = cl_min(sp, b);
+        }
+        slice = cl_min(slice, sp);
+    }
+    for (int cur = 0; cur < p->num_active && !c->overflow; cur++) {
+        uint32_t h = qh_get(c, cur); thr_t t = q_get(c, cur);
+        if (t.top < slice && slice < t.bottom) {
+            /* splitThreshold + divideThreshold (Kernels.cl:928-979) */
+            float sx = intersect_x(h, t, slice);
+            uint32_t bh; thr_t b;
+            if (hdr_pos(h)) {
+                thr_t nb = { slice, t.bottom, sx, t.right };
+                thr_t nt = { t.top, slice, t.left, sx };
+                b = nb; t = nt; bh = h & O_UNPERSIST;
+            } else {
+                thr_t nb = { slice, t.bottom, t.left, sx };
+                thr_t nt = { t.top, slice, sx, t.right };
+                b = nb; t = nt; bh = h; h = h & O_UNPERSIST;
+            }
+            q_set(c, cur, h, t);
+            if (hdr_persist(bh) || ((b.bottom - b.top) >= O_MINCROP))
+                insert_threshold(c, bh, b);
+        }
+    }
+    return slice;
+}
+
+/* verticalAdvance (Kernels.cl:1744-1824) */
+static void vertical_advance(col_t *c, parse_t *p)
+{
+    if (p->sx1 != 1.0f) return;
+    for (int i = 0; i < p->num_active; i++) pass_header(c, qh_get(c, i));
+    float next_break = cl_min(c->float_height, p->pixel_y);
+    float active_bottom = c->qlen > 0 ? q_get(c, 0).bottom : FLT_MAX;
+    if (active_bottom == p->sy1) {
+        while (p->num_active > 0) {
+            uint32_t h = qh_get(c, 0);
+            if (hdr_persist_bot(h)) pass_header(c, h);
+            q_pop(c); p->num_active--;
+        }
+    }
+    float next_bottom;
+    if (p->num_active > 0) {
+        next_bottom = cl_min(active_bottom, next_break);
+    } else {
+        float next_top = p->num_active < c->qlen ? q_get(c, p->num_active).top : FLT_MAX;
+        if (next_top > p->sy1) {
+            next_bottom = cl_min(next_break, next_top);
+        } else {
+            next_bottom = cl_min(next_break, split_next(c, p));
+            while (p->num_active > 0 && q_get(c, 0).top == q_get(c, 0).bottom) {
+                uint32_t h = qh_get(c, 0);
+                if (hdr_persist_top(h)) pass_header(c, h);
+                q_pop(c); p->num_active--;
+            }
+            for (int i = 0; i < p->num_active; i++) {
+                if (q_get(c, i).top > 0.0f) {
+                    uint32_t h = qh_get(c, i);
+                    if (hdr_persist_top(h)) pass_header(c, h);
+                }
+            }
+        }
+    }
+    p->sy0 = p->sy1;
+    p->sy1 = next_bottom;
+    p->sx0 = p->sx1 = 0.0f;
+    p->dsx0 = p->dsx1 = 0.0;
+    p->current = 0;
+}
+
+/* horizontalAdvance + thresholdMidXLow (Kernels.cl:916-926, 1826-1851) */
+static void horizontal_advance(col_t *c, parse_t *p)
+{
+    float nx; double dnx;
+    if (p->current < p->num_active) {
+        thr_t t = q_get(c, p->current); uint32_t h = qh_get(c, p->current);
+        float x;
+        if (c->spec->coverage_mode == 1) {
+            /* thresholdMidXLow as written: yMid first, then (yMid - start.y).
+             * At y ~ 200 yMid rounds by up to 7.6e-6, which a shallow edge's
+             * inverse slope amplifies past 1e-5 of coverage. */
+            float ymid = p->sy0 + ((p->sy1 - p->sy0) * 0.5f);
+            x = intersect_x(h, t, ymid);
+        } else {
+            /* same point, evaluated as ((sy0 - start.y) + half) * inv + left:
+             * every intermediate is bounded by the threshold's own height */
+            if (t.left == t.right) x = t.left;
+            else {
+                float sy = hdr_pos(h) ? t.top : t.bottom;
+                float d = (p->sy0 - sy) + ((p->sy1 - p->sy0) * 0.5f);
+                x = (d * inv_slope(h, t)) + t.left;
+            }
+        }
+        double dymid = (double)p->sy0 + ((double)p->sy1 - (double)p->sy0) * 0.5;
+        double dx = intersect_x_f64(h, t, dymid);
+        if (x >= 1.0f) { nx = 0.0f; dnx = 0.0; }
+        else { nx = cl_max(0.0f, x); dnx = dx < 0.0 ? 0.0 : (dx > 1.0 ? 1.0 : dx); }
+    } else { nx = 1.0f; dnx = 1.0; }
+    p->sx0 = p->sx1; p->sx1 = nx;
+    p->dsx0 = p->dsx1; p->dsx1 = dnx;
+}
+
+/* renderThresholdArray (Kernels.cl:1978-2028) */
+static void render_column(col_t *c, const colorctx_t *cs0, uint32_t *out, float *out_f32, double *out_f64)
+{
+    const go_spec *sp = c->spec;
+    parse_t p; memset(&p, 0, sizeof(p));
+    colorctx_t cs = *cs0;
+    p.sx0 = 0.0f; p.sy0 = 0.0f; p.sx1 = 1.0f; p.sy1 = 0.0f;
+    p.dsx0 = 0.0; p.dsx1 = 1.0;
+    {   /* initRandomField (Kernels.cl:1597-1606) */
+        uint32_t mask = (uint32_t)sp->random_field_size - 1u;
+        uint32_t start = c->thread_unique & mask;
+        uint32_t bits; memcpy(&bits, &cs.random[start], 4);
+        p.rnd_cursor = bits & mask;
+    }
+    cs.absx = c->dx; cs.absy = c->dy;
+    int yint = -1;
+    for (p.pixel_y = 1.0f; p.pixel_y <= c->float_height; p.pixel_y += 1.0f) {
+        yint++;
+        /* calculatePixel (Kernels.cl:1881-1916) */
+        while ((p.sx1 < 1.0f) || (p.sy1 < p.pixel_y)) {
+            if (--c->guard < 0) { c->overflow = 2; return; }
+            vertical_advance(c, &p);
+            if (c->overflow) return;
+            horizontal_advance(c, &p);
+            float col[4];
+            determine_color(c, &cs, col);
+            float area = (p.sx1 - p.sx0) * (p.sy1 - p.sy0);
+            double darea = (p.dsx1 - p.dsx0) * ((double)p.sy1 - (double)p.sy0);
+            float adj = area;
+            double dadj = darea;
+            {
+                uint32_t mask = (uint32_t)sp->random_field_size - 1u;
+                p.rnd_cursor = (p.rnd_cursor + 1u) & mask;      /* getRandom */
+                float rnd = cs.random[p.rnd_cursor];
+                if (sp->stochastic_factor != 0.0f) {
+                    adj = area + (area * rnd * sp->stochastic_factor);
+                    dadj = darea + darea * (double)rnd * (double)sp->stochastic_factor;
+                }
+            }
+            for (int i = 0; i < 4; i++) { p.acc[i] += col[i] * adj; p.acc[4 + i] += adj; }
+            for (int i = 0; i < 4; i++) p.dacc[i] += (double)col[i] * dadj;
+            p.dacc[4] += dadj;
+            if (p.current < p.num_active) pass_header(c, qh_get(c, p.current));
+            p.current++;
+        }
+        float color[4];
+        for (int i = 0; i < 4; i++) color[i] = p.acc[i] / p.acc[4 + i];
+        long pix = (long)(c->dy + yint) * c->bw + c->dx;
+        {   /* colorToSolidPixel_Word32_BGRA (Kernels.cl:842-844), rtz */
+            float ch[3] = { color[2] * 255.0f, color[1] * 255.0f, color[0] * 255.0f };
+            uint32_t w = 0xFF000000u;
+            for (int i = 0; i < 3; i++) {
+                float v = ch[i];
+                uint32_t u = (v == v) ? (v >= 255.0f ? 255u : (v <= 0.0f ? 0u : (uint32_t)v)) : 0u;
+                w |= u << (8 * i);
+            }
+            out[pix] = w;
+        }
+        if (out_f32) for (int i = 0; i < 4; i++) out_f32[4 * pix + i] = color[i];
+        if (out_f64) for (int i = 0; i < 4; i++) out_f64[4 * pix + i] = p.dacc[i] / p.dacc[4];
+        memset(p.acc, 0, sizeof(p.acc));
+        memset(p.dacc, 0, sizeof(p.dacc));
+        p.sx0 = 0.0f; p.sy0 = p.pixel_y;
+        p.dsx0 = 0.0;
+        cs.absy += 1;
+    }
+}
+
+/* initTileState / isActiveThread (Kernels.cl:1692-1722) */
+static int init_column(col_t *c, const go_spec *sp, const uint8_t *tile_rec, int column,
+                       int bw, int bh, int job_index)
+{
+    int32_t box[4]; int16_t hd16, vd16; int32_t ca, ss, sl;
+    memcpy(box, tile_rec, 16); memcpy(&hd16, tile_rec + 16, 2); memcpy(&vd16, tile_rec + 18, 2);
+    memcpy(&ca, tile_rec + 20, 4); memcpy(&ss, tile_rec + 24, 4); memcpy(&sl, tile_rec + 28, 4);
+    int hd = hd16, vd = vd16, cd = sp->compute_depth;
+    int dd = imax(0, vd - (cd - hd));
+    c->spec = sp;
+    c->shape_start = ss; c->num_shapes = sl;
+    c->ix = ((1 << hd) - 1) & column;
+    c->iy = (column >> hd) << dd;
+    c->dx = c->ix + box[0]; c->dy = c->iy + box[1];
+    c->thread_unique = ((uint32_t)ca + (1u << cd) + (uint32_t)job_index * (1u << (2 * cd))) * 7919u;
+    c->int_height = imin(1 << dd, bh - c->dy);
+    c->float_height = (float)c->int_height;
+    c->tile_w = box[2] - box[0]; c->tile_h = box[3] - box[1];
+    c->bw = bw; c->bh = bh;
+    return (c->iy < c->tile_h) && (c->dx < bw) && (c->dy < bh);
+}
+
+typedef struct {
+    thr_t *q; uint32_t *qh; int32_t *shape_indices;
+} colscratch_t;
+
+static int run_column(const go_spec *sp, const go_job_in *in, int tile, int column,
+                      colscratch_t *scr, uint32_t *out, float *out_f32, double *out_f64,
+                      go_job_dump *dump)
+{
+    col_t c; memset(&c, 0, sizeof(c));
+    int slot = tile * sp->threads_per_tile + column;
+    if (!init_column(&c, sp, in->tiles + 32L * tile, column, in->width, in->height, in->job_index)) {
+        if (dump) dump->queue_len[slot] = -1;
+        return 0;
+    }
+    int maxt = sp->max_thresholds;
+    c.q = scr->q; c.qh = scr->qh; c.qstart = maxt; c.qlen = 0;
+    c.guard = GO_SECTION_GUARD;
+    c.shape_indices = scr->shape_indices;
+    memset(c.shape_indices, 0, sizeof(int32_t) * (size_t)sp->max_shapes);
+    build_threshold_array(&c, in->geo, in->shapes);
+    if (c.overflow) return -1;
+    if (dump) {
+        int k = c.qlen < dump->cap ? c.qlen : dump->cap;
+        long base = (long)slot * dump->cap;
+        dump->queue_len[slot] = c.qlen;
+        memcpy(dump->pre_thresholds + 4L * base, c.q + c.qstart, 16L * k);
+        memcpy(dump->pre_headers + base, c.qh + c.qstart, 4L * k);
+    }
+    sort_thresholds(&c);
+    if (dump) {
+        int k = c.qlen < dump->cap ? c.qlen : dump->cap;
+        long base = (long)slot * dump->cap;
+        memcpy(dump->thresholds + 4L * base, c.q + c.qstart, 16L * k);
+        memcpy(dump->headers + base, c.qh + c.qstart, 4L * k);
+        dump->shape_bits[slot] = c.shape_bits;
+        dump->shape_stack[2 * slot] = c.stack[0];
+        dump->shape_stack[2 * slot + 1] = c.stack[1];
+        memcpy(dump->shape_indices + (long)slot * sp->max_shapes, c.shape_indices,
+               sizeof(int32_t) * (size_t)sp->max_shapes);
+    }
+    colorctx_t cs;
+    cs.substances = in->substances; cs.pict = in->pict_data; cs.uses = in->pict_uses;
+    cs.shapes = in->shapes; cs.random = in->random_field;
+    memcpy(cs.bg, in->background, 16);
+    render_column(&c, &cs, out, out_f32, out_f64);
+    return c.overflow ? -c.overflow : 0;
+}
+
+static colscratch_t scratch_new(const go_spec *sp)
+{
+    colscratch_t s;
+    s.q = (thr_t *)malloc(sizeof(thr_t) * (size_t)sp->max_thresholds);
+    s.qh = (uint32_t *)malloc(4u * (size_t)sp->max_thresholds);
+    s.shape_indices = (int32_t *)malloc(4u * (size_t)sp->max_shapes);
+    return s;
+}
+static void scratch_free(colscratch_t *s) { free(s->q); free(s->qh); free(s->shape_indices); }
+
+int go_raster_job(const go_spec *sp, const go_job_in *in, uint32_t *out,
+                  float *out_f32, double *out_f64, go_job_dump *dump)
+{
+    colscratch_t scr = scratch_new(sp);
+    int rc = 0;
+    for (int t = 0; t < in->n_tiles && rc == 0; t++)
+        for (int col = 0; col < sp->threads_per_tile && rc == 0; col++)
+            rc = run_column(sp, in, t, col, &scr, out, out_f32, out_f64, dump);
+    scratch_free(&scr);
+    return rc;
+}
+
+/* ---- multi-threaded whole-scene driver (for the CPU baseline) ----------- */
+typedef struct {
+    const go_spec *sp; const go_jobs *jobs; const go_serialized *sc; const go_job_in *common;
+    uint32_t *out; float *out_f32; double *out_f64;
+    int *next; pthread_mutex_t *mu; int total; int *rc;
+    int32_t *tile_job; int n_jobs;
+} worker_t;
+
+static void *worker_main(void *arg)
+{
+    worker_t *w = (worker_t *)arg;
+    colscratch_t scr = scratch_new(w->sp);
+    for (;;) {
+        pthread_mutex_lock(w->mu);
+        int t = (*w->next)++;
+        pthread_mutex_unlock(w->mu);
+        if (t >= w->total) break;
+        int job = w->tile_job[t];
+        go_job_in in = *w->common;
+        /* queueRasterJobs zips the reversed job list with [0..]
+         * (CallKernels.hs:241, 255): job j of the traversal gets index
+         * n_jobs-1-j */
+        in.job_index = w->n_jobs - 1 - job;
+        in.shapes = w->jobs->shape_refs + 16L * w->jobs->job_shape_offset[job];
+        in.tiles = w->jobs->tiles + 32L * w->jobs->job_tile_offset[job];
+        in.n_tiles = w->jobs->job_tile_offset[job + 1] - w->jobs->job_tile_offset[job];
+        int local = t - w->jobs->job_tile_offset[job];
+        for (int col = 0; col < w->sp->threads_per_tile; col++) {
+            int rc = run_column(w->sp, &in, local, col, &scr, w->out, w->out_f32, w->out_f64, NULL);
+            if (rc) *w->rc = rc;
+        }
+    }
+    scratch_free(&scr);
+    return NULL;
+}
+
+int go_render_scene(const go_spec *sp, const go_serialized *sc, const go_jobs *jobs,
+                    const go_job_in *common, uint32_t *out, float *out_f32, double *out_f64,
+                    int nthreads)
+{
+    int total = jobs->n_tiles, next = 0, rc = 0;
+    pthread_mutex_t mu = PTHREAD_MUTEX_INITIALIZER;
+    int32_t *tile_job = (int32_t *)malloc(4u * (size_t)(total + 1));
+    for (int j = 0; j < jobs->n_jobs; j++)
+        for (int t = jobs->job_tile_offset[j]; t < jobs->job_tile_offset[j + 1]; t++) tile_job[t] = j;
+    if (nthreads < 1) nthreads = 1;
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nthreads);
+    worker_t w = { sp, jobs, sc, common, out, out_f32, out_f64, &next, &mu, total, &rc, tile_job, jobs->n_jobs };
+    for (int i = 0; i < nthreads; i++) pthread_create(&th[i], NULL, worker_main, &w);
+    for (int i = 0; i < nthreads; i++) pthread_join(th[i], NULL);
+    free(th); free(tile_job);
+    return rc;
+}

@@ -1,0 +1,203 @@
+"""Parity of the HIP path (through the C ABI) with the oracle.
+
+Integer/byte outputs (records, headers, their order, shape state, BGRA
+words) must be bit-identical; pre-quantisation colours must lie within 1e-5
+of the oracle's FP64 integration of the same sections.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import gudni_amd as G
+from gudni_amd import scenes as S
+from oracle import oracle as O
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+COVERAGE_TOL = 1e-5
+
+
+def say(msg):
+    print(f"[gpu-parity] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def comb_scene(teeth=100, size=256):
+    """One zig-zag polygon whose middle columns cross 2*teeth edges inside a
+    single 256-pixel column segment: the live queue outgrows the fast pass's
+    rings and the unit is re-run by the full-capacity pass."""
+    pts = [(10.0, 10.0)]
+    for i in range(teeth):
+        pts.append((200.5, 10.0 + 2.2 * i + 0.55))
+        pts.append((20.5, 10.0 + 2.2 * i + 1.1))
+    pts.append((10.0, 10.0 + 2.2 * teeth + 1.0))
+    return S.build_scene(size, size, [(S.solid_tag(0), [S.straight_pairs(pts)])],
+                         [[0.2, 0.4, 0.8, 0.9]], name="comb")
+
+
+def textured_scene():
+    rng = S.SplitMix64(11)
+    tex = (rng.next(32 * 32) & np.uint64(0xFFFFFFFF)).astype(np.uint32).view(np.uint8)
+    shapes = [(S.solid_tag(0), [S.rectangle_pairs(3.3, 4.1, 40.7, 30.2)]),
+              (S.picture_tag(1), [S.straight_pairs([(10.2, 2.5), (60.1, 20.3), (30.4, 55.9)])]),
+              (S.picture_tag(2), [S.rectangle_pairs(35.5, 35.5, 63.0, 63.5)])]
+    subs = [[0.9, 0.1, 0.1, 0.6], S.picture_substance(0), S.picture_substance(1)]
+    uses = [(-8.0, -3.0, 32, 32, 0, 1.0), (-30.0, -30.0, 32, 32, 0, 0.5)]
+    return S.build_scene(64, 64, shapes, subs, tex, uses, background=(0.1, 0.2, 0.3, 1.0),
+                         name="textured")
+
+
+def compound_scene():
+    """openRectangle-style compound (Layout/Draw.hs:96-103): a rectangle with a
+    subtracted inner rectangle of the same substance, over a second shape."""
+    shapes = [(S.solid_tag(0), [S.rectangle_pairs(4.25, 4.25, 28.75, 28.75)]),
+              (S.solid_tag(0, subtract=True), [S.rectangle_pairs(8.5, 8.5, 24.5, 24.5)]),
+              (S.solid_tag(1), [S.straight_pairs([(0.5, 16.5), (31.5, 2.5), (31.5, 30.5)])])]
+    return S.build_scene(32, 32, shapes, [[0.0, 0.5, 1.0, 0.8], [1.0, 0.8, 0.0, 0.5]],
+                         name="compound")
+
+
+def check_frames(scene, gpu_u32, gpu_f32, ref):
+    bad = H.first_mismatch(gpu_u32, ref["out"])
+    assert bad is None, f"{scene.name}: first BGRA mismatch at {bad}"
+    err = np.abs(gpu_f32.astype(np.float64) - ref["f64"]).max() if gpu_f32 is not None else 0.0
+    assert err <= COVERAGE_TOL, f"{scene.name}: coverage error {err} > {COVERAGE_TOL}"
+    return err
+
+
+@pytest.mark.parametrize("name", ["tinySquare", "compound", "textured", "cfg1_small"])
+def test_small_scenes_level2(rasterizer, name):
+    scene = {"tinySquare": H.tiny_square, "compound": compound_scene, "textured": textured_scene,
+             "cfg1_small": lambda: S.config1(200, 128, 3)}[name]()
+    say(f"level2 {name} {scene.width}x{scene.height}")
+    ref = H.oracle_render(scene)
+    assert ref["rc"] == 0
+    u32, f32 = H.gpu_render_level2(rasterizer, scene)
+    check_frames(scene, u32, f32, ref)
+
+
+def test_tiny_square_values(rasterizer):
+    """Closed form: pixel (0,0) is 0.81 red over white, (1,1) pure red."""
+    scene = H.tiny_square()
+    say("tinySquare closed form")
+    u32, f32 = H.gpu_render_level2(rasterizer, scene)
+    assert u32[1, 1] == 0xFFFF0000
+    assert u32[5, 5] == 0xFFFFFFFF
+    assert abs(f32[0, 0, 1] - (1.0 - 0.81)) < 1e-5
+
+
+def test_cfg1_dump_parity(rasterizer):
+    """Per-column records after generate and after sort, shape bits, initial
+    stack and shape indices: bit-identical to the oracle on two jobs."""
+    scene = S.config1()
+    spec, geo, jobs = G.prepare(scene)
+    rasterizer.upload_common(geo.geo, scene)
+    ospec = H.oracle_spec(spec)
+    og = O.Geometry(scene)
+    oj = O.JobList(ospec, og, scene.width, scene.height)
+    for j in (0, jobs.n_jobs - 1):
+        say(f"cfg1 dump job {j}")
+        tiles, shapes = jobs.job(j)
+        d = rasterizer.dump_job(shapes, tiles, scene.width, scene.height, jobs.n_jobs - 1 - j, cap=64)
+        rc, _, od = O.raster_job_dump(scene, ospec, og, oj, j, cap=64)
+        assert rc == 0
+        np.testing.assert_array_equal(d["qlen"], od["qlen"])
+        assert d["qlen"].max() <= 64
+        for k in ("pre_thr", "thr"):
+            np.testing.assert_array_equal(d[k].view(np.uint32), od[k].view(np.uint32), err_msg=k)
+        for k in ("pre_hdr", "hdr", "shape_bits", "stack"):
+            np.testing.assert_array_equal(d[k], od[k], err_msg=k)
+        bits = od["shape_bits"].astype(np.int64)
+        mask = np.arange(spec.max_shapes)[None, :] < np.minimum(bits, spec.max_shapes)[:, None]
+        np.testing.assert_array_equal(np.where(mask, d["shape_indices"], 0),
+                                      np.where(mask, od["shape_indices"], 0))
+
+
+def test_cfg1_level1_and_level2(rasterizer):
+    scene = S.config1()
+    say("cfg1 oracle")
+    ref = H.oracle_render(scene)
+    assert ref["rc"] == 0
+    say("cfg1 level1 (per job, host buffers)")
+    u1 = H.gpu_render_level1(rasterizer, scene)
+    assert H.first_mismatch(u1, ref["out"]) is None
+    say("cfg1 level2 (whole frame, device buffers)")
+    u2, f2 = H.gpu_render_level2(rasterizer, scene)
+    err = check_frames(scene, u2, f2, ref)
+    say(f"cfg1 coverage max |f32 - f64| = {err:.3g}")
+
+
+def test_cfg2_scaled(rasterizer):
+    scene = S.scaled_config2(1024, 12500, 7)
+    say("cfg2 1024^2 scaled")
+    ref = H.oracle_render(scene, nthreads=16)
+    assert ref["rc"] == 0
+    u2, f2 = H.gpu_render_level2(rasterizer, scene)
+    err = check_frames(scene, u2, f2, ref)
+    say(f"cfg2-scaled coverage max err {err:.3g}")
+
+
+def test_cfg2_full_frame(rasterizer):
+    """The full 4096^2, 200k-shape workload: every BGRA word equals the
+    oracle's."""
+    scene = S.config2()
+    say("cfg2 full oracle (16 threads)")
+    ref = H.oracle_render(scene, want_f64=False, nthreads=16)
+    assert ref["rc"] == 0
+    say("cfg2 full gpu")
+    u2, _ = H.gpu_render_level2(rasterizer, scene)
+    bad = H.first_mismatch(u2, ref["out"])
+    assert bad is None, f"cfg2 first mismatch {bad}"
+
+
+def test_retry_pass_comb(rasterizer):
+    """Queues longer than the fast ring go through the full-capacity pass and
+    still match the oracle bit for bit."""
+    scene = comb_scene()
+    say("comb (retry pass)")
+    ref = H.oracle_render(scene)
+    assert ref["rc"] == 0
+    u2, f2 = H.gpu_render_level2(rasterizer, scene)
+    check_frames(scene, u2, f2, ref)
+
+
+def test_queue_overflow_is_an_error():
+    """maxThresholds smaller than a column's queue: the reference would
+    overwrite the neighbouring column (Kernels.cl:375-380); both the oracle and
+    the HIP path report it."""
+    scene = comb_scene(teeth=40)
+    spec = G.default_spec(max_thresholds=16)
+    say("overflow: oracle")
+    ref = H.oracle_render(scene, spec)
+    assert ref["rc"] == -1
+    say("overflow: gpu")
+    r = G.Rasterizer(spec)
+    try:
+        with pytest.raises(G.GudniError) as ei:
+            H.gpu_render_level2(r, scene, spec)
+        assert ei.value.code == -3
+    finally:
+        r.close()
+
+
+def test_empty_and_offcanvas(rasterizer):
+    """A scene whose only shape lies off the canvas is culled (excludeBox):
+    every pixel is the background."""
+    scene = S.build_scene(40, 24, [(S.solid_tag(0), [S.rectangle_pairs(50, 50, 60, 60)])],
+                          [[1, 0, 0, 1]], background=(0.25, 0.5, 0.75, 1.0), name="offcanvas")
+    say("offcanvas")
+    ref = H.oracle_render(scene)
+    u2, f2 = H.gpu_render_level2(rasterizer, scene)
+    check_frames(scene, u2, f2, ref)
+    assert np.all(u2 == u2[0, 0])
+
+
+def test_nonsquare_canvas(rasterizer):
+    scene = S.config1(300, 200, 13)
+    scene.height = 120
+    say("non-square 200x120")
+    ref = H.oracle_render(scene)
+    u2, f2 = H.gpu_render_level2(rasterizer, scene)
+    check_frames(scene, u2, f2, ref)
