@@ -127,13 +127,18 @@ def main():
     # per-kernel timing: HIP events around each pass, on the launch stream
     r.enable_timing(True)
     fast, retry = [], []
+    stages = np.zeros(4, np.float64)
     for _ in range(max(3, min(args.steps, 10))):
         r.frame_render(out.data_ptr())
         r.frame_check()
         tm = r.last_timings()
         fast.append(float(tm[0]))
         retry.append(float(tm[1]))
+        stages += r.last_stage_cycles().astype(np.float64)
     r.enable_timing(False)
+    tot = stages[:3].sum()
+    stage_share = {k: round(float(v / tot), 4) for k, v in zip(("generate", "sort", "render"), stages[:3])} \
+        if tot > 0 else None
     fast_ms = float(np.mean(fast))
 
     t = time.perf_counter()
@@ -164,7 +169,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "raster_pass (fast pass)", "kernel_ms": fast_ms,
-                     "algorithmic_bytes": b_alg, "retry_pass_ms": float(np.mean(retry))},
+                     "algorithmic_bytes": b_alg, "retry_pass_ms": float(np.mean(retry)),
+                     "stage_cycle_share": stage_share},
         "cpu_baseline": cpu,
         "host_stages_s": {"serialize_and_bin": t_host, "upload": t_upload, "readback": t_read},
     }

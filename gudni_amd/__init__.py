@@ -232,6 +232,12 @@ class Rasterizer:
         self._check(self.L.gudni_last_timings(self.ctx, _ptr(ms), 2))
         return ms
 
+    def last_stage_cycles(self):
+        """(generate, sort, render, units) shader cycles summed over waves."""
+        c = np.zeros(4, np.uint64)
+        self._check(self.L.gudni_last_stage_cycles(self.ctx, _ptr(c), 4))
+        return c
+
 
 def prepare(scene, spec: RasterSpec | None = None):
     """Host stages for a scene: (spec, Geometry, RasterJobs)."""

@@ -70,6 +70,7 @@ SIGNATURES = {
     "gudni_frame_check": (C.c_int, [C.c_void_p]),
     "gudni_enable_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "gudni_last_timings": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
+    "gudni_last_stage_cycles": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
     "gudni_dump_job": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
                                  C.c_int32, C.c_int32, C.c_int32, C.POINTER(Dump)]),
 }

@@ -67,6 +67,26 @@ constexpr uint32_t kStatusGuard = 2u;      // section guard hit
 
 struct Thr { float top, bottom, left, right; };
 
+// The scene buffers are read-only for the whole launch.  Reading them through
+// the constant address space says so, which lets wave-uniform reads (the
+// shape records and strand headers every lane walks) go to the scalar unit.
+#define GUDNI_CONST __attribute__((address_space(4)))
+template <typename T>
+__device__ __forceinline__ const GUDNI_CONST T *ro(const T *p)
+{
+    return (const GUDNI_CONST T *)(p);
+}
+__device__ __forceinline__ float4 ld4(const GUDNI_CONST uint8_t *p)
+{
+    const GUDNI_CONST float *q = reinterpret_cast<const GUDNI_CONST float *>(p);
+    return make_float4(q[0], q[1], q[2], q[3]);
+}
+__device__ __forceinline__ float2 ld2(const GUDNI_CONST uint8_t *p)
+{
+    const GUDNI_CONST float *q = reinterpret_cast<const GUDNI_CONST float *>(p);
+    return make_float2(q[0], q[1]);
+}
+
 __device__ __forceinline__ float cl_max(float x, float y) { return x < y ? y : x; }
 __device__ __forceinline__ float cl_min(float x, float y) { return y < x ? y : x; }
 
@@ -137,6 +157,9 @@ struct Frame {
     float4 *dump_thr; uint32_t *dump_hdr;
     int32_t *dump_qlen; uint32_t *dump_bits; uint64_t *dump_stack; int32_t *dump_sidx;
     int32_t dump_cap, threads_per_tile;
+    // stage cycle counters (null unless timing): [0] generate [1] sort
+    // [2] render [3] work units, summed over waves
+    unsigned long long *stage_cycles;
 };
 
 // A lane's threshold queue: a ring in the interleaved scratch.  Logical
@@ -266,12 +289,12 @@ __device__ void spawn(Queue &q, const Column &c, uint32_t bit, const Trav &l, co
 }
 
 // searchTree (Kernels.cl:1337-1357) over the strand's Eytzinger tree
-__device__ __forceinline__ void search_tree(Trav &tr, const float4 *tree, int tree_size,
+__device__ __forceinline__ void search_tree(Trav &tr, const GUDNI_CONST uint8_t *tree, int tree_size,
                                             float dx, float dy, bool is_left)
 {
     tr.index = 0;
     while (tr.index < tree_size) {
-        const float4 n = tree[tr.index];
+        const float4 n = ld4(tree + 16 * tr.index);
         const float px = n.x - dx, py = n.y - dy, cx = n.z - dx, cy = n.w - dy;
         if ((tr.xpos < px) || (is_left && tr.xpos == px)) {
             tr.right = make_float2(px, py);
@@ -287,24 +310,44 @@ __device__ __forceinline__ void search_tree(Trav &tr, const float4 *tree, int tr
 // buildThresholdArray (Kernels.cl:1540-1595).  The shape and strand loops
 // are wave-uniform (every lane walks the tile's whole shape list); only the
 // range test, tree searches and record pushes diverge.
+//
+// kCompact (a tile of at most maxShapes shapes, i.e. every tile above the
+// 8-pixel floor): the 127-shape cap cannot bind, and the column's shape bit
+// b is a monotone relabelling of the tile-local shape index n (b = number of
+// contributing shapes before n).  Labelling headers and the stack by n
+// instead visits the same shapes in the same order everywhere, and lets the
+// compositing walk read tags and colours from a per-tile table in LDS
+// instead of chasing shapeIndices[b] -> shape record -> substance in memory.
+// `contrib` records which n contributed, so the reference's labels can be
+// recovered (dump path).
+template <bool kCompact>
 __device__ void generate(const Frame &f, Queue &q, const Column &c, int32_t *sidx,
-                         uint32_t &shape_bits, uint64_t &stack0, uint64_t &stack1)
+                         uint32_t &shape_bits, uint64_t &stack0, uint64_t &stack1,
+                         uint64_t &contrib0, uint64_t &contrib1)
 {
     const float dx = (float)c.dx, dy = (float)c.dy;
     const uint32_t max_shapes = (uint32_t)f.max_shapes;
-    for (int n = 0; n < c.num_shapes; ++n) {
+    // The shape and strand walks are the same for every lane of the wave;
+    // readfirstlane makes that explicit so their loads go to the scalar unit.
+    const int num_shapes = (int)__builtin_amdgcn_readfirstlane(c.num_shapes);
+    const int shape_start = (int)__builtin_amdgcn_readfirstlane(c.shape_start);
+    for (int n = 0; n < num_shapes; ++n) {
         if (shape_bits >= max_shapes || q.spilled || q.overflow) break;
-        const int shape_index = c.shape_start + n;
-        const ShapeRec sr = f.shapes[shape_index];
-        const uint8_t *strand = f.geo + 16ull * sr.geo_start;
+        const int shape_index = shape_start + n;
+        const GUDNI_CONST uint32_t *srec =
+            reinterpret_cast<const GUDNI_CONST uint32_t *>(ro(f.shapes) + shape_index);
+        uint64_t offset = 16ull * __builtin_amdgcn_readfirstlane(srec[2]);
+        const uint32_t num_strands = __builtin_amdgcn_readfirstlane(srec[3]);
+        const uint32_t label = kCompact ? (uint32_t)n : shape_bits;
         bool added = false, enclosed_shape = false;
-        for (uint32_t s = 0; s < sr.num_strands; ++s) {
-            const uint16_t size = *reinterpret_cast<const uint16_t *>(strand);
-            const float4 *sp = reinterpret_cast<const float4 *>(strand);
+        for (uint32_t s = 0; s < num_strands; ++s) {
+            const GUDNI_CONST uint8_t *strand = ro(f.geo) + offset;
+            // u16 size in 8-byte units, then u16 0: read as one dword
+            const uint32_t size = *reinterpret_cast<const GUDNI_CONST uint32_t *>(strand) & 0xFFFFu;
             bool enclosed_strand = false;
             // traverseTree (Kernels.cl:1378-1408)
-            const float2 right = *reinterpret_cast<const float2 *>(strand + 8);
-            const float4 lc = sp[1];
+            const float2 right = ld2(strand + 8);
+            const float4 lc = ld4(strand + 16);
             Trav l;
             l.right = make_float2(right.x - dx, right.y - dy);
             l.left = make_float2(lc.x - dx, lc.y - dy);
@@ -316,19 +359,24 @@ __device__ void generate(const Frame &f, Queue &q, const Column &c, int32_t *sid
                 l.xpos = cl_max(0.0f, l.left.x);
                 r.xpos = cl_min(1.0f, l.right.x);
                 const int tree_size = ((int)size - 4) / 2;
-                search_tree(l, sp + 2, tree_size, dx, dy, true);
-                search_tree(r, sp + 2, tree_size, dx, dy, false);
-                spawn(q, c, shape_bits, l, r, added, enclosed_strand);
+                search_tree(l, strand + 32, tree_size, dx, dy, true);
+                search_tree(r, strand + 32, tree_size, dx, dy, false);
+                spawn(q, c, label, l, r, added, enclosed_strand);
             }
-            strand += 8u * size;
+            offset += 8u * size;
             enclosed_shape = enclosed_shape != enclosed_strand;
         }
         if (enclosed_shape) {
-            if (shape_bits < 64u) stack0 ^= 1ull << shape_bits;
-            else stack1 ^= 1ull << (shape_bits - 64u);
+            if (label < 64u) stack0 ^= 1ull << label;
+            else stack1 ^= 1ull << (label - 64u);
         }
         if (added || enclosed_shape) {
-            if (shape_bits < max_shapes) sidx[(int)shape_bits * kWave] = shape_index;
+            if (kCompact) {
+                if (label < 64u) contrib0 |= 1ull << label;
+                else contrib1 |= 1ull << (label - 64u);
+            } else if (shape_bits < max_shapes) {
+                sidx[(int)shape_bits * kWave] = shape_index;
+            }
             shape_bits += 1u;
         }
     }
@@ -531,9 +579,18 @@ __device__ float4 read_color(const Frame &f, uint64_t sid, bool solid, int ax, i
     return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-// determineColor (Kernels.cl:1447-1513)
+// Per-wave LDS table of a compact tile's shapes: tag and (solid) colour by
+// tile-local index.
+struct TileTable {
+    uint64_t tag[128];
+    float4 color[128];
+};
+
+// determineColor (Kernels.cl:1447-1513).  kCompact: the stack is labelled by
+// tile-local shape index and tags/colours come from the LDS table.
+template <bool kCompact>
 __device__ float4 determine_color(const Frame &f, const Stack &st, const int32_t *sidx,
-                                  int ax, int ay)
+                                  const TileTable *tt, int ax, int ay, bool &positional)
 {
     int top_bit = f.max_shapes;
     float4 base = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -549,8 +606,9 @@ __device__ float4 determine_color(const Frame &f, const Stack &st, const int32_t
             done = true;
             last_is_set = true;
         } else {
-            const int ref = sidx[top_bit * kWave];
-            const uint64_t tag = f.shapes[ref].tag;
+            uint64_t tag;
+            if (kCompact) tag = tt->tag[top_bit];
+            else tag = f.shapes[sidx[top_bit * kWave]].tag;
             const uint64_t sid = tag & kTagIdMask;
             const bool is_cont = (tag & kTagCompoundMask) == kTagContinue;
             const bool is_add = (tag & kTagCompoundMask) == kTagAdd;
@@ -561,7 +619,12 @@ __device__ float4 determine_color(const Frame &f, const Stack &st, const int32_t
                 }
                 should_composite = false;
             } else {
-                next = read_color(f, sid, (tag & kTagTypeMask) == kTagSolid, ax, ay);
+                const bool solid = (tag & kTagTypeMask) == kTagSolid;
+                if (kCompact && solid) next = tt->color[top_bit];
+                else {
+                    next = read_color(f, sid, solid, ax, ay);
+                    positional = positional || !solid;
+                }
                 last_is_set = is_add || is_cont;
             }
             last_id = sid;
@@ -590,8 +653,9 @@ __device__ __forceinline__ uint32_t bgra_word(float4 c)
 
 // renderThresholdArray + calculatePixel + horizontalAdvance + sectionColor
 // (Kernels.cl:1724-1742, 1826-1851, 1881-1916, 1978-2028)
+template <bool kCompact>
 __device__ uint32_t render(const Frame &f, Queue &q, const Column &c, Stack st,
-                           const int32_t *sidx)
+                           const int32_t *sidx, const TileTable *tt)
 {
     Sweep p;
     p.current = 0; p.num_active = 0;
@@ -605,6 +669,9 @@ __device__ uint32_t render(const Frame &f, Queue &q, const Column &c, Stack st,
     }
     long guard = kSectionGuard;
     int ay = c.dy;
+    Stack cache[2] = {{0, 0}, {0, 0}};
+    float4 cache_col[2] = {make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
+    bool cache_ok[2] = {false, false};
     int yint = -1;
     for (p.pixel_y = 1.0f; p.pixel_y <= c.float_height; p.pixel_y += 1.0f) {
         ++yint;
@@ -637,7 +704,20 @@ __device__ uint32_t render(const Frame &f, Queue &q, const Column &c, Stack st,
             }
             p.sx0 = p.sx1;
             p.sx1 = nx;
-            const float4 col = determine_color(f, st, sidx, c.dx, ay);
+            // A section's colour is a function of its stack alone unless a
+            // picture substance (sampled at the pixel's position) was read,
+            // so the colours of the last two stacks are kept.
+            float4 col;
+            if (st.s0 == cache[0].s0 && st.s1 == cache[0].s1 && cache_ok[0]) {
+                col = cache_col[0];
+            } else if (st.s0 == cache[1].s0 && st.s1 == cache[1].s1 && cache_ok[1]) {
+                col = cache_col[1];
+            } else {
+                bool positional = false;
+                col = determine_color<kCompact>(f, st, sidx, tt, c.dx, ay, positional);
+                cache[1] = cache[0]; cache_col[1] = cache_col[0]; cache_ok[1] = cache_ok[0];
+                cache[0] = st; cache_col[0] = col; cache_ok[0] = !positional;
+            }
             const float area = (p.sx1 - p.sx0) * (p.sy1 - p.sy0);
             float adj = area;
             if (stochastic) {   // getRandom (Kernels.cl:1608-1613)
@@ -687,16 +767,127 @@ __device__ __forceinline__ uint32_t wave_uniform(uint32_t v)
     return __builtin_amdgcn_readfirstlane(v);
 }
 
+// Number of contributing shapes with tile-local index below n: the
+// reference's shape bit of shape n in a compact tile.
+__device__ __forceinline__ uint32_t bit_of(uint64_t c0, uint64_t c1, uint32_t n)
+{
+    if (n < 64u) return (uint32_t)__popcll(c0 & ((1ull << n) - 1ull));
+    return (uint32_t)(__popcll(c0) + __popcll(c1 & ((1ull << (n - 64u)) - 1ull)));
+}
+
+__device__ __forceinline__ uint32_t relabel(uint32_t h, uint64_t c0, uint64_t c1)
+{
+    return (h & ~kShapeBitMask) | bit_of(c0, c1, h & kShapeBitMask);
+}
+
+// Copy the queue into the dump buffers, in the reference's labels.
+template <bool kCompact>
+__device__ void dump_queue(const Frame &f, const Queue &q, int dslot, bool pre, uint64_t c0,
+                           uint64_t c1)
+{
+    const int k = min(q.len, f.dump_cap);
+    float4 *thr = pre ? f.dump_pre_thr : f.dump_thr;
+    uint32_t *hdr = pre ? f.dump_pre_hdr : f.dump_hdr;
+    for (int i = 0; i < k; ++i) {
+        const Thr x = q.get(i);
+        const uint32_t h = q.get_h(i);
+        thr[(size_t)dslot * f.dump_cap + i] = make_float4(x.top, x.bottom, x.left, x.right);
+        hdr[(size_t)dslot * f.dump_cap + i] = kCompact ? relabel(h, c0, c1) : h;
+    }
+}
+
+// One work unit: 64 columns of one tile, generate -> sort -> render per lane,
+// with wave-uniform boundaries between the stages so the optional stage
+// clocks bracket whole stages of the wave.
+template <bool kCompact>
+__device__ void run_unit(const Frame &f, int lane, uint32_t unit, Queue &q, int32_t *sidx,
+                         TileTable *tt, uint32_t &status)
+{
+    const int tile = (int)(unit >> 2), quarter = (int)(unit & 3u);
+    const TileRec t = f.tiles[tile];
+    const int job = f.tile_job ? f.tile_job[tile] : 0;
+    const int column = quarter * kWave + lane;
+    Column c;
+    const bool active = init_column(f, t, job, column, c);
+    const int dslot = tile * f.threads_per_tile + column;
+    const bool timed = f.stage_cycles != nullptr;
+    const long long t0 = timed ? clock64() : 0;
+    if (kCompact) {
+        // the tile's tags and solid colours, by tile-local index
+        for (int n = lane; n < c.num_shapes; n += kWave) {
+            const uint64_t tag = f.shapes[c.shape_start + n].tag;
+            tt->tag[n] = tag;
+            tt->color[n] = ((tag & kTagTypeMask) == kTagSolid) ? f.substances[tag & kTagIdMask]
+                                                               : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    uint32_t bits = 0;
+    uint64_t s0 = 0, s1 = 0, c0 = 0, c1 = 0;
+    if (active) generate<kCompact>(f, q, c, sidx, bits, s0, s1, c0, c1);
+    const bool ok = active && !q.spilled && !q.overflow;
+    const long long t1 = timed ? clock64() : 0;
+    if (ok && f.dump_qlen) dump_queue<kCompact>(f, q, dslot, true, c0, c1);
+    if (ok) sort_queue(q);
+    if (ok && f.dump_qlen) {
+        dump_queue<kCompact>(f, q, dslot, false, c0, c1);
+        f.dump_qlen[dslot] = q.len;
+        f.dump_bits[dslot] = bits;
+        uint64_t r0 = s0, r1 = s1;
+        if (kCompact) {     // the initial stack in the reference's labels
+            r0 = r1 = 0;
+            for (uint32_t n = 0; n < 128u; ++n) {
+                const bool on = n < 64u ? ((s0 >> n) & 1ull) : ((s1 >> (n - 64u)) & 1ull);
+                if (on) {
+                    const uint32_t b = bit_of(c0, c1, n);
+                    if (b < 64u) r0 |= 1ull << b; else r1 |= 1ull << (b - 64u);
+                }
+            }
+        }
+        f.dump_stack[2 * (size_t)dslot] = r0;
+        f.dump_stack[2 * (size_t)dslot + 1] = r1;
+        int b = 0;
+        for (int i = 0; i < f.max_shapes; ++i) f.dump_sidx[(size_t)dslot * f.max_shapes + i] = 0;
+        if (kCompact) {
+            for (int n = 0; n < 128; ++n) {
+                const bool on = n < 64 ? ((c0 >> n) & 1ull) : ((c1 >> (n - 64)) & 1ull);
+                if (on && b < f.max_shapes) f.dump_sidx[(size_t)dslot * f.max_shapes + b++] = c.shape_start + n;
+            }
+        } else {
+            for (int i = 0; i < (int)bits && i < f.max_shapes; ++i)
+                f.dump_sidx[(size_t)dslot * f.max_shapes + i] = sidx[i * kWave];
+        }
+    }
+    const long long t2 = timed ? clock64() : 0;
+    if (ok) status = render<kCompact>(f, q, c, Stack{s0, s1}, sidx, tt);
+    if (active && q.overflow) status |= kStatusOverflow;
+    if (!active && f.dump_qlen) f.dump_qlen[dslot] = -1;
+    if (timed) {
+        const long long t3 = clock64();
+        if (lane == 0) {
+            atomicAdd(f.stage_cycles + 0, (unsigned long long)(t1 - t0));
+            atomicAdd(f.stage_cycles + 1, (unsigned long long)(t2 - t1));
+            atomicAdd(f.stage_cycles + 2, (unsigned long long)(t3 - t2));
+            atomicAdd(f.stage_cycles + 3, 1ull);
+        }
+    }
+}
+
 // One persistent pass.  A wave claims a work unit, its lanes run their
 // columns to completion, and the wave loops until the list is empty.
 __global__ void __launch_bounds__(256) raster_pass(Frame f, int from_retry)
 {
+    __shared__ TileTable tables[4];
     const int lane = threadIdx.x & (kWave - 1);
-    const int slot = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
+    const int wave = threadIdx.x / kWave;
+    const int slot = blockIdx.x * (blockDim.x / kWave) + wave;
     const int cap = 1 << f.cap_log2;
     float4 *q_thr = f.q_thr + (size_t)slot * cap * kWave + lane;
     uint32_t *q_hdr = f.q_hdr + (size_t)slot * cap * kWave + lane;
     int32_t *sidx = f.shape_idx + (size_t)slot * f.max_shapes * kWave + lane;
+    TileTable *tt = &tables[wave];
     const int n_work = from_retry ? (int)min((uint32_t)f.retry_capacity, f.retry[0]) : f.n_work;
     for (;;) {
         uint32_t w = 0;
@@ -704,54 +895,14 @@ __global__ void __launch_bounds__(256) raster_pass(Frame f, int from_retry)
         w = wave_uniform(w);
         if ((int)w >= n_work) break;
         const uint32_t unit = from_retry ? f.retry[1 + w] : f.work[w];
-        const int tile = (int)(unit >> 2), quarter = (int)(unit & 3u);
-        const TileRec t = f.tiles[tile];
-        const int job = f.tile_job ? f.tile_job[tile] : 0;
-        const int column = quarter * kWave + lane;
-        Column c;
-        const bool active = init_column(f, t, job, column, c);
+        const int count = f.tiles[unit >> 2].shape_count;   // wave-uniform
         Queue q;
         q.thr = q_thr; q.hdr = q_hdr;
         q.head = 0; q.len = 0; q.mask = cap - 1; q.maxt = f.max_thresholds;
         q.spilled = false; q.overflow = false;
         uint32_t status = 0;
-        const int dslot = tile * f.threads_per_tile + column;
-        if (active) {
-            uint32_t bits = 0;
-            uint64_t s0 = 0, s1 = 0;
-            generate(f, q, c, sidx, bits, s0, s1);
-            if (!q.spilled && !q.overflow) {
-                if (f.dump_qlen) {
-                    const int k = min(q.len, f.dump_cap);
-                    for (int i = 0; i < k; ++i) {
-                        const Thr x = q.get(i);
-                        f.dump_pre_thr[(size_t)dslot * f.dump_cap + i] = make_float4(x.top, x.bottom, x.left, x.right);
-                        f.dump_pre_hdr[(size_t)dslot * f.dump_cap + i] = q.get_h(i);
-                    }
-                }
-                sort_queue(q);
-                if (f.dump_qlen) {
-                    const int k = min(q.len, f.dump_cap);
-                    for (int i = 0; i < k; ++i) {
-                        const Thr x = q.get(i);
-                        f.dump_thr[(size_t)dslot * f.dump_cap + i] = make_float4(x.top, x.bottom, x.left, x.right);
-                        f.dump_hdr[(size_t)dslot * f.dump_cap + i] = q.get_h(i);
-                    }
-                    f.dump_qlen[dslot] = q.len;
-                    f.dump_bits[dslot] = bits;
-                    f.dump_stack[2 * (size_t)dslot] = s0;
-                    f.dump_stack[2 * (size_t)dslot + 1] = s1;
-                    for (int i = 0; i < f.max_shapes; ++i)
-                        f.dump_sidx[(size_t)dslot * f.max_shapes + i] =
-                            (i < (int)bits) ? sidx[i * kWave] : 0;
-                }
-                Stack st{s0, s1};
-                status = render(f, q, c, st, sidx);
-            }
-            if (q.overflow) status |= kStatusOverflow;
-        } else if (f.dump_qlen) {
-            f.dump_qlen[dslot] = -1;
-        }
+        if (count <= f.max_shapes) run_unit<true>(f, lane, unit, q, sidx, tt, status);
+        else run_unit<false>(f, lane, unit, q, sidx, tt, status);
         if (status) atomicOr(f.status, status);
         // a lane whose ring was too small hands the whole unit to the retry pass
         const bool spilled = __any(q.spilled);
@@ -820,6 +971,7 @@ struct gudni_ctx {
     DevBuf tiles, tile_job, shapes, work, out, out_f32;
     DevBuf fast_thr, fast_hdr, fast_sidx, slow_thr, slow_hdr, slow_sidx;
     DevBuf control;   // [0] fast counter [1] slow counter [2] status [3..] retry
+    DevBuf cycles;    // stage cycle counters of the fast pass (timing only)
     DevBuf dump;      // dump staging
     int32_t retry_capacity = 0;
     // uploaded frame (Level 2)
@@ -830,6 +982,7 @@ struct gudni_ctx {
     bool timing = false, timed_render = false;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     float last_ms[2] = {0, 0};
+    unsigned long long last_cycles[4] = {0, 0, 0, 0};
 };
 
 #define CHECK_HIP(ctx, expr)                                                        \
@@ -1028,7 +1181,8 @@ void gudni_destroy(gudni_ctx *ctx)
     DevBuf *bufs[] = {&ctx->geo, &ctx->subs, &ctx->pict, &ctx->uses, &ctx->random,
                       &ctx->tiles, &ctx->tile_job, &ctx->shapes, &ctx->work, &ctx->out,
                       &ctx->out_f32, &ctx->fast_thr, &ctx->fast_hdr, &ctx->fast_sidx,
-                      &ctx->slow_thr, &ctx->slow_hdr, &ctx->slow_sidx, &ctx->control, &ctx->dump};
+                      &ctx->slow_thr, &ctx->slow_hdr, &ctx->slow_sidx, &ctx->control, &ctx->dump,
+                      &ctx->cycles};
     for (DevBuf *b : bufs) b->release();
     for (hipEvent_t &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -1302,6 +1456,11 @@ int gudni_frame_render(gudni_ctx *ctx, uint32_t *out_dev, float *out_f32_dev)
     if (ctx->timing) {
         for (hipEvent_t &e : ctx->ev)
             if (!e) CHECK_HIP(ctx, hipEventCreate(&e));
+        CHECK_HIP(ctx, ctx->cycles.ensure(8 * sizeof(unsigned long long)));
+        hipLaunchKernelGGL(dev::reset_counters, dim3(1), dim3(64), 0, ctx->stream,
+                           (uint32_t *)ctx->cycles.p, 16);
+        CHECK_HIP(ctx, hipGetLastError());
+        f.stage_cycles = (unsigned long long *)ctx->cycles.p;
         CHECK_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     }
     ctx->timed_render = ctx->timing;
@@ -1322,6 +1481,8 @@ int gudni_frame_check(gudni_ctx *ctx)
         CHECK_HIP(ctx, hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]));
         ctx->last_ms[0] = a;
         ctx->last_ms[1] = b;
+        CHECK_HIP(ctx, hipMemcpy(ctx->last_cycles, ctx->cycles.p, sizeof(ctx->last_cycles),
+                                 hipMemcpyDeviceToHost));
         ctx->timed_render = false;
     }
     return GUDNI_OK;
@@ -1338,6 +1499,13 @@ int gudni_last_timings(gudni_ctx *ctx, float *ms, int32_t n)
 {
     if (!ctx || !ms) return GUDNI_ERR_ARG;
     for (int i = 0; i < n && i < 2; ++i) ms[i] = ctx->last_ms[i];
+    return GUDNI_OK;
+}
+
+int gudni_last_stage_cycles(gudni_ctx *ctx, uint64_t *cycles, int32_t n)
+{
+    if (!ctx || !cycles) return GUDNI_ERR_ARG;
+    for (int i = 0; i < n && i < 4; ++i) cycles[i] = ctx->last_cycles[i];
     return GUDNI_OK;
 }
 

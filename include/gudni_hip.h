@@ -172,6 +172,10 @@ int gudni_frame_check(gudni_ctx *ctx);
  * reset + fast pass, [1] retry pass. */
 int gudni_enable_timing(gudni_ctx *ctx, int enable);
 int gudni_last_timings(gudni_ctx *ctx, float *ms, int32_t n);
+/* Shader-clock cycles of the last timed render's fast pass, summed over
+ * waves: [0] threshold generation, [1] sort, [2] coverage/compositing sweep,
+ * [3] work units processed. */
+int gudni_last_stage_cycles(gudni_ctx *ctx, uint64_t *cycles, int32_t n);
 
 /* Test hook: per-column threshold records after generate (queue order) and
  * after sort, plus the initial shape state, for one RasterJob.  Slot =

@@ -88,23 +88,20 @@ def test_tiny_square_values(rasterizer):
     assert abs(f32[0, 0, 1] - (1.0 - 0.81)) < 1e-5
 
 
-def test_cfg1_dump_parity(rasterizer):
-    """Per-column records after generate and after sort, shape bits, initial
-    stack and shape indices: bit-identical to the oracle on two jobs."""
-    scene = S.config1()
+def dump_parity(rasterizer, scene, job_ids, cap=64):
     spec, geo, jobs = G.prepare(scene)
     rasterizer.upload_common(geo.geo, scene)
     ospec = H.oracle_spec(spec)
     og = O.Geometry(scene)
     oj = O.JobList(ospec, og, scene.width, scene.height)
-    for j in (0, jobs.n_jobs - 1):
-        say(f"cfg1 dump job {j}")
+    for j in job_ids(jobs.n_jobs):
+        say(f"{scene.name} dump job {j}")
         tiles, shapes = jobs.job(j)
-        d = rasterizer.dump_job(shapes, tiles, scene.width, scene.height, jobs.n_jobs - 1 - j, cap=64)
-        rc, _, od = O.raster_job_dump(scene, ospec, og, oj, j, cap=64)
+        d = rasterizer.dump_job(shapes, tiles, scene.width, scene.height, jobs.n_jobs - 1 - j, cap=cap)
+        rc, _, od = O.raster_job_dump(scene, ospec, og, oj, j, cap=cap)
         assert rc == 0
         np.testing.assert_array_equal(d["qlen"], od["qlen"])
-        assert d["qlen"].max() <= 64
+        assert d["qlen"].max() <= cap
         for k in ("pre_thr", "thr"):
             np.testing.assert_array_equal(d[k].view(np.uint32), od[k].view(np.uint32), err_msg=k)
         for k in ("pre_hdr", "hdr", "shape_bits", "stack"):
@@ -113,6 +110,20 @@ def test_cfg1_dump_parity(rasterizer):
         mask = np.arange(spec.max_shapes)[None, :] < np.minimum(bits, spec.max_shapes)[:, None]
         np.testing.assert_array_equal(np.where(mask, d["shape_indices"], 0),
                                       np.where(mask, od["shape_indices"], 0))
+
+
+def test_cfg1_dump_parity(rasterizer):
+    """Per-column records after generate and after sort, shape bits, initial
+    stack and shape indices: bit-identical to the oracle on two jobs.  cfg1's
+    tiles sit at the 8-pixel floor with ~750 shapes each (general path)."""
+    dump_parity(rasterizer, S.config1(), lambda n: (0, n - 1))
+
+
+def test_cfg2_dump_parity(rasterizer):
+    """The same on a 1024^2 slice of cfg2, whose tiles hold <= 126 shapes
+    (compact path: records labelled by tile-local shape index internally and
+    relabelled to the reference's shape bits here)."""
+    dump_parity(rasterizer, S.scaled_config2(1024, 12500, 7), lambda n: range(n), cap=96)
 
 
 def test_cfg1_level1_and_level2(rasterizer):
