@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -127,6 +128,7 @@ __device__ __forceinline__ bool is_below(uint32_t ah, const Thr &a, uint32_t bh,
 struct Frame {
     const uint8_t *geo;           // strand heap
     const ShapeRec *shapes;       // shape references of the launch
+    const float4 *bounds;         // per reference: strand extent (xmin, ymin, xmax, ymax)
     const TileRec *tiles;         // tile records, shape_start absolute
     const int32_t *tile_job;      // job index per tile
     const uint32_t *work;         // work units: tile << 2 | quarter
@@ -321,7 +323,8 @@ __device__ __forceinline__ void search_tree(Trav &tr, const GUDNI_CONST uint8_t 
 // `contrib` records which n contributed, so the reference's labels can be
 // recovered (dump path).
 template <bool kCompact>
-__device__ void generate(const Frame &f, Queue &q, const Column &c, int32_t *sidx,
+__device__ void generate(const Frame &f, Queue &q, const Column &c, bool active, int lane,
+                         float span0, float span1, int32_t *sidx,
                          uint32_t &shape_bits, uint64_t &stack0, uint64_t &stack1,
                          uint64_t &contrib0, uint64_t &contrib1)
 {
@@ -331,8 +334,35 @@ __device__ void generate(const Frame &f, Queue &q, const Column &c, int32_t *sid
     // readfirstlane makes that explicit so their loads go to the scalar unit.
     const int num_shapes = (int)__builtin_amdgcn_readfirstlane(c.num_shapes);
     const int shape_start = (int)__builtin_amdgcn_readfirstlane(c.shape_start);
-    for (int n = 0; n < num_shapes; ++n) {
-        if (shape_bits >= max_shapes || q.spilled || q.overflow) break;
+    // A shape none of whose strands reaches a column adds no threshold and no
+    // enclosure there, so it gets no shape bit and can be skipped.  The wave
+    // tests 64 shapes at a time against [span0, span1] (its columns' x extent
+    // widened by a pixel, a superset of checkInRange's per-column test) and
+    // walks only the candidates, in shape order.
+    // A lane also skips a shape lying wholly below its segment (every point
+    // at least a pixel under the segment's bottom): its thresholds all start
+    // at or below RENDEREND, so none is added and none encloses.
+    const float below = dy + c.float_height + 1.0f;
+    bool alive = active;
+    for (int base = 0; base < num_shapes; base += kWave) {
+        alive = alive && shape_bits < max_shapes && !q.spilled && !q.overflow;
+        if (!__any(alive)) break;
+        const int ni = base + lane;
+        bool cand = false;
+        float ymin = 0.0f;
+        if (ni < num_shapes) {
+            const float4 b = f.bounds[shape_start + ni];
+            cand = (b.x <= span1) && (b.z > span0);
+            ymin = b.y;
+        }
+        uint64_t pending = __ballot(cand);
+        while (pending) {
+            const int i = __ffsll((long long)pending) - 1;
+            const int n = base + i;
+            pending &= pending - 1;
+            const float shape_top = __shfl(ymin, i);
+            alive = alive && shape_bits < max_shapes && !q.spilled && !q.overflow;
+            if (!alive || shape_top >= below) continue;
         const int shape_index = shape_start + n;
         const GUDNI_CONST uint32_t *srec =
             reinterpret_cast<const GUDNI_CONST uint32_t *>(ro(f.shapes) + shape_index);
@@ -378,6 +408,7 @@ __device__ void generate(const Frame &f, Queue &q, const Column &c, int32_t *sid
                 sidx[(int)shape_bits * kWave] = shape_index;
             }
             shape_bits += 1u;
+        }
         }
     }
 }
@@ -527,18 +558,6 @@ __device__ void vertical_advance(Queue &q, const Column &c, Sweep &p, Stack &st)
     p.current = 0;
 }
 
-// findTop (Kernels.cl:281-292).  maxShapes <= 127, so only stack sections 0
-// and 1 can hold bits and ignore_above never exceeds 127.
-__device__ __forceinline__ int find_top(const Stack &st, int ignore_above)
-{
-    const int sec = ignore_above >> 6, bits = ignore_above & 63;
-    uint64_t s = (sec == 1 ? st.s1 : st.s0) & ~(~0ull << bits);
-    int base = sec << 6;
-    if (s == 0 && sec == 1) { s = st.s0; base = 0; }
-    const int top = s ? 64 - __clzll((long long)s) : 0;
-    return base + top - 1;
-}
-
 __device__ __forceinline__ float4 composite(const float4 fg, const float4 bg)
 {   // composite (Kernels.cl:878-887)
     const float a = fg.w + bg.w * (1.0f - fg.w);
@@ -592,17 +611,25 @@ template <bool kCompact>
 __device__ float4 determine_color(const Frame &f, const Stack &st, const int32_t *sidx,
                                   const TileTable *tt, int ax, int ay, bool &positional)
 {
-    int top_bit = f.max_shapes;
+    // The walk visits set stack bits from the top (findTop with a falling
+    // ignoreAbove, Kernels.cl:281-292): pop the highest bit of a working copy.
+    // Bits at or above maxShapes are never set.
+    uint64_t w0 = st.s0, w1 = st.s1;
     float4 base = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float4 next = base;
+    float4 next = base;      // REFERENCE: colour; ACCURATE: premultiplied colour
     uint64_t last_id = ~0ull;
     bool last_is_continue = true, last_is_set = false;
+    const bool ref_mode = f.coverage_mode == GUDNI_COVERAGE_REFERENCE;
     for (;;) {
         bool should_composite = true;
         bool done = false;
-        top_bit = find_top(st, top_bit);
+        int top_bit;
+        if (w1) { top_bit = 127 - __clzll((long long)w1); w1 &= ~(1ull << (top_bit - 64)); }
+        else if (w0) { top_bit = 63 - __clzll((long long)w0); w0 &= ~(1ull << top_bit); }
+        else top_bit = -1;
         if (top_bit < 0) {
             next = f.background;
+            if (!ref_mode) next = make_float4(next.x * next.w, next.y * next.w, next.z * next.w, next.w);
             done = true;
             last_is_set = true;
         } else {
@@ -620,20 +647,43 @@ __device__ float4 determine_color(const Frame &f, const Stack &st, const int32_t
                 should_composite = false;
             } else {
                 const bool solid = (tag & kTagTypeMask) == kTagSolid;
-                if (kCompact && solid) next = tt->color[top_bit];
-                else {
+                if (kCompact && solid) {
+                    next = tt->color[top_bit];          // already in the mode's form
+                } else {
                     next = read_color(f, sid, solid, ax, ay);
                     positional = positional || !solid;
+                    if (!ref_mode)
+                        next = make_float4(next.x * next.w, next.y * next.w, next.z * next.w, next.w);
                 }
                 last_is_set = is_add || is_cont;
             }
             last_id = sid;
         }
         if (should_composite && last_is_set) {
-            base = composite(base, next);
+            if (ref_mode) {
+                base = composite(base, next);
+            } else {
+                // premultiplied: base.xyz = P, base.w = A; the alpha update is
+                // the reference's alphaOut expression, so A == 1 terminates at
+                // the same shape
+                const float t = 1.0f - base.w;
+                base.x = base.x + next.x * t;
+                base.y = base.y + next.y * t;
+                base.z = base.z + next.z * t;
+                base.w = base.w + next.w * t;
+            }
             if (base.w == 1.0f) done = true;
         }
         if (done) break;
+    }
+    if (!ref_mode) {
+        if (base.w > 0.0f) {
+            base.x = base.x / base.w;
+            base.y = base.y / base.w;
+            base.z = base.z / base.w;
+        } else {
+            base = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
     }
     return base;
 }
@@ -669,9 +719,8 @@ __device__ uint32_t render(const Frame &f, Queue &q, const Column &c, Stack st,
     }
     long guard = kSectionGuard;
     int ay = c.dy;
-    Stack cache[2] = {{0, 0}, {0, 0}};
-    float4 cache_col[2] = {make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
-    bool cache_ok[2] = {false, false};
+
+
     int yint = -1;
     for (p.pixel_y = 1.0f; p.pixel_y <= c.float_height; p.pixel_y += 1.0f) {
         ++yint;
@@ -704,20 +753,8 @@ __device__ uint32_t render(const Frame &f, Queue &q, const Column &c, Stack st,
             }
             p.sx0 = p.sx1;
             p.sx1 = nx;
-            // A section's colour is a function of its stack alone unless a
-            // picture substance (sampled at the pixel's position) was read,
-            // so the colours of the last two stacks are kept.
-            float4 col;
-            if (st.s0 == cache[0].s0 && st.s1 == cache[0].s1 && cache_ok[0]) {
-                col = cache_col[0];
-            } else if (st.s0 == cache[1].s0 && st.s1 == cache[1].s1 && cache_ok[1]) {
-                col = cache_col[1];
-            } else {
-                bool positional = false;
-                col = determine_color<kCompact>(f, st, sidx, tt, c.dx, ay, positional);
-                cache[1] = cache[0]; cache_col[1] = cache_col[0]; cache_ok[1] = cache_ok[0];
-                cache[0] = st; cache_col[0] = col; cache_ok[0] = !positional;
-            }
+            bool positional = false;
+            const float4 col = determine_color<kCompact>(f, st, sidx, tt, c.dx, ay, positional);
             const float area = (p.sx1 - p.sx0) * (p.sy1 - p.sy0);
             float adj = area;
             if (stochastic) {   // getRandom (Kernels.cl:1608-1613)
@@ -814,11 +851,16 @@ __device__ void run_unit(const Frame &f, int lane, uint32_t unit, Queue &q, int3
     const long long t0 = timed ? clock64() : 0;
     if (kCompact) {
         // the tile's tags and solid colours, by tile-local index
+        const bool premultiply = f.coverage_mode != GUDNI_COVERAGE_REFERENCE;
         for (int n = lane; n < c.num_shapes; n += kWave) {
             const uint64_t tag = f.shapes[c.shape_start + n].tag;
             tt->tag[n] = tag;
-            tt->color[n] = ((tag & kTagTypeMask) == kTagSolid) ? f.substances[tag & kTagIdMask]
-                                                               : make_float4(0.f, 0.f, 0.f, 0.f);
+            float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
+            if ((tag & kTagTypeMask) == kTagSolid) {
+                col = f.substances[tag & kTagIdMask];
+                if (premultiply) col = make_float4(col.x * col.w, col.y * col.w, col.z * col.w, col.w);
+            }
+            tt->color[n] = col;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -826,7 +868,15 @@ __device__ void run_unit(const Frame &f, int lane, uint32_t unit, Queue &q, int3
     }
     uint32_t bits = 0;
     uint64_t s0 = 0, s1 = 0, c0 = 0, c1 = 0;
-    if (active) generate<kCompact>(f, q, c, sidx, bits, s0, s1, c0, c1);
+    {
+        // x extent of this unit's columns (isActiveThread's internalX)
+        const int hd = t.hdepth;
+        const int w = 1 << hd;
+        const int x0 = t.box[0] + (hd >= 6 ? ((quarter * kWave) & (w - 1)) : 0);
+        const int x1 = x0 + (hd >= 6 ? kWave : w);
+        generate<kCompact>(f, q, c, active, lane, (float)x0 - 1.0f, (float)x1 + 1.0f, sidx, bits,
+                           s0, s1, c0, c1);
+    }
     const bool ok = active && !q.spilled && !q.overflow;
     const long long t1 = timed ? clock64() : 0;
     if (ok && f.dump_qlen) dump_queue<kCompact>(f, q, dslot, true, c0, c1);
@@ -968,7 +1018,7 @@ struct gudni_ctx {
     std::vector<uint8_t> geo_host;
     std::vector<float4> subs_host;
     // per launch buffers
-    DevBuf tiles, tile_job, shapes, work, out, out_f32;
+    DevBuf tiles, tile_job, shapes, xrange, work, out, out_f32;
     DevBuf fast_thr, fast_hdr, fast_sidx, slow_thr, slow_hdr, slow_sidx;
     DevBuf control;   // [0] fast counter [1] slow counter [2] status [3..] retry
     DevBuf cycles;    // stage cycle counters of the fast pass (timing only)
@@ -1179,7 +1229,7 @@ void gudni_destroy(gudni_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevBuf *bufs[] = {&ctx->geo, &ctx->subs, &ctx->pict, &ctx->uses, &ctx->random,
-                      &ctx->tiles, &ctx->tile_job, &ctx->shapes, &ctx->work, &ctx->out,
+                      &ctx->tiles, &ctx->tile_job, &ctx->shapes, &ctx->xrange, &ctx->work, &ctx->out,
                       &ctx->out_f32, &ctx->fast_thr, &ctx->fast_hdr, &ctx->fast_sidx,
                       &ctx->slow_thr, &ctx->slow_hdr, &ctx->slow_sidx, &ctx->control, &ctx->dump,
                       &ctx->cycles};
@@ -1245,7 +1295,8 @@ int gudni_upload_common(gudni_ctx *ctx, const void *geo, size_t geo_bytes, const
 // substance ids inside the substance table and picture ids inside the usage
 // table.
 static int validate_job(gudni_ctx *ctx, const ShapeRec *shapes, int32_t n_shapes,
-                        const TileRec *tiles, int32_t n_tiles, int32_t w, int32_t h)
+                        const TileRec *tiles, int32_t n_tiles, int32_t w, int32_t h,
+                        std::vector<float> &xrange)
 {
     if (w <= 0 || h <= 0 || n_tiles < 0 || n_shapes < 0)
         return set_error(ctx, GUDNI_ERR_ARG, "bad job size");
@@ -1270,6 +1321,12 @@ static int validate_job(gudni_ctx *ctx, const ShapeRec *shapes, int32_t n_shapes
                 return set_error(ctx, GUDNI_ERR_ARG, "picture substance refers to a missing usage");
         }
         size_t pos = 16u * (size_t)r.geo_start;
+        // Extent of the shape's strands: x from the end points, y over every
+        // point (curves stay inside their control hull).  NaN coordinates can
+        // never pass the kernel's range tests, so they are left out; a NaN y
+        // makes the y extent unknown, which disables the y test.
+        float xlo = INFINITY, xhi = -INFINITY, ylo = INFINITY, yhi = -INFINITY;
+        bool y_nan = false;
         for (uint32_t k = 0; k < r.num_strands; ++k) {
             if (pos + 8 > geo_bytes)
                 return set_error(ctx, GUDNI_ERR_ARG, "strand header outside the geometry heap");
@@ -1277,8 +1334,23 @@ static int validate_job(gudni_ctx *ctx, const ShapeRec *shapes, int32_t n_shapes
             std::memcpy(&size, ctx->geo_host.data() + pos, 2);
             if (size < 4 || (size & 1) || pos + 8u * size > geo_bytes)
                 return set_error(ctx, GUDNI_ERR_ARG, "malformed strand in the geometry heap");
+            const float *pts = reinterpret_cast<const float *>(ctx->geo_host.data() + pos + 8);
+            const int npts = (int)size - 1;         // right, left, control, tree pairs
+            if (pts[2] < xlo) xlo = pts[2];
+            if (pts[0] > xhi) xhi = pts[0];
+            for (int p = 0; p < npts; ++p) {
+                const float y = pts[2 * p + 1];
+                if (y != y) y_nan = true;
+                if (y < ylo) ylo = y;
+                if (y > yhi) yhi = y;
+            }
             pos += 8u * size;
         }
+        if (y_nan) ylo = -INFINITY;
+        xrange[4 * (size_t)s + 0] = xlo;
+        xrange[4 * (size_t)s + 1] = ylo;
+        xrange[4 * (size_t)s + 2] = xhi;
+        xrange[4 * (size_t)s + 3] = yhi;
     }
     return GUDNI_OK;
 }
@@ -1290,7 +1362,8 @@ static int run_job(gudni_ctx *ctx, const void *shapes, int32_t n_shapes, const v
     (void)frame;
     if (!ctx) return GUDNI_ERR_ARG;
     if (!ctx->random.p) return set_error(ctx, GUDNI_ERR_ARG, "gudni_upload_common was not called");
-    int rc = validate_job(ctx, (const ShapeRec *)shapes, n_shapes, (const TileRec *)tiles, n_tiles, w, h);
+    std::vector<float> xr(4 * (size_t)n_shapes + 4);
+    int rc = validate_job(ctx, (const ShapeRec *)shapes, n_shapes, (const TileRec *)tiles, n_tiles, w, h, xr);
     if (rc) return rc;
     CHECK_HIP(ctx, hipSetDevice(ctx->device));
     rc = ensure_scratch(ctx);
@@ -1303,6 +1376,7 @@ static int run_job(gudni_ctx *ctx, const void *shapes, int32_t n_shapes, const v
     CHECK_HIP(ctx, upload(ctx, ctx->tiles, (const uint8_t *)tiles, 32u * (size_t)n_tiles));
     CHECK_HIP(ctx, upload(ctx, ctx->tile_job, tile_job.data(), 4u * tile_job.size()));
     CHECK_HIP(ctx, upload(ctx, ctx->shapes, (const uint8_t *)shapes, 16u * (size_t)n_shapes));
+    CHECK_HIP(ctx, upload(ctx, ctx->xrange, xr.data(), 16u * (size_t)n_shapes));
     CHECK_HIP(ctx, upload(ctx, ctx->work, work.data(), 4u * work.size()));
     const size_t npix = (size_t)w * (size_t)h;
     CHECK_HIP(ctx, ctx->out.ensure(4 * npix));
@@ -1310,6 +1384,7 @@ static int run_job(gudni_ctx *ctx, const void *shapes, int32_t n_shapes, const v
         CHECK_HIP(ctx, hipMemcpyAsync(ctx->out.p, host_out, 4 * npix, hipMemcpyHostToDevice, ctx->stream));
     dev::Frame f = base_frame(ctx);
     f.shapes = (const ShapeRec *)ctx->shapes.p;
+    f.bounds = (const float4 *)ctx->xrange.p;
     f.tiles = (const TileRec *)ctx->tiles.p;
     f.tile_job = (const int32_t *)ctx->tile_job.p;
     f.work = (const uint32_t *)ctx->work.p;
@@ -1412,8 +1487,9 @@ int gudni_frame_upload(gudni_ctx *ctx, const gudni_jobs *jobs, int32_t width, in
             tile_job[t] = v.n_jobs - 1 - j;
         }
     }
+    std::vector<float> xr(4 * (size_t)v.n_refs + 4);
     int rc = validate_job(ctx, (const ShapeRec *)v.shape_refs, v.n_refs, tiles.data(), v.n_tiles,
-                          width, height);
+                          width, height, xr);
     if (rc) return rc;
     CHECK_HIP(ctx, hipSetDevice(ctx->device));
     rc = ensure_scratch(ctx);
@@ -1425,6 +1501,7 @@ int gudni_frame_upload(gudni_ctx *ctx, const gudni_jobs *jobs, int32_t width, in
     CHECK_HIP(ctx, upload(ctx, ctx->tiles, tiles.data(), 32u * tiles.size()));
     CHECK_HIP(ctx, upload(ctx, ctx->tile_job, tile_job.data(), 4u * tile_job.size()));
     CHECK_HIP(ctx, upload(ctx, ctx->shapes, v.shape_refs, 16u * (size_t)v.n_refs));
+    CHECK_HIP(ctx, upload(ctx, ctx->xrange, xr.data(), 16u * (size_t)v.n_refs));
     CHECK_HIP(ctx, upload(ctx, ctx->work, work.data(), 4u * work.size()));
     CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->frame_w = width;
@@ -1443,6 +1520,7 @@ int gudni_frame_render(gudni_ctx *ctx, uint32_t *out_dev, float *out_f32_dev)
     if (!ctx->frame_ready) return set_error(ctx, GUDNI_ERR_ARG, "gudni_frame_upload was not called");
     dev::Frame f = base_frame(ctx);
     f.shapes = (const ShapeRec *)ctx->shapes.p;
+    f.bounds = (const float4 *)ctx->xrange.p;
     f.tiles = (const TileRec *)ctx->tiles.p;
     f.tile_job = (const int32_t *)ctx->tile_job.p;
     f.work = (const uint32_t *)ctx->work.p;

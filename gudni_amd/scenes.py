@@ -205,7 +205,8 @@ def _blob_pairs(rng, cx, cy, r, k):
 
 
 def config2(n_blobs: int = 180_000, n_glyphs: int = 20_000, size: int = 4096,
-            seed: int = SEEDS[2], texture_side: int = 1024) -> Scene:
+            seed: int = SEEDS[2], texture_side: int = 1024, radius_median: float = 8.0,
+            em_scale: float = 1.0, picture_share: int = 1, k_range=(4, 16)) -> Scene:
     """4096x4096, 200k shapes, ~2.1M quadratics.  Blobs: centre uniform,
     radius 8*exp(1.2*N(0,1)) (median 8 px), 4-16 quadratics.  Glyph-like
     outlines: 4-30 random quadratics inside a 12-48 px em box.  Substances:
@@ -216,11 +217,11 @@ def config2(n_blobs: int = 180_000, n_glyphs: int = 20_000, size: int = 4096,
     rng = SplitMix64(seed)
     cx = rng.uniform(n_blobs) * size
     cy = rng.uniform(n_blobs) * size
-    r = 8.0 * np.exp(1.2 * rng.normal(n_blobs))
-    kb = rng.integers(4, 16, n_blobs)
+    r = radius_median * np.exp(1.2 * rng.normal(n_blobs))
+    kb = rng.integers(k_range[0], k_range[1], n_blobs)
     blob_pairs, _ = _blob_pairs(rng, cx, cy, r, kb)
     kg = rng.integers(4, 30, n_glyphs)
-    em = 12.0 + rng.uniform(n_glyphs) * 36.0
+    em = (12.0 + rng.uniform(n_glyphs) * 36.0) * em_scale
     gx = rng.uniform(n_glyphs) * size
     gy = rng.uniform(n_glyphs) * size
     tg = int(kg.sum())
@@ -241,7 +242,7 @@ def config2(n_blobs: int = 180_000, n_glyphs: int = 20_000, size: int = 4096,
     pairs = pairs_all[np.repeat(start_all[perm], k) + within]
     # substances
     kind = rng.next(n) % np.uint64(10)
-    picture = kind == np.uint64(9)
+    picture = kind >= np.uint64(10 - picture_share)
     colors = random_colors(rng, n)
     tex = noise_texture(rng, texture_side)
     subs = colors.copy()
@@ -270,3 +271,28 @@ def scaled_config2(size: int, n_shapes: int, seed: int, glyph_share: float = 0.1
     n_g = int(round(n_shapes * glyph_share))
     return config2(n_blobs=n_shapes - n_g, n_glyphs=max(n_g, 1), size=size, seed=seed,
                    texture_side=256)
+
+
+def batch_scene(size: int, n_shapes: int, seed: int, k_range=(3, 12)) -> Scene:
+    """One scene of the batched workload: config-2-style blobs and glyph
+    outlines on a size x size canvas, radii scaled so that the expected
+    overdraw matches the 4096^2 scene's, solid substances only."""
+    radius = 8.0 * (size / 4096.0) * math.sqrt(200_000.0 / max(n_shapes, 1))
+    radius = float(min(max(radius, 1.5), size / 8.0))
+    n_g = max(1, n_shapes // 10)
+    sc = config2(n_blobs=max(1, n_shapes - n_g), n_glyphs=n_g, size=size, seed=seed,
+                 texture_side=16, radius_median=radius, em_scale=max(0.25, radius / 8.0),
+                 picture_share=0, k_range=k_range)
+    sc.name = f"batch{size}x{n_shapes}"
+    return sc
+
+
+def config3(n_scenes: int = 256, seed: int = SEEDS[3]):
+    """256 independent scenes: side uniform over 256..2048, shape count
+    log-uniform over [100, 50000], 3-12 quadratics per blob."""
+    rng = SplitMix64(seed)
+    sides = rng.integers(256, 2048, n_scenes)
+    counts = np.exp(np.log(100.0) + rng.uniform(n_scenes) * (np.log(50_000.0) - np.log(100.0)))
+    seeds = rng.next(n_scenes)
+    return [batch_scene(int(s), int(c), int(z & np.uint64(0x7FFFFFFF)))
+            for s, c, z in zip(sides, counts, seeds)]

@@ -51,15 +51,23 @@ typedef struct gudni_spec {
                                      GUDNI_COVERAGE_REFERENCE                 */
 } gudni_spec;
 
-/* Section mid-x evaluation (thresholdMidXLow, Kernels.cl:916-926).  Records,
- * their order and the section structure are identical in both modes.
+/* Evaluation of section areas and colours.  Records, their order, the
+ * section structure and the shapes each colour walk visits are identical in
+ * both modes.
+ * Section mid-x (thresholdMidXLow, Kernels.cl:916-926):
  * REFERENCE evaluates yMid = y0 + (y1 - y0)/2 and then x(yMid) exactly as the
  * reference writes it; in tall column segments yMid's rounding (ulp(200) =
  * 1.5e-5) times a shallow edge's inverse slope costs up to ~1e-5 of
  * coverage.  ACCURATE evaluates the same point as
  * ((y0 - start.y) + (y1 - y0)/2) * invSlope + left, whose intermediates are
  * bounded by the threshold's own height (coverage within ~1e-6 of an FP64
- * integration of the same sections). */
+ * integration of the same sections).
+ * Compositing (determineColor, Kernels.cl:1447-1513): REFERENCE
+ * un-premultiplies after every shape (three divisions per step, composite,
+ * Kernels.cl:878-887); ACCURATE keeps the accumulated colour premultiplied
+ * (P += rgb*a*(1-A), A += a*(1-A) with the reference's alpha expression, so
+ * the walk stops at the same shape) and divides once per walk.  Frames of the
+ * two modes differ by at most one 8-bit step. */
 #define GUDNI_COVERAGE_ACCURATE  0
 #define GUDNI_COVERAGE_REFERENCE 1
 

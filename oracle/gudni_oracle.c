@@ -911,13 +911,21 @@ static void read_color(const colorctx_t *cs, uint64_t sid, int solid, float *out
     } else { out[0] = out[1] = out[2] = out[3] = 0.0f; }
 }
 
-/* determineColor (Kernels.cl:1447-1513) */
-static void determine_color(col_t *c, const colorctx_t *cs, float *base)
+/* determineColor (Kernels.cl:1447-1513).
+ * coverage_mode 1 composites as the reference writes it: every step
+ * un-premultiplies (composite, Kernels.cl:878-887).  coverage_mode 0 keeps the
+ * accumulated colour premultiplied, P += rgb*a*(1-A), A += a*(1-A) (the
+ * reference's alphaOut expression, so termination at A == 1 is identical),
+ * and divides once at the end.  dbase receives the same walk in double. */
+static void determine_color(col_t *c, const colorctx_t *cs, float *base, double *dbase)
 {
     int top_bit = c->spec->max_shapes;
     float next[4] = {0, 0, 0, 0};
     uint64_t last_id = ~0ull;
     int last_is_continue = 1, last_is_set = 0, done = 0;
+    int ref_mode = c->spec->coverage_mode == 1;
+    float P[3] = {0, 0, 0}, A = 0.0f;
+    double dP[3] = {0, 0, 0}, dA = 0.0;
     base[0] = base[1] = base[2] = base[3] = 0.0f;
     while (!done) {
         int should_composite = 1;
@@ -946,12 +954,30 @@ static void determine_color(col_t *c, const colorctx_t *cs, float *base)
             last_id = sid;
         }
         if (should_composite && last_is_set) {
-            float o[4];
-            composite(base, next, o);
-            memcpy(base, o, 16);
-            if (base[3] == 1.0f) done = 1;
+            double dt = 1.0 - dA;
+            for (int i = 0; i < 3; i++) dP[i] += (double)next[i] * (double)next[3] * dt;
+            dA += (double)next[3] * dt;
+            if (ref_mode) {
+                float o[4];
+                composite(base, next, o);
+                memcpy(base, o, 16);
+                if (base[3] == 1.0f) done = 1;
+            } else {
+                float t = 1.0f - A;
+                for (int i = 0; i < 3; i++) P[i] = P[i] + (next[i] * next[3]) * t;
+                A = A + next[3] * t;
+                if (A == 1.0f) done = 1;
+            }
         }
     }
+    if (!ref_mode) {
+        if (A > 0.0f) { for (int i = 0; i < 3; i++) base[i] = P[i] / A; }
+        else { base[0] = base[1] = base[2] = 0.0f; }
+        base[3] = A;
+    }
+    if (dA > 0.0) { for (int i = 0; i < 3; i++) dbase[i] = dP[i] / dA; }
+    else { dbase[0] = dbase[1] = dbase[2] = 0.0; }
+    dbase[3] = dA;
 }
 
 static int count_active(col_t *c, float *next_top)
@@ -1118,7 +1144,8 @@ static void render_column(col_t *c, const colorctx_t *cs0, uint32_t *out, float 
             if (c->overflow) return;
             horizontal_advance(c, &p);
             float col[4];
-            determine_color(c, &cs, col);
+            double dcol[4];
+            determine_color(c, &cs, col, dcol);
             float area = (p.sx1 - p.sx0) * (p.sy1 - p.sy0);
             double darea = (p.dsx1 - p.dsx0) * ((double)p.sy1 - (double)p.sy0);
             float adj = area;
@@ -1133,7 +1160,7 @@ static void render_column(col_t *c, const colorctx_t *cs0, uint32_t *out, float 
                 }
             }
             for (int i = 0; i < 4; i++) { p.acc[i] += col[i] * adj; p.acc[4 + i] += adj; }
-            for (int i = 0; i < 4; i++) p.dacc[i] += (double)col[i] * dadj;
+            for (int i = 0; i < 4; i++) p.dacc[i] += dcol[i] * dadj;
             p.dacc[4] += dadj;
             if (p.current < p.num_active) pass_header(c, qh_get(c, p.current));
             p.current++;
