@@ -50,6 +50,7 @@ def cpu_baseline(scene, seconds_target=15.0):
     the same frame: whole raster jobs, taken in order until ~seconds_target."""
     from oracle import oracle as O
     spec = O.default_spec()
+    spec.coverage_mode = 1   # the reference's own arithmetic, as its CPU path would run
     og = O.Geometry(scene)
     oj = O.JobList(spec, og, scene.width, scene.height)
     threads = min(16, os.cpu_count() or 1)
@@ -66,7 +67,67 @@ def cpu_baseline(scene, seconds_target=15.0):
     dt = time.perf_counter() - t0
     return {"value": pixels / dt / 1e9, "unit": "Gpixels/s", "cores": threads, "kind": "port",
             "sample": f"{jobs_done} of {oj.n_jobs} raster jobs ({pixels} pixels) of the same "
-                      f"4096x4096 frame, oracle C restatement, {dt:.1f} s"}
+                      f"4096x4096 frame, oracle C restatement in the reference's arithmetic, "
+                      f"{dt:.1f} s"}
+
+
+def batched_leg(spec, steps, log_fn):
+    """The batched workload: 256 scenes of 256^2..2048^2 with 100..50k shapes,
+    tile size per scene-size group chosen by the autotuner, all scenes in one
+    persistent launch replayed from a captured hipGraph; compared with the same
+    scenes rendered by one launch each."""
+    import torch
+    import gudni_amd as G
+    from gudni_amd import autotune, scenes as S
+    t = time.perf_counter()
+    scenes = S.config3()
+    t_gen = time.perf_counter() - t
+    r = G.Rasterizer(spec)
+    stream = torch.cuda.current_stream()
+    r.set_stream(stream.cuda_stream)
+    t = time.perf_counter()
+    table = autotune.tune(r, scenes, per_bucket=2, reps=2, log=log_fn)
+    t_tune = time.perf_counter() - t
+    items, chosen, t_prep = autotune.prepare_batch(scenes, table)
+    t = time.perf_counter()
+    b = G.Batch(r, items)
+    t_up = time.perf_counter() - t
+    pixels = sum(s.width * s.height for s in scenes)
+    curves = sum(s.n_curves for s in scenes)
+    b.render(use_graph=True)
+    b.check()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(steps):
+        b.render(use_graph=True)
+    stop.record(stream)
+    lat = b.check()
+    ms = start.elapsed_time(stop) / steps
+    # sequential: one launch per scene
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(len(scenes) + 1)]
+    evs[0].record(stream)
+    for i in range(len(scenes)):
+        b.render_scene(i)
+        evs[i + 1].record(stream)
+    b.check()
+    seq = np.array([evs[i].elapsed_time(evs[i + 1]) for i in range(len(scenes))])
+    seq_lat = np.cumsum(seq)
+    b.close()
+    r.close()
+    return {
+        "scenes": len(scenes), "pixels": pixels, "curves": curves,
+        "ms_per_batch": ms, "gpixels_per_s": pixels / (ms * 1e-3) / 1e9,
+        "mcurves_per_s": curves / (ms * 1e-3) / 1e6,
+        "latency_p50_ms": float(np.percentile(lat, 50)), "latency_p99_ms": float(np.percentile(lat, 99)),
+        "sequential_ms": float(seq.sum()),
+        "sequential_gpixels_per_s": pixels / (seq.sum() * 1e-3) / 1e9,
+        "sequential_latency_p50_ms": float(np.percentile(seq_lat, 50)),
+        "sequential_latency_p99_ms": float(np.percentile(seq_lat, 99)),
+        "speedup_vs_sequential": float(seq.sum() / ms),
+        "tile_size_by_group": table, "graph": True,
+        "host_s": {"generate": t_gen, "autotune": t_tune, "serialize_and_bin": t_prep, "upload": t_up},
+    }
 
 
 def main():
@@ -76,6 +137,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
+    ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -163,6 +226,11 @@ def main():
     gpix = npix / (ms * 1e-3) / 1e9
     mcurves = scene.n_curves / (ms * 1e-3) / 1e6
 
+    batched = None
+    if not args.no_batch:
+        log("batched workload (256 scenes)")
+        batched = batched_leg(spec, args.batch_steps, log)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle on a bounded sample)")
@@ -183,6 +251,7 @@ def main():
                      "kernel": "raster_pass (fast pass)", "kernel_ms": fast_ms,
                      "algorithmic_bytes": b_alg, "retry_pass_ms": float(np.mean(retry)),
                      "stage_cycle_share": stage_share},
+        "batched": batched,
         "cpu_baseline": cpu,
         "host_stages_s": {"serialize_and_bin": t_host, "upload": t_upload, "readback": t_read},
     }

@@ -24,7 +24,7 @@ from . import _lib
 from ._lib import Spec as RasterSpec
 
 __all__ = ["RasterSpec", "default_spec", "Geometry", "RasterJobs", "Rasterizer", "GudniError",
-           "random_field"]
+           "random_field", "Batch"]
 
 ERRORS = {-1: "bad argument", -2: "HIP runtime error", -3: "threshold queue overflow",
           -4: "section guard exceeded", -5: "no HIP device"}
@@ -245,3 +245,66 @@ def prepare(scene, spec: RasterSpec | None = None):
     geo = Geometry(scene)
     jobs = RasterJobs(spec, geo, scene.width, scene.height)
     return spec, geo, jobs
+
+
+class Batch:
+    """Many scenes resident on the device, rendered in one persistent launch
+    (optionally replayed from a captured hipGraph)."""
+
+    def __init__(self, rasterizer: Rasterizer, items, rnd: np.ndarray | None = None):
+        """items: list of (scene, Geometry, RasterJobs)."""
+        self.r = rasterizer
+        L = rasterizer.L
+        rnd = random_field(rasterizer.spec.random_field_size) if rnd is None else rnd
+        self._keep = [rnd]
+        arr = (_lib.BatchScene * len(items))()
+        for k, (scene, geo, jobs) in enumerate(items):
+            subs = np.ascontiguousarray(scene.substances, np.float32)
+            pict = np.ascontiguousarray(scene.pict_data, np.uint8)
+            uses = np.ascontiguousarray(scene.pict_uses, np.uint8)
+            self._keep += [geo, jobs, subs, pict, uses]
+            a = arr[k]
+            a.geo = geo.geo.ctypes.data if geo.geo.size else None
+            a.geo_bytes = geo.geo.size
+            a.substances = subs.ctypes.data
+            a.n_substances = subs.shape[0]
+            a.pict_data = pict.ctypes.data if pict.size else None
+            a.pict_bytes = pict.size
+            a.pict_uses = uses.ctypes.data if uses.size else None
+            a.n_pict_uses = uses.shape[0]
+            a.jobs = jobs.handle
+            a.width, a.height = scene.width, scene.height
+            for c in range(4):
+                a.background[c] = float(scene.background[c])
+        self.sizes = [(s.width, s.height) for s, _, _ in items]
+        self.handle = C.c_void_p()
+        rasterizer._check(L.gudni_batch_upload(rasterizer.ctx, len(items), arr, _ptr(rnd), rnd.size,
+                                               C.byref(self.handle)))
+
+    def render(self, use_graph: bool = False):
+        self.r._check(self.r.L.gudni_batch_render(self.r.ctx, self.handle, 1 if use_graph else 0))
+
+    def render_scene(self, i: int):
+        self.r._check(self.r.L.gudni_batch_render_scene(self.r.ctx, self.handle, i))
+
+    def check(self):
+        lat = np.zeros(len(self.sizes), np.float32)
+        self.r._check(self.r.L.gudni_batch_check(self.r.ctx, self.handle, _ptr(lat)))
+        return lat
+
+    def frame(self, i: int) -> np.ndarray:
+        w, h = self.sizes[i]
+        out = np.zeros(w * h, np.uint32)
+        self.r._check(self.r.L.gudni_batch_frame(self.r.ctx, self.handle, i, _ptr(out)))
+        return out.reshape(h, w)
+
+    def close(self):
+        if self.handle:
+            self.r.L.gudni_batch_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

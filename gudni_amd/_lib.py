@@ -43,6 +43,13 @@ class Dump(C.Structure):
                 ("shape_indices", C.c_void_p)]
 
 
+class BatchScene(C.Structure):
+    _fields_ = [("geo", C.c_void_p), ("geo_bytes", C.c_size_t), ("substances", C.c_void_p),
+                ("n_substances", C.c_int32), ("pict_data", C.c_void_p), ("pict_bytes", C.c_size_t),
+                ("pict_uses", C.c_void_p), ("n_pict_uses", C.c_int32), ("jobs", C.c_void_p),
+                ("width", C.c_int32), ("height", C.c_int32), ("background", C.c_float * 4)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "gudni_default_spec": (None, [C.POINTER(Spec)]),
@@ -73,6 +80,13 @@ SIGNATURES = {
     "gudni_last_stage_cycles": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
     "gudni_dump_job": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
                                  C.c_int32, C.c_int32, C.c_int32, C.POINTER(Dump)]),
+    "gudni_batch_upload": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(BatchScene), C.c_void_p,
+                                     C.c_int32, C.POINTER(C.c_void_p)]),
+    "gudni_batch_render": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "gudni_batch_render_scene": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
+    "gudni_batch_check": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gudni_batch_frame": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
+    "gudni_batch_free": (None, [C.c_void_p]),
 }
 
 _lib = None

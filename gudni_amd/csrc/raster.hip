@@ -125,25 +125,33 @@ __device__ __forceinline__ bool is_below(uint32_t ah, const Thr &a, uint32_t bh,
 }
 
 // Launch-wide read-only inputs.
-struct Frame {
+// One scene: its read-only buffers and its frame.
+struct alignas(16) SceneDesc {
     const uint8_t *geo;           // strand heap
-    const ShapeRec *shapes;       // shape references of the launch
+    const ShapeRec *shapes;       // shape references (tile shape_start indexes here)
     const float4 *bounds;         // per reference: strand extent (xmin, ymin, xmax, ymax)
+    const float4 *substances;
+    const uint8_t *pict;
+    const PictUse *uses;
+    uint32_t *out;                // BGRA words
+    float4 *out_f32;              // optional pre-quantisation colours
+    float bg[4];                  // background RGBA
+    int32_t width, height;
+    int32_t pad0, pad1;
+};
+
+struct Frame {
+    const SceneDesc *scenes;      // scene table (one entry for a single frame)
+    const int32_t *tile_scene;    // scene of each tile (null: all scene 0)
+    unsigned long long *scene_done; // wall clock when a scene's last unit ended
     const TileRec *tiles;         // tile records, shape_start absolute
     const int32_t *tile_job;      // job index per tile
     const uint32_t *work;         // work units: tile << 2 | quarter
     int32_t n_work;
-    const float4 *substances;
-    const uint8_t *pict;
-    const PictUse *uses;
     const float *random;
-    float4 background;
-    int32_t width, height;
     int32_t compute_depth, max_shapes, max_thresholds, random_mask;
     float stochastic;
     int32_t coverage_mode;
-    uint32_t *out;                // BGRA words
-    float4 *out_f32;              // optional pre-quantisation colours
     // scratch
     float4 *q_thr;                // [slots][cap][64]
     uint32_t *q_hdr;              // [slots][cap][64]
@@ -323,7 +331,7 @@ __device__ __forceinline__ void search_tree(Trav &tr, const GUDNI_CONST uint8_t 
 // `contrib` records which n contributed, so the reference's labels can be
 // recovered (dump path).
 template <bool kCompact>
-__device__ void generate(const Frame &f, Queue &q, const Column &c, bool active, int lane,
+__device__ void generate(const Frame &f, const SceneDesc &s, Queue &q, const Column &c, bool active, int lane,
                          float span0, float span1, int32_t *sidx,
                          uint32_t &shape_bits, uint64_t &stack0, uint64_t &stack1,
                          uint64_t &contrib0, uint64_t &contrib1)
@@ -351,7 +359,7 @@ __device__ void generate(const Frame &f, Queue &q, const Column &c, bool active,
         bool cand = false;
         float ymin = 0.0f;
         if (ni < num_shapes) {
-            const float4 b = f.bounds[shape_start + ni];
+            const float4 b = s.bounds[shape_start + ni];
             cand = (b.x <= span1) && (b.z > span0);
             ymin = b.y;
         }
@@ -365,13 +373,13 @@ __device__ void generate(const Frame &f, Queue &q, const Column &c, bool active,
             if (!alive || shape_top >= below) continue;
         const int shape_index = shape_start + n;
         const GUDNI_CONST uint32_t *srec =
-            reinterpret_cast<const GUDNI_CONST uint32_t *>(ro(f.shapes) + shape_index);
+            reinterpret_cast<const GUDNI_CONST uint32_t *>(ro(s.shapes) + shape_index);
         uint64_t offset = 16ull * __builtin_amdgcn_readfirstlane(srec[2]);
         const uint32_t num_strands = __builtin_amdgcn_readfirstlane(srec[3]);
         const uint32_t label = kCompact ? (uint32_t)n : shape_bits;
         bool added = false, enclosed_shape = false;
-        for (uint32_t s = 0; s < num_strands; ++s) {
-            const GUDNI_CONST uint8_t *strand = ro(f.geo) + offset;
+        for (uint32_t si = 0; si < num_strands; ++si) {
+            const GUDNI_CONST uint8_t *strand = ro(s.geo) + offset;
             // u16 size in 8-byte units, then u16 0: read as one dword
             const uint32_t size = *reinterpret_cast<const GUDNI_CONST uint32_t *>(strand) & 0xFFFFu;
             bool enclosed_strand = false;
@@ -457,11 +465,13 @@ struct Sweep {
 
 struct Stack {
     uint64_t s0, s1;
+    uint32_t dirty;   // groups of 32 bits touched since their composite was cached
     __device__ __forceinline__ void flip(uint32_t h)
     {
         const uint32_t b = h & kShapeBitMask;
         if (b < 64u) s0 ^= 1ull << b;
         else if (b < 128u) s1 ^= 1ull << (b - 64u);
+        dirty |= 1u << ((b >> 5) & 3u);
     }
 };
 
@@ -579,19 +589,19 @@ __device__ __forceinline__ int f2i_rtz(float v)
 }
 
 // readColor (Kernels.cl:1411-1442): solid colour or nearest texel
-__device__ float4 read_color(const Frame &f, uint64_t sid, bool solid, int ax, int ay)
+__device__ float4 read_color(const SceneDesc &s, uint64_t sid, bool solid, int ax, int ay)
 {
-    const float4 sub = f.substances[sid];
+    const float4 sub = s.substances[sid];
     if (solid) return sub;
     const uint32_t pict_id = __float_as_uint(sub.x);
-    const PictUse u = f.uses[pict_id];
+    const PictUse u = s.uses[pict_id];
     float scale = u.scale;
     scale = scale < 0.0000001f ? 0.0000001f : scale;
     const int rx = f2i_rtz(((float)ax / scale) - u.tx);
     const int ry = f2i_rtz(((float)ay / scale) - u.ty);
     if (rx >= 0 && ry >= 0 && rx < u.w && ry < u.h) {
         const uchar4 px = *reinterpret_cast<const uchar4 *>(
-            f.pict + u.mem_offset + 4ll * ((long long)ry * u.w + rx));
+            s.pict + u.mem_offset + 4ll * ((long long)ry * u.w + rx));
         return make_float4((float)px.x / 255.0f, (float)px.y / 255.0f, (float)px.z / 255.0f,
                            (float)px.w / 255.0f);
     }
@@ -603,12 +613,83 @@ __device__ float4 read_color(const Frame &f, uint64_t sid, bool solid, int ax, i
 struct TileTable {
     uint64_t tag[128];
     float4 color[128];
+    // grouped colours (accurate mode): per lane, the premultiplied composite
+    // of each group of 32 tile-local shapes, [group][component][lane]
+    float gc[4][4][64];
+    uint64_t pic0, pic1;   // tile-local shapes whose substance is a picture
+    int grouped;           // the tile qualifies for grouped colours
 };
+
+// Composite of one group's set shapes, top-down, premultiplied.
+__device__ __forceinline__ float4 group_composite(const SceneDesc &s, const TileTable *tt,
+                                                  uint32_t word, int g, int ax, int ay)
+{
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    while (word) {
+        const int b = 31 - __clz((int)word);
+        word &= ~(1u << b);
+        const int n = 32 * g + b;
+        float4 pm;
+        const uint64_t tag = tt->tag[n];
+        if ((tag & kTagTypeMask) == kTagSolid) {
+            pm = tt->color[n];
+        } else {
+            const float4 c = read_color(s, tag & kTagIdMask, false, ax, ay);
+            pm = make_float4(c.x * c.w, c.y * c.w, c.z * c.w, c.w);
+        }
+        const float t = 1.0f - acc.w;
+        acc.x = acc.x + pm.x * t;
+        acc.y = acc.y + pm.y * t;
+        acc.z = acc.z + pm.z * t;
+        acc.w = acc.w + pm.w * t;
+    }
+    return acc;
+}
+
+// Accurate-mode colour of a grouped tile: over(group 3 .. group 0,
+// background), recomputing only the dirty groups' cached composites.
+__device__ float4 grouped_color(const SceneDesc &s, TileTable *tt, Stack &st, int lane, int ax,
+                                int ay)
+{
+    const uint32_t words[4] = {(uint32_t)st.s0, (uint32_t)(st.s0 >> 32), (uint32_t)st.s1,
+                               (uint32_t)(st.s1 >> 32)};
+    float P0 = 0.f, P1 = 0.f, P2 = 0.f, A = 0.f;
+#pragma unroll
+    for (int g = 3; g >= 0; --g) {
+        float4 gcol;
+        if (st.dirty & (1u << g)) {
+            gcol = group_composite(s, tt, words[g], g, ax, ay);
+            tt->gc[g][0][lane] = gcol.x;
+            tt->gc[g][1][lane] = gcol.y;
+            tt->gc[g][2][lane] = gcol.z;
+            tt->gc[g][3][lane] = gcol.w;
+        } else {
+            gcol = make_float4(tt->gc[g][0][lane], tt->gc[g][1][lane], tt->gc[g][2][lane],
+                               tt->gc[g][3][lane]);
+        }
+        const float t = 1.0f - A;
+        P0 = P0 + gcol.x * t;
+        P1 = P1 + gcol.y * t;
+        P2 = P2 + gcol.z * t;
+        A = A + gcol.w * t;
+    }
+    st.dirty = 0;
+    {
+        const float4 pm = make_float4(s.bg[0] * s.bg[3], s.bg[1] * s.bg[3], s.bg[2] * s.bg[3], s.bg[3]);
+        const float t = 1.0f - A;
+        P0 = P0 + pm.x * t;
+        P1 = P1 + pm.y * t;
+        P2 = P2 + pm.z * t;
+        A = A + pm.w * t;
+    }
+    if (A > 0.0f) return make_float4(P0 / A, P1 / A, P2 / A, A);
+    return make_float4(0.f, 0.f, 0.f, 0.f);
+}
 
 // determineColor (Kernels.cl:1447-1513).  kCompact: the stack is labelled by
 // tile-local shape index and tags/colours come from the LDS table.
 template <bool kCompact>
-__device__ float4 determine_color(const Frame &f, const Stack &st, const int32_t *sidx,
+__device__ float4 determine_color(const Frame &f, const SceneDesc &s, const Stack &st, const int32_t *sidx,
                                   const TileTable *tt, int ax, int ay, bool &positional)
 {
     // The walk visits set stack bits from the top (findTop with a falling
@@ -628,14 +709,14 @@ __device__ float4 determine_color(const Frame &f, const Stack &st, const int32_t
         else if (w0) { top_bit = 63 - __clzll((long long)w0); w0 &= ~(1ull << top_bit); }
         else top_bit = -1;
         if (top_bit < 0) {
-            next = f.background;
+            next = make_float4(s.bg[0], s.bg[1], s.bg[2], s.bg[3]);
             if (!ref_mode) next = make_float4(next.x * next.w, next.y * next.w, next.z * next.w, next.w);
             done = true;
             last_is_set = true;
         } else {
             uint64_t tag;
             if (kCompact) tag = tt->tag[top_bit];
-            else tag = f.shapes[sidx[top_bit * kWave]].tag;
+            else tag = s.shapes[sidx[top_bit * kWave]].tag;
             const uint64_t sid = tag & kTagIdMask;
             const bool is_cont = (tag & kTagCompoundMask) == kTagContinue;
             const bool is_add = (tag & kTagCompoundMask) == kTagAdd;
@@ -650,7 +731,7 @@ __device__ float4 determine_color(const Frame &f, const Stack &st, const int32_t
                 if (kCompact && solid) {
                     next = tt->color[top_bit];          // already in the mode's form
                 } else {
-                    next = read_color(f, sid, solid, ax, ay);
+                    next = read_color(s, sid, solid, ax, ay);
                     positional = positional || !solid;
                     if (!ref_mode)
                         next = make_float4(next.x * next.w, next.y * next.w, next.z * next.w, next.w);
@@ -704,8 +785,8 @@ __device__ __forceinline__ uint32_t bgra_word(float4 c)
 // renderThresholdArray + calculatePixel + horizontalAdvance + sectionColor
 // (Kernels.cl:1724-1742, 1826-1851, 1881-1916, 1978-2028)
 template <bool kCompact>
-__device__ uint32_t render(const Frame &f, Queue &q, const Column &c, Stack st,
-                           const int32_t *sidx, const TileTable *tt)
+__device__ uint32_t render(const Frame &f, const SceneDesc &s, Queue &q, const Column &c, Stack st,
+                           const int32_t *sidx, TileTable *tt)
 {
     Sweep p;
     p.current = 0; p.num_active = 0;
@@ -722,10 +803,19 @@ __device__ uint32_t render(const Frame &f, Queue &q, const Column &c, Stack st,
 
 
     int yint = -1;
+    const bool grouped = kCompact && tt->grouped;
+    const int lane = threadIdx.x & (kWave - 1);
+    st.dirty = 0xFu;
     for (p.pixel_y = 1.0f; p.pixel_y <= c.float_height; p.pixel_y += 1.0f) {
         ++yint;
         float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float area_sum = 0.0f;
+        if (grouped) {
+            // picture colours depend on the pixel: their groups are stale
+            const uint64_t p0 = st.s0 & tt->pic0, p1 = st.s1 & tt->pic1;
+            st.dirty |= ((uint32_t)p0 ? 1u : 0u) | ((p0 >> 32) ? 2u : 0u) |
+                        ((uint32_t)p1 ? 4u : 0u) | ((p1 >> 32) ? 8u : 0u);
+        }
         while ((p.sx1 < 1.0f) || (p.sy1 < p.pixel_y)) {
             if (--guard < 0) return kStatusGuard;
             vertical_advance(q, c, p, st);
@@ -753,8 +843,13 @@ __device__ uint32_t render(const Frame &f, Queue &q, const Column &c, Stack st,
             }
             p.sx0 = p.sx1;
             p.sx1 = nx;
-            bool positional = false;
-            const float4 col = determine_color<kCompact>(f, st, sidx, tt, c.dx, ay, positional);
+            float4 col;
+            if (grouped) {
+                col = grouped_color(s, tt, st, lane, c.dx, ay);
+            } else {
+                bool positional = false;
+                col = determine_color<kCompact>(f, s, st, sidx, tt, c.dx, ay, positional);
+            }
             const float area = (p.sx1 - p.sx0) * (p.sy1 - p.sy0);
             float adj = area;
             if (stochastic) {   // getRandom (Kernels.cl:1608-1613)
@@ -771,9 +866,9 @@ __device__ uint32_t render(const Frame &f, Queue &q, const Column &c, Stack st,
         }
         const float4 color = make_float4(acc.x / area_sum, acc.y / area_sum, acc.z / area_sum,
                                          acc.w / area_sum);
-        const long long pix = (long long)(c.dy + yint) * f.width + c.dx;
-        f.out[pix] = bgra_word(color);
-        if (f.out_f32) f.out_f32[pix] = color;
+        const long long pix = (long long)(c.dy + yint) * s.width + c.dx;
+        s.out[pix] = bgra_word(color);
+        if (s.out_f32) s.out_f32[pix] = color;
         p.sx0 = 0.0f;
         p.sy0 = p.pixel_y;
         ++ay;
@@ -782,7 +877,7 @@ __device__ uint32_t render(const Frame &f, Queue &q, const Column &c, Stack st,
 }
 
 // initTileState / isActiveThread (Kernels.cl:1692-1722)
-__device__ __forceinline__ bool init_column(const Frame &f, const TileRec &t, int job, int column,
+__device__ __forceinline__ bool init_column(const Frame &f, const SceneDesc &s, const TileRec &t, int job, int column,
                                             Column &c)
 {
     const int hd = t.hdepth, vd = t.vdepth, cd = f.compute_depth;
@@ -792,11 +887,11 @@ __device__ __forceinline__ bool init_column(const Frame &f, const TileRec &t, in
     c.dx = ix + t.box[0];
     c.dy = iy + t.box[1];
     c.unique = ((uint32_t)t.column_allocation + (1u << cd) + (uint32_t)job * (1u << (2 * cd))) * 7919u;
-    c.int_height = min(1 << dd, f.height - c.dy);
+    c.int_height = min(1 << dd, s.height - c.dy);
     c.float_height = (float)c.int_height;
     c.shape_start = t.shape_start;
     c.num_shapes = t.shape_count;
-    return (iy < t.box[3] - t.box[1]) && (c.dx < f.width) && (c.dy < f.height);
+    return (iy < t.box[3] - t.box[1]) && (c.dx < s.width) && (c.dy < s.height);
 }
 
 __device__ __forceinline__ uint32_t wave_uniform(uint32_t v)
@@ -837,7 +932,7 @@ __device__ void dump_queue(const Frame &f, const Queue &q, int dslot, bool pre, 
 // with wave-uniform boundaries between the stages so the optional stage
 // clocks bracket whole stages of the wave.
 template <bool kCompact>
-__device__ void run_unit(const Frame &f, int lane, uint32_t unit, Queue &q, int32_t *sidx,
+__device__ void run_unit(const Frame &f, const SceneDesc &s, int lane, uint32_t unit, Queue &q, int32_t *sidx,
                          TileTable *tt, uint32_t &status)
 {
     const int tile = (int)(unit >> 2), quarter = (int)(unit & 3u);
@@ -845,22 +940,49 @@ __device__ void run_unit(const Frame &f, int lane, uint32_t unit, Queue &q, int3
     const int job = f.tile_job ? f.tile_job[tile] : 0;
     const int column = quarter * kWave + lane;
     Column c;
-    const bool active = init_column(f, t, job, column, c);
+    const bool active = init_column(f, s, t, job, column, c);
     const int dslot = tile * f.threads_per_tile + column;
     const bool timed = f.stage_cycles != nullptr;
     const long long t0 = timed ? clock64() : 0;
     if (kCompact) {
         // the tile's tags and solid colours, by tile-local index
         const bool premultiply = f.coverage_mode != GUDNI_COVERAGE_REFERENCE;
+        bool all_add = true;
+        uint64_t pic_lo = 0, pic_hi = 0;
         for (int n = lane; n < c.num_shapes; n += kWave) {
-            const uint64_t tag = f.shapes[c.shape_start + n].tag;
+            const uint64_t tag = s.shapes[c.shape_start + n].tag;
             tt->tag[n] = tag;
             float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
             if ((tag & kTagTypeMask) == kTagSolid) {
-                col = f.substances[tag & kTagIdMask];
+                col = s.substances[tag & kTagIdMask];
                 if (premultiply) col = make_float4(col.x * col.w, col.y * col.w, col.z * col.w, col.w);
             }
             tt->color[n] = col;
+            all_add = all_add && (tag & kTagCompoundMask) == kTagAdd;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // grouped colours need every shape added with its own substance
+        bool distinct = true;
+        for (int n = lane; n < c.num_shapes; n += kWave) {
+            const uint64_t sid = tt->tag[n] & kTagIdMask;
+            for (int m = n + 1; m < c.num_shapes && distinct; ++m)
+                distinct = (tt->tag[m] & kTagIdMask) != sid;
+            if ((tt->tag[n] & kTagTypeMask) != kTagSolid) {
+                if (n < 64) pic_lo |= 1ull << n; else pic_hi |= 1ull << (n - 64);
+            }
+        }
+        // OR-reduce the picture masks and AND-reduce the flags over the wave
+        for (int o = 32; o > 0; o >>= 1) {
+            pic_lo |= __shfl_xor(pic_lo, o);
+            pic_hi |= __shfl_xor(pic_hi, o);
+        }
+        const bool grouped_tile = premultiply && __all(all_add && distinct);
+        if (lane == 0) {
+            tt->pic0 = pic_lo;
+            tt->pic1 = pic_hi;
+            tt->grouped = grouped_tile ? 1 : 0;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -874,7 +996,7 @@ __device__ void run_unit(const Frame &f, int lane, uint32_t unit, Queue &q, int3
         const int w = 1 << hd;
         const int x0 = t.box[0] + (hd >= 6 ? ((quarter * kWave) & (w - 1)) : 0);
         const int x1 = x0 + (hd >= 6 ? kWave : w);
-        generate<kCompact>(f, q, c, active, lane, (float)x0 - 1.0f, (float)x1 + 1.0f, sidx, bits,
+        generate<kCompact>(f, s, q, c, active, lane, (float)x0 - 1.0f, (float)x1 + 1.0f, sidx, bits,
                            s0, s1, c0, c1);
     }
     const bool ok = active && !q.spilled && !q.overflow;
@@ -911,7 +1033,7 @@ __device__ void run_unit(const Frame &f, int lane, uint32_t unit, Queue &q, int3
         }
     }
     const long long t2 = timed ? clock64() : 0;
-    if (ok) status = render<kCompact>(f, q, c, Stack{s0, s1}, sidx, tt);
+    if (ok) status = render<kCompact>(f, s, q, c, Stack{s0, s1, 0u}, sidx, tt);
     if (active && q.overflow) status |= kStatusOverflow;
     if (!active && f.dump_qlen) f.dump_qlen[dslot] = -1;
     if (timed) {
@@ -946,14 +1068,27 @@ __global__ void __launch_bounds__(256) raster_pass(Frame f, int from_retry)
         if ((int)w >= n_work) break;
         const uint32_t unit = from_retry ? f.retry[1 + w] : f.work[w];
         const int count = f.tiles[unit >> 2].shape_count;   // wave-uniform
+        const int scene = f.tile_scene
+                              ? (int)__builtin_amdgcn_readfirstlane((uint32_t)f.tile_scene[unit >> 2])
+                              : 0;
+        SceneDesc s;                                  // wave-uniform: scalar loads
+        {
+            const GUDNI_CONST uint64_t *src =
+                reinterpret_cast<const GUDNI_CONST uint64_t *>(ro(f.scenes + scene));
+            uint64_t *dst = reinterpret_cast<uint64_t *>(&s);
+#pragma unroll
+            for (int i = 0; i < (int)(sizeof(SceneDesc) / 8); ++i) dst[i] = src[i];
+        }
         Queue q;
         q.thr = q_thr; q.hdr = q_hdr;
         q.head = 0; q.len = 0; q.mask = cap - 1; q.maxt = f.max_thresholds;
         q.spilled = false; q.overflow = false;
         uint32_t status = 0;
-        if (count <= f.max_shapes) run_unit<true>(f, lane, unit, q, sidx, tt, status);
-        else run_unit<false>(f, lane, unit, q, sidx, tt, status);
+        if (count <= f.max_shapes) run_unit<true>(f, s, lane, unit, q, sidx, tt, status);
+        else run_unit<false>(f, s, lane, unit, q, sidx, tt, status);
         if (status) atomicOr(f.status, status);
+        if (f.scene_done && lane == 0)
+            atomicMax(f.scene_done + scene, (unsigned long long)wall_clock64());
         // a lane whose ring was too small hands the whole unit to the retry pass
         const bool spilled = __any(q.spilled);
         if (spilled && lane == 0 && !from_retry) {
@@ -968,6 +1103,15 @@ __global__ void reset_counters(uint32_t *counters, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) counters[i] = 0u;
+}
+
+// Start of a batch: clears the per-scene completion times and stamps the
+// start time in done[0].
+__global__ void batch_begin(unsigned long long *done, int n_scenes)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) done[0] = wall_clock64();
+    else if (i <= n_scenes) done[i] = 0ull;
 }
 
 } // namespace dev
@@ -1021,6 +1165,8 @@ struct gudni_ctx {
     DevBuf tiles, tile_job, shapes, xrange, work, out, out_f32;
     DevBuf fast_thr, fast_hdr, fast_sidx, slow_thr, slow_hdr, slow_sidx;
     DevBuf control;   // [0] fast counter [1] slow counter [2] status [3..] retry
+    DevBuf scene_table;           // single-frame scene descriptor
+    dev::SceneDesc single_scene;  // host copy of what scene_table holds
     DevBuf cycles;    // stage cycle counters of the fast pass (timing only)
     DevBuf dump;      // dump staging
     int32_t retry_capacity = 0;
@@ -1114,10 +1260,6 @@ static dev::Frame base_frame(gudni_ctx *ctx)
 {
     dev::Frame f;
     std::memset(&f, 0, sizeof(f));
-    f.geo = (const uint8_t *)ctx->geo.p;
-    f.substances = (const float4 *)ctx->subs.p;
-    f.pict = (const uint8_t *)ctx->pict.p;
-    f.uses = (const PictUse *)ctx->uses.p;
     f.random = (const float *)ctx->random.p;
     f.compute_depth = ceil_log2(ctx->spec.threads_per_tile);
     f.max_shapes = ctx->spec.max_shapes;
@@ -1127,6 +1269,41 @@ static dev::Frame base_frame(gudni_ctx *ctx)
     f.coverage_mode = ctx->spec.coverage_mode;
     f.threads_per_tile = ctx->spec.threads_per_tile;
     return f;
+}
+
+// The one-entry scene table of a single-frame launch; uploaded only when a
+// field changed since the last launch (normally never between frames).
+static int set_single_scene(gudni_ctx *ctx, const dev::SceneDesc &sd, dev::Frame &f)
+{
+    if (!ctx->scene_table.p || std::memcmp(&sd, &ctx->single_scene, sizeof(sd)) != 0) {
+        CHECK_HIP(ctx, ctx->scene_table.ensure(sizeof(dev::SceneDesc)));
+        ctx->single_scene = sd;
+        CHECK_HIP(ctx, hipMemcpyAsync(ctx->scene_table.p, &ctx->single_scene, sizeof(sd),
+                                      hipMemcpyHostToDevice, ctx->stream));
+    }
+    f.scenes = (const dev::SceneDesc *)ctx->scene_table.p;
+    f.tile_scene = nullptr;
+    f.scene_done = nullptr;
+    return GUDNI_OK;
+}
+
+static dev::SceneDesc single_desc(gudni_ctx *ctx, const float bg[4], int32_t w, int32_t h,
+                                  uint32_t *out, float *out_f32)
+{
+    dev::SceneDesc sd;
+    std::memset(&sd, 0, sizeof(sd));
+    sd.geo = (const uint8_t *)ctx->geo.p;
+    sd.shapes = (const ShapeRec *)ctx->shapes.p;
+    sd.bounds = (const float4 *)ctx->xrange.p;
+    sd.substances = (const float4 *)ctx->subs.p;
+    sd.pict = (const uint8_t *)ctx->pict.p;
+    sd.uses = (const PictUse *)ctx->uses.p;
+    sd.out = out;
+    sd.out_f32 = (float4 *)out_f32;
+    for (int i = 0; i < 4; ++i) sd.bg[i] = bg[i];
+    sd.width = w;
+    sd.height = h;
+    return sd;
 }
 
 // Both passes, stream-ordered.  The retry pass runs unconditionally; with no
@@ -1184,6 +1361,34 @@ static hipError_t upload(gudni_ctx *ctx, DevBuf &b, const T *src, size_t bytes)
     return hipSuccess;
 }
 
+// Host views of a scene's common buffers, for validation.
+struct CommonView {
+    const uint8_t *geo;
+    size_t geo_bytes;
+    const float4 *subs;
+    int32_t n_subs, n_uses;
+};
+
+static CommonView ctx_view(gudni_ctx *ctx)
+{
+    return CommonView{ctx->geo_host.data(), ctx->geo_host.size(), ctx->subs_host.data(),
+                      ctx->n_substances, ctx->n_uses};
+}
+
+static int validate_uses(gudni_ctx *ctx, const void *pict_uses, int32_t n_pict_uses,
+                         size_t pict_bytes)
+{
+    for (int32_t i = 0; i < n_pict_uses; ++i) {
+        PictUse u;
+        std::memcpy(&u, (const uint8_t *)pict_uses + 24 * (size_t)i, 24);
+        const int64_t need = (int64_t)u.mem_offset + 4 * (int64_t)u.w * (int64_t)u.h;
+        if (u.w < 0 || u.h < 0 || u.mem_offset < 0 || (u.mem_offset & 3) || need > (int64_t)pict_bytes)
+            return set_error(ctx, GUDNI_ERR_ARG, "picture usage " + std::to_string(i) +
+                                                     " outside the picture data");
+    }
+    return GUDNI_OK;
+}
+
 extern "C" {
 
 int gudni_create(const gudni_spec *spec, int device, gudni_ctx **out)
@@ -1232,7 +1437,7 @@ void gudni_destroy(gudni_ctx *ctx)
                       &ctx->tiles, &ctx->tile_job, &ctx->shapes, &ctx->xrange, &ctx->work, &ctx->out,
                       &ctx->out_f32, &ctx->fast_thr, &ctx->fast_hdr, &ctx->fast_sidx,
                       &ctx->slow_thr, &ctx->slow_hdr, &ctx->slow_sidx, &ctx->control, &ctx->dump,
-                      &ctx->cycles};
+                      &ctx->cycles, &ctx->scene_table};
     for (DevBuf *b : bufs) b->release();
     for (hipEvent_t &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -1264,13 +1469,9 @@ int gudni_upload_common(gudni_ctx *ctx, const void *geo, size_t geo_bytes, const
     if ((geo_bytes && !geo) || (n_substances && !substances) || (pict_bytes && !pict_data) ||
         (n_pict_uses && !pict_uses))
         return set_error(ctx, GUDNI_ERR_ARG, "gudni_upload_common: null buffer");
-    for (int32_t i = 0; i < n_pict_uses; ++i) {
-        PictUse u;
-        std::memcpy(&u, (const uint8_t *)pict_uses + 24 * (size_t)i, 24);
-        const int64_t need = (int64_t)u.mem_offset + 4 * (int64_t)u.w * (int64_t)u.h;
-        if (u.w < 0 || u.h < 0 || u.mem_offset < 0 || (u.mem_offset & 3) || need > (int64_t)pict_bytes)
-            return set_error(ctx, GUDNI_ERR_ARG, "gudni_upload_common: picture usage " +
-                                                     std::to_string(i) + " outside the picture data");
+    {
+        const int rc = validate_uses(ctx, pict_uses, n_pict_uses, pict_bytes);
+        if (rc) return rc;
     }
     ctx->geo_host.assign((const uint8_t *)geo, (const uint8_t *)geo + geo_bytes);
     ctx->subs_host.resize((size_t)n_substances);
@@ -1294,9 +1495,9 @@ int gudni_upload_common(gudni_ctx *ctx, const void *geo, size_t geo_bytes, const
 // geometry heap with even 8-byte sizes >= 4 (so float4 reads stay aligned),
 // substance ids inside the substance table and picture ids inside the usage
 // table.
-static int validate_job(gudni_ctx *ctx, const ShapeRec *shapes, int32_t n_shapes,
-                        const TileRec *tiles, int32_t n_tiles, int32_t w, int32_t h,
-                        std::vector<float> &xrange)
+static int validate_job(gudni_ctx *ctx, const CommonView &cv, const ShapeRec *shapes,
+                        int32_t n_shapes, const TileRec *tiles, int32_t n_tiles, int32_t w,
+                        int32_t h, std::vector<float> &xrange)
 {
     if (w <= 0 || h <= 0 || n_tiles < 0 || n_shapes < 0)
         return set_error(ctx, GUDNI_ERR_ARG, "bad job size");
@@ -1309,15 +1510,15 @@ static int validate_job(gudni_ctx *ctx, const ShapeRec *shapes, int32_t n_shapes
         if (r.box[0] < 0 || r.box[1] < 0)
             return set_error(ctx, GUDNI_ERR_ARG, "tile box has a negative origin");
     }
-    const size_t geo_bytes = ctx->geo_host.size();
+    const size_t geo_bytes = cv.geo_bytes;
     for (int32_t s = 0; s < n_shapes; ++s) {
         const ShapeRec &r = shapes[s];
         const uint64_t sid = r.tag & dev::kTagIdMask;
-        if (sid >= (uint64_t)ctx->n_substances)
+        if (sid >= (uint64_t)cv.n_subs)
             return set_error(ctx, GUDNI_ERR_ARG, "shape substance id outside the substance table");
         if ((r.tag & dev::kTagTypeMask) != dev::kTagSolid) {
-            const uint32_t pid = __builtin_bit_cast(uint32_t, ctx->subs_host[sid].x);
-            if (pid >= (uint32_t)ctx->n_uses)
+            const uint32_t pid = __builtin_bit_cast(uint32_t, cv.subs[sid].x);
+            if (pid >= (uint32_t)cv.n_uses)
                 return set_error(ctx, GUDNI_ERR_ARG, "picture substance refers to a missing usage");
         }
         size_t pos = 16u * (size_t)r.geo_start;
@@ -1331,10 +1532,10 @@ static int validate_job(gudni_ctx *ctx, const ShapeRec *shapes, int32_t n_shapes
             if (pos + 8 > geo_bytes)
                 return set_error(ctx, GUDNI_ERR_ARG, "strand header outside the geometry heap");
             uint16_t size;
-            std::memcpy(&size, ctx->geo_host.data() + pos, 2);
+            std::memcpy(&size, cv.geo + pos, 2);
             if (size < 4 || (size & 1) || pos + 8u * size > geo_bytes)
                 return set_error(ctx, GUDNI_ERR_ARG, "malformed strand in the geometry heap");
-            const float *pts = reinterpret_cast<const float *>(ctx->geo_host.data() + pos + 8);
+            const float *pts = reinterpret_cast<const float *>(cv.geo + pos + 8);
             const int npts = (int)size - 1;         // right, left, control, tree pairs
             if (pts[2] < xlo) xlo = pts[2];
             if (pts[0] > xhi) xhi = pts[0];
@@ -1363,7 +1564,8 @@ static int run_job(gudni_ctx *ctx, const void *shapes, int32_t n_shapes, const v
     if (!ctx) return GUDNI_ERR_ARG;
     if (!ctx->random.p) return set_error(ctx, GUDNI_ERR_ARG, "gudni_upload_common was not called");
     std::vector<float> xr(4 * (size_t)n_shapes + 4);
-    int rc = validate_job(ctx, (const ShapeRec *)shapes, n_shapes, (const TileRec *)tiles, n_tiles, w, h, xr);
+    int rc = validate_job(ctx, ctx_view(ctx), (const ShapeRec *)shapes, n_shapes,
+                          (const TileRec *)tiles, n_tiles, w, h, xr);
     if (rc) return rc;
     CHECK_HIP(ctx, hipSetDevice(ctx->device));
     rc = ensure_scratch(ctx);
@@ -1383,17 +1585,12 @@ static int run_job(gudni_ctx *ctx, const void *shapes, int32_t n_shapes, const v
     if (host_out)
         CHECK_HIP(ctx, hipMemcpyAsync(ctx->out.p, host_out, 4 * npix, hipMemcpyHostToDevice, ctx->stream));
     dev::Frame f = base_frame(ctx);
-    f.shapes = (const ShapeRec *)ctx->shapes.p;
-    f.bounds = (const float4 *)ctx->xrange.p;
     f.tiles = (const TileRec *)ctx->tiles.p;
     f.tile_job = (const int32_t *)ctx->tile_job.p;
     f.work = (const uint32_t *)ctx->work.p;
     f.n_work = (int32_t)work.size();
-    f.background = make_float4(bg[0], bg[1], bg[2], bg[3]);
-    f.width = w;
-    f.height = h;
-    f.out = (uint32_t *)ctx->out.p;
-    f.out_f32 = nullptr;
+    rc = set_single_scene(ctx, single_desc(ctx, bg, w, h, (uint32_t *)ctx->out.p, nullptr), f);
+    if (rc) return rc;
     size_t slots = 0, o_pre = 0, o_preh = 0, o_thr = 0, o_hdr = 0, o_q = 0, o_bits = 0, o_stack = 0,
            o_sidx = 0, total = 0;
     if (dump) {
@@ -1488,8 +1685,8 @@ int gudni_frame_upload(gudni_ctx *ctx, const gudni_jobs *jobs, int32_t width, in
         }
     }
     std::vector<float> xr(4 * (size_t)v.n_refs + 4);
-    int rc = validate_job(ctx, (const ShapeRec *)v.shape_refs, v.n_refs, tiles.data(), v.n_tiles,
-                          width, height, xr);
+    int rc = validate_job(ctx, ctx_view(ctx), (const ShapeRec *)v.shape_refs, v.n_refs,
+                          tiles.data(), v.n_tiles, width, height, xr);
     if (rc) return rc;
     CHECK_HIP(ctx, hipSetDevice(ctx->device));
     rc = ensure_scratch(ctx);
@@ -1519,18 +1716,13 @@ int gudni_frame_render(gudni_ctx *ctx, uint32_t *out_dev, float *out_f32_dev)
     if (!ctx || !out_dev) return GUDNI_ERR_ARG;
     if (!ctx->frame_ready) return set_error(ctx, GUDNI_ERR_ARG, "gudni_frame_upload was not called");
     dev::Frame f = base_frame(ctx);
-    f.shapes = (const ShapeRec *)ctx->shapes.p;
-    f.bounds = (const float4 *)ctx->xrange.p;
     f.tiles = (const TileRec *)ctx->tiles.p;
     f.tile_job = (const int32_t *)ctx->tile_job.p;
     f.work = (const uint32_t *)ctx->work.p;
     f.n_work = ctx->frame_n_work;
-    f.background = make_float4(ctx->background[0], ctx->background[1], ctx->background[2],
-                               ctx->background[3]);
-    f.width = ctx->frame_w;
-    f.height = ctx->frame_h;
-    f.out = out_dev;
-    f.out_f32 = (float4 *)out_f32_dev;
+    int rc0 = set_single_scene(ctx, single_desc(ctx, ctx->background, ctx->frame_w, ctx->frame_h,
+                                                out_dev, out_f32_dev), f);
+    if (rc0) return rc0;
     if (ctx->timing) {
         for (hipEvent_t &e : ctx->ev)
             if (!e) CHECK_HIP(ctx, hipEventCreate(&e));
@@ -1585,6 +1777,283 @@ int gudni_last_stage_cycles(gudni_ctx *ctx, uint64_t *cycles, int32_t n)
     if (!ctx || !cycles) return GUDNI_ERR_ARG;
     for (int i = 0; i < n && i < 4; ++i) cycles[i] = ctx->last_cycles[i];
     return GUDNI_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// Batches
+// ---------------------------------------------------------------------------
+struct gudni_batch {
+    int32_t n_scenes = 0;
+    int device = 0;
+    DevBuf geo, subs, pict, uses, shapes, bounds, tiles, tile_job, tile_scene, work, scenes, done,
+        frames, random;
+    std::vector<size_t> frame_off;
+    std::vector<int32_t> width, height;
+    std::vector<int32_t> work_begin, work_count;
+    int32_t n_work = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void *captured_control = nullptr;
+    hipStream_t capture_stream = nullptr;   // capture needs a stream other than the null stream
+};
+
+static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+int gudni_batch_upload(gudni_ctx *ctx, int32_t n_scenes, const gudni_batch_scene *scenes,
+                       const float *random_field, int32_t n_random, gudni_batch **out)
+{
+    if (!ctx || !scenes || !out || n_scenes <= 0) return GUDNI_ERR_ARG;
+    if (!random_field || n_random != ctx->spec.random_field_size)
+        return set_error(ctx, GUDNI_ERR_ARG, "gudni_batch_upload: random field must hold "
+                                             "random_field_size floats");
+    *out = nullptr;
+    std::vector<uint8_t> geo, pict, uses;
+    std::vector<float4> subs;
+    std::vector<ShapeRec> refs;
+    std::vector<float> bounds;
+    std::vector<TileRec> tiles;
+    std::vector<int32_t> tile_job, tile_scene, wbegin, wcount;
+    std::vector<uint32_t> work;
+    std::vector<size_t> geo_off, subs_off, pict_off, uses_off, refs_off, frame_off;
+    size_t frame_words = 0;
+    for (int32_t s = 0; s < n_scenes; ++s) {
+        const gudni_batch_scene &sc = scenes[s];
+        if (!sc.jobs || sc.width <= 0 || sc.height <= 0 || sc.n_substances < 0 ||
+            sc.n_pict_uses < 0 || (sc.geo_bytes && !sc.geo) || (sc.n_substances && !sc.substances) ||
+            (sc.pict_bytes && !sc.pict_data) || (sc.n_pict_uses && !sc.pict_uses))
+            return set_error(ctx, GUDNI_ERR_ARG, "gudni_batch_upload: bad scene " + std::to_string(s));
+        int rc = validate_uses(ctx, sc.pict_uses, sc.n_pict_uses, sc.pict_bytes);
+        if (rc) return rc;
+        gudni_jobs_view v;
+        gudni_jobs_view_get(sc.jobs, &v);
+        std::vector<float4> sub_host((size_t)sc.n_substances);
+        if (sc.n_substances) std::memcpy(sub_host.data(), sc.substances, 16u * (size_t)sc.n_substances);
+        const CommonView cv{(const uint8_t *)sc.geo, sc.geo_bytes, sub_host.data(), sc.n_substances,
+                            sc.n_pict_uses};
+        std::vector<TileRec> t(( const TileRec *)v.tiles, (const TileRec *)v.tiles + v.n_tiles);
+        std::vector<int32_t> tj((size_t)v.n_tiles);
+        for (int j = 0; j < v.n_jobs; ++j)
+            for (int i = v.job_tile_offset[j]; i < v.job_tile_offset[j + 1]; ++i) {
+                t[i].shape_start += v.job_shape_offset[j];
+                tj[i] = v.n_jobs - 1 - j;
+            }
+        std::vector<float> xr(4 * (size_t)v.n_refs + 4);
+        rc = validate_job(ctx, cv, (const ShapeRec *)v.shape_refs, v.n_refs, t.data(), v.n_tiles,
+                          sc.width, sc.height, xr);
+        if (rc) return set_error(ctx, rc, "scene " + std::to_string(s) + ": " + ctx->error);
+        geo_off.push_back(geo.size());
+        geo.insert(geo.end(), (const uint8_t *)sc.geo, (const uint8_t *)sc.geo + sc.geo_bytes);
+        geo.resize(align_up(geo.size(), 16));
+        subs_off.push_back(subs.size());
+        subs.insert(subs.end(), sub_host.begin(), sub_host.end());
+        pict_off.push_back(pict.size());
+        if (sc.pict_bytes)
+            pict.insert(pict.end(), (const uint8_t *)sc.pict_data, (const uint8_t *)sc.pict_data + sc.pict_bytes);
+        pict.resize(align_up(pict.size(), 16));
+        uses_off.push_back(uses.size() / 24);
+        if (sc.n_pict_uses)
+            uses.insert(uses.end(), (const uint8_t *)sc.pict_uses,
+                        (const uint8_t *)sc.pict_uses + 24 * (size_t)sc.n_pict_uses);
+        refs_off.push_back(refs.size());
+        refs.insert(refs.end(), (const ShapeRec *)v.shape_refs, (const ShapeRec *)v.shape_refs + v.n_refs);
+        bounds.insert(bounds.end(), xr.begin(), xr.begin() + 4 * (size_t)v.n_refs);
+        const int32_t tile_base = (int32_t)tiles.size();
+        std::vector<uint32_t> w;
+        build_work(ctx->spec, t.data(), v.n_tiles, sc.width, sc.height, w);
+        wbegin.push_back((int32_t)work.size());
+        wcount.push_back((int32_t)w.size());
+        for (uint32_t u : w) work.push_back(u + ((uint32_t)tile_base << 2));
+        tiles.insert(tiles.end(), t.begin(), t.end());
+        tile_job.insert(tile_job.end(), tj.begin(), tj.end());
+        tile_scene.insert(tile_scene.end(), (size_t)v.n_tiles, s);
+        frame_off.push_back(frame_words);
+        frame_words += (size_t)sc.width * sc.height;
+    }
+    if (tiles.size() >= (1u << 30))
+        return set_error(ctx, GUDNI_ERR_ARG, "gudni_batch_upload: too many tiles");
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    int rc = ensure_scratch(ctx);
+    if (rc) return rc;
+    rc = ensure_control(ctx, (int)work.size());
+    if (rc) return rc;
+    gudni_batch *b = new gudni_batch();
+    b->n_scenes = n_scenes;
+    b->device = ctx->device;
+    b->n_work = (int32_t)work.size();
+    b->frame_off = frame_off;
+    b->work_begin = wbegin;
+    b->work_count = wcount;
+    for (int32_t s = 0; s < n_scenes; ++s) {
+        b->width.push_back(scenes[s].width);
+        b->height.push_back(scenes[s].height);
+    }
+    auto up = [&](DevBuf &d, const void *src, size_t bytes) -> hipError_t {
+        hipError_t e = d.ensure(bytes);
+        if (e != hipSuccess) return e;
+        return bytes ? hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
+    };
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = up(b->geo, geo.data(), geo.size());
+    if (e == hipSuccess) e = up(b->subs, subs.data(), 16 * subs.size());
+    if (e == hipSuccess) e = up(b->pict, pict.data(), pict.size());
+    if (e == hipSuccess) e = up(b->uses, uses.data(), uses.size());
+    if (e == hipSuccess) e = up(b->shapes, refs.data(), 16 * refs.size());
+    if (e == hipSuccess) e = up(b->bounds, bounds.data(), 4 * bounds.size());
+    if (e == hipSuccess) e = up(b->tiles, tiles.data(), 32 * tiles.size());
+    if (e == hipSuccess) e = up(b->tile_job, tile_job.data(), 4 * tile_job.size());
+    if (e == hipSuccess) e = up(b->tile_scene, tile_scene.data(), 4 * tile_scene.size());
+    if (e == hipSuccess) e = up(b->work, work.data(), 4 * work.size());
+    if (e == hipSuccess) e = up(b->random, random_field, 4 * (size_t)n_random);
+    if (e == hipSuccess) e = b->frames.ensure(4 * frame_words);
+    if (e == hipSuccess) e = hipMemset(b->frames.p, 0, 4 * frame_words);
+    if (e == hipSuccess) e = b->done.ensure(8 * ((size_t)n_scenes + 1));
+    if (e == hipSuccess) e = hipMemset(b->done.p, 0, 8 * ((size_t)n_scenes + 1));
+    std::vector<dev::SceneDesc> descs((size_t)n_scenes);
+    for (int32_t s = 0; s < n_scenes && e == hipSuccess; ++s) {
+        dev::SceneDesc &d = descs[s];
+        std::memset(&d, 0, sizeof(d));
+        d.geo = (const uint8_t *)b->geo.p + geo_off[s];
+        d.shapes = (const ShapeRec *)b->shapes.p + refs_off[s];
+        d.bounds = (const float4 *)b->bounds.p + refs_off[s];
+        d.substances = (const float4 *)b->subs.p + subs_off[s];
+        d.pict = (const uint8_t *)b->pict.p + pict_off[s];
+        d.uses = (const PictUse *)b->uses.p + uses_off[s];
+        d.out = (uint32_t *)b->frames.p + frame_off[s];
+        d.out_f32 = nullptr;
+        for (int i = 0; i < 4; ++i) d.bg[i] = scenes[s].background[i];
+        d.width = scenes[s].width;
+        d.height = scenes[s].height;
+    }
+    if (e == hipSuccess) e = up(b->scenes, descs.data(), sizeof(dev::SceneDesc) * descs.size());
+    if (e != hipSuccess) {
+        ctx->error = std::string("gudni_batch_upload: ") + hipGetErrorString(e);
+        gudni_batch_free(b);
+        return GUDNI_ERR_HIP;
+    }
+    *out = b;
+    return GUDNI_OK;
+}
+
+static dev::Frame batch_frame(gudni_ctx *ctx, gudni_batch *b)
+{
+    dev::Frame f = base_frame(ctx);
+    f.scenes = (const dev::SceneDesc *)b->scenes.p;
+    f.tile_scene = (const int32_t *)b->tile_scene.p;
+    f.scene_done = (unsigned long long *)b->done.p + 1;
+    f.tiles = (const TileRec *)b->tiles.p;
+    f.tile_job = (const int32_t *)b->tile_job.p;
+    f.work = (const uint32_t *)b->work.p;
+    f.n_work = b->n_work;
+    f.random = (const float *)b->random.p;
+    return f;
+}
+
+static int batch_launch(gudni_ctx *ctx, gudni_batch *b)
+{
+    const int threads = 256, blocks = (b->n_scenes + 1 + threads - 1) / threads;
+    hipLaunchKernelGGL(dev::batch_begin, dim3(blocks), dim3(threads), 0, ctx->stream,
+                       (unsigned long long *)b->done.p, b->n_scenes);
+    CHECK_HIP(ctx, hipGetLastError());
+    return launch_passes(ctx, batch_frame(ctx, b));
+}
+
+int gudni_batch_render(gudni_ctx *ctx, gudni_batch *b, int use_graph)
+{
+    if (!ctx || !b) return GUDNI_ERR_ARG;
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    int rc = ensure_control(ctx, b->n_work);
+    if (rc) return rc;
+    ctx->timed_render = false;
+    if (!use_graph) return batch_launch(ctx, b);
+    if (b->exec && b->captured_control != ctx->control.p) {
+        (void)hipGraphExecDestroy(b->exec);
+        (void)hipGraphDestroy(b->graph);
+        b->exec = nullptr;
+        b->graph = nullptr;
+    }
+    if (!b->exec) {
+        // The launches are recorded on the batch's own stream and the graph is
+        // replayed on the context's stream, which may be the null stream.
+        if (!b->capture_stream)
+            CHECK_HIP(ctx, hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking));
+        CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        const hipStream_t user = ctx->stream;
+        ctx->stream = b->capture_stream;
+        hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+            ctx->stream = user;
+            CHECK_HIP(ctx, e);
+        }
+        rc = batch_launch(ctx, b);
+        hipGraph_t g = nullptr;
+        e = hipStreamEndCapture(ctx->stream, &g);
+        ctx->stream = user;
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        CHECK_HIP(ctx, e);
+        b->graph = g;
+        CHECK_HIP(ctx, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        b->captured_control = ctx->control.p;
+    }
+    CHECK_HIP(ctx, hipGraphLaunch(b->exec, ctx->stream));
+    return GUDNI_OK;
+}
+
+int gudni_batch_render_scene(gudni_ctx *ctx, gudni_batch *b, int32_t scene)
+{
+    if (!ctx || !b || scene < 0 || scene >= b->n_scenes) return GUDNI_ERR_ARG;
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    int rc = ensure_control(ctx, b->n_work);
+    if (rc) return rc;
+    ctx->timed_render = false;
+    dev::Frame f = batch_frame(ctx, b);
+    f.work = (const uint32_t *)b->work.p + b->work_begin[scene];
+    f.n_work = b->work_count[scene];
+    f.scene_done = nullptr;
+    return launch_passes(ctx, f);
+}
+
+int gudni_batch_check(gudni_ctx *ctx, gudni_batch *b, float *latency_ms)
+{
+    if (!ctx || !b) return GUDNI_ERR_ARG;
+    int rc = read_status(ctx);
+    if (rc) return rc;
+    if (latency_ms) {
+        std::vector<unsigned long long> done((size_t)b->n_scenes + 1);
+        CHECK_HIP(ctx, hipMemcpy(done.data(), b->done.p, 8 * done.size(), hipMemcpyDeviceToHost));
+        int rate_khz = 0;
+        CHECK_HIP(ctx, hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device));
+        if (rate_khz <= 0) rate_khz = 100000;
+        for (int32_t s = 0; s < b->n_scenes; ++s)
+            latency_ms[s] = done[s + 1] > done[0] ? (float)((double)(done[s + 1] - done[0]) / rate_khz) : 0.0f;
+    }
+    return GUDNI_OK;
+}
+
+int gudni_batch_frame(gudni_ctx *ctx, gudni_batch *b, int32_t scene, uint32_t *host_out)
+{
+    if (!ctx || !b || !host_out || scene < 0 || scene >= b->n_scenes) return GUDNI_ERR_ARG;
+    CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CHECK_HIP(ctx, hipMemcpy(host_out, (uint32_t *)b->frames.p + b->frame_off[scene],
+                             4 * (size_t)b->width[scene] * b->height[scene], hipMemcpyDeviceToHost));
+    return GUDNI_OK;
+}
+
+void gudni_batch_free(gudni_batch *b)
+{
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
+    DevBuf *bufs[] = {&b->geo, &b->subs, &b->pict, &b->uses, &b->shapes, &b->bounds, &b->tiles,
+                      &b->tile_job, &b->tile_scene, &b->work, &b->scenes, &b->done, &b->frames,
+                      &b->random};
+    for (DevBuf *d : bufs) d->release();
+    delete b;
 }
 
 } // extern "C"

@@ -185,6 +185,41 @@ int gudni_last_timings(gudni_ctx *ctx, float *ms, int32_t n);
  * [3] work units processed. */
 int gudni_last_stage_cycles(gudni_ctx *ctx, uint64_t *cycles, int32_t n);
 
+/* ---- Batches: many independent scenes in one persistent launch ----------- */
+/* No counterpart in the reference (it renders one scene per frame, one job
+ * per launch, CallKernels.hs:218-242).  Each scene brings the buffers
+ * gudni_upload_common takes plus its jobs; the batch concatenates them in
+ * device memory with one frame per scene, and gudni_batch_render runs every
+ * tile of every scene through the same kernels in one persistent launch
+ * (optionally replayed from a captured hipGraph). */
+typedef struct gudni_batch_scene {
+    const void *geo;          size_t geo_bytes;
+    const void *substances;   int32_t n_substances;
+    const void *pict_data;    size_t pict_bytes;
+    const void *pict_uses;    int32_t n_pict_uses;
+    const gudni_jobs *jobs;
+    int32_t width, height;
+    float background[4];
+} gudni_batch_scene;
+
+typedef struct gudni_batch gudni_batch;
+int  gudni_batch_upload(gudni_ctx *ctx, int32_t n_scenes, const gudni_batch_scene *scenes,
+                        const float *random_field, int32_t n_random, gudni_batch **out);
+/* Renders every scene (stream-ordered).  use_graph: on the first call the
+ * launch sequence is captured into a hipGraph on a stream the batch owns,
+ * then that graph is replayed on the context's stream (the null stream
+ * included) by this and every later call. */
+int  gudni_batch_render(gudni_ctx *ctx, gudni_batch *b, int use_graph);
+/* Renders one scene of the batch by itself (the sequential-launch baseline). */
+int  gudni_batch_render_scene(gudni_ctx *ctx, gudni_batch *b, int32_t scene);
+/* Waits, reports overflow/guard errors, and fills latency_ms[n_scenes]
+ * (may be NULL): device time from the start of the last gudni_batch_render
+ * to the end of each scene's last work unit. */
+int  gudni_batch_check(gudni_ctx *ctx, gudni_batch *b, float *latency_ms);
+/* Copies one scene's BGRA frame (width*height words) to host memory. */
+int  gudni_batch_frame(gudni_ctx *ctx, gudni_batch *b, int32_t scene, uint32_t *host_out);
+void gudni_batch_free(gudni_batch *b);
+
 /* Test hook: per-column threshold records after generate (queue order) and
  * after sort, plus the initial shape state, for one RasterJob.  Slot =
  * tile_in_job * threads_per_tile + column; `cap` records are kept per slot,

@@ -608,6 +608,9 @@ typedef struct {
     long guard;                     /* remaining section iterations */
     /* ShapeState */
     uint32_t shape_bits; int32_t *shape_indices; uint64_t stack[O_STACK_SECTIONS];
+    /* accurate mode on a tile of <= maxShapes shapes, all added, with
+     * distinct substances: colours by groups of 32 tile-local shapes */
+    int grouped;
 } col_t;
 
 static inline int    hdr_pos(uint32_t h)        { return (h & O_POSITIVE_SLOPE) != 0; }
@@ -911,6 +914,43 @@ static void read_color(const colorctx_t *cs, uint64_t sid, int solid, float *out
     } else { out[0] = out[1] = out[2] = out[3] = 0.0f; }
 }
 
+/* Accurate-mode colour of a grouped tile (see run_column): every visited
+ * shape composites its own colour, so the walk is a plain premultiplied
+ * "over" of the set shapes, evaluated as over(group 3, group 2, group 1,
+ * group 0, background) where group g holds tile-local shapes
+ * [32g, 32g+32) composited top-down. */
+static void grouped_color(col_t *c, const colorctx_t *cs, float *base)
+{
+    float P[3] = {0, 0, 0}, A = 0.0f;
+    for (int g = 3; g >= 0; g--) {
+        float gp[3] = {0, 0, 0}, ga = 0.0f;
+        for (int b = c->spec->max_shapes - 1; b >= 0; b--) {
+            if (!((c->stack[b >> 6] >> (b & 63)) & 1ull)) continue;
+            int n = c->shape_indices[b] - c->shape_start;
+            if ((n >> 5) != g) continue;
+            uint64_t tag; memcpy(&tag, cs->shapes + 16L * c->shape_indices[b], 8);
+            float col[4];
+            read_color(cs, tag & TAG_SUBSTANCEID_MASK, (tag & TAG_SUBSTANCETYPE_MASK) == TAG_SOLID, col);
+            float pm[4] = { col[0] * col[3], col[1] * col[3], col[2] * col[3], col[3] };
+            float t = 1.0f - ga;
+            for (int i = 0; i < 3; i++) gp[i] = gp[i] + pm[i] * t;
+            ga = ga + pm[3] * t;
+        }
+        float t = 1.0f - A;
+        for (int i = 0; i < 3; i++) P[i] = P[i] + gp[i] * t;
+        A = A + ga * t;
+    }
+    {
+        float pm[4] = { cs->bg[0] * cs->bg[3], cs->bg[1] * cs->bg[3], cs->bg[2] * cs->bg[3], cs->bg[3] };
+        float t = 1.0f - A;
+        for (int i = 0; i < 3; i++) P[i] = P[i] + pm[i] * t;
+        A = A + pm[3] * t;
+    }
+    if (A > 0.0f) { for (int i = 0; i < 3; i++) base[i] = P[i] / A; }
+    else { base[0] = base[1] = base[2] = 0.0f; }
+    base[3] = A;
+}
+
 /* determineColor (Kernels.cl:1447-1513).
  * coverage_mode 1 composites as the reference writes it: every step
  * un-premultiplies (composite, Kernels.cl:878-887).  coverage_mode 0 keeps the
@@ -974,6 +1014,7 @@ static void determine_color(col_t *c, const colorctx_t *cs, float *base, double 
         if (A > 0.0f) { for (int i = 0; i < 3; i++) base[i] = P[i] / A; }
         else { base[0] = base[1] = base[2] = 0.0f; }
         base[3] = A;
+        if (c->grouped) grouped_color(c, cs, base);
     }
     if (dA > 0.0) { for (int i = 0; i < 3; i++) dbase[i] = dP[i] / dA; }
     else { dbase[0] = dbase[1] = dbase[2] = 0.0; }
@@ -1249,6 +1290,19 @@ static int run_column(const go_spec *sp, const go_job_in *in, int tile, int colu
         dump->shape_stack[2 * slot + 1] = c.stack[1];
         memcpy(dump->shape_indices + (long)slot * sp->max_shapes, c.shape_indices,
                sizeof(int32_t) * (size_t)sp->max_shapes);
+    }
+    {   /* grouped evaluation: accurate mode, <= maxShapes shapes, all added,
+         * distinct substance ids */
+        int ok = sp->coverage_mode == 0 && c.num_shapes <= sp->max_shapes;
+        for (int a = 0; ok && a < c.num_shapes; a++) {
+            uint64_t ta; memcpy(&ta, in->shapes + 16L * (c.shape_start + a), 8);
+            if ((ta & TAG_COMPOUND_MASK) != TAG_ADD) ok = 0;
+            for (int b2 = a + 1; ok && b2 < c.num_shapes; b2++) {
+                uint64_t tb; memcpy(&tb, in->shapes + 16L * (c.shape_start + b2), 8);
+                if ((ta & TAG_SUBSTANCEID_MASK) == (tb & TAG_SUBSTANCEID_MASK)) ok = 0;
+            }
+        }
+        c.grouped = ok;
     }
     colorctx_t cs;
     cs.substances = in->substances; cs.pict = in->pict_data; cs.uses = in->pict_uses;
