@@ -224,6 +224,62 @@ class Rasterizer:
     def frame_check(self):
         self._check(self.L.gudni_frame_check(self.ctx))
 
+    # Level 2 with stage (a) on the device ------------------------------------
+    def _scene_desc(self, geometry: Geometry, scene, rnd):
+        rnd = random_field(self.spec.random_field_size) if rnd is None else rnd
+        arrs = [np.ascontiguousarray(geometry.geo, np.uint8),
+                np.ascontiguousarray(geometry.shapes, np.uint8),
+                np.ascontiguousarray(geometry.bbox, np.float32),
+                np.ascontiguousarray(scene.substances, np.float32),
+                np.ascontiguousarray(scene.pict_data, np.uint8),
+                np.ascontiguousarray(scene.pict_uses, np.uint8),
+                np.ascontiguousarray(rnd, np.float32)]
+        geo, shapes, bbox, subs, pict, uses, rnd = arrs
+        d = _lib.SceneDesc(_ptr(geo) if geo.size else None, geo.size,
+                           _ptr(shapes) if shapes.size else None, shapes.size // 16,
+                           _ptr(bbox) if bbox.size else None, _ptr(subs) if subs.size else None,
+                           subs.shape[0], _ptr(pict) if pict.size else None, pict.size,
+                           _ptr(uses) if uses.size else None, uses.shape[0], _ptr(rnd), rnd.size,
+                           scene.width, scene.height,
+                           (C.c_float * 4)(*[float(x) for x in scene.background]))
+        return d, arrs
+
+    def scene_upload(self, geometry: Geometry, scene, rnd: np.ndarray | None = None):
+        """Scene buffers for device binning: strand heap, shape records and boxes."""
+        d, arrs = self._scene_desc(geometry, scene, rnd)
+        self._check(self.L.gudni_scene_upload(self.ctx, C.byref(d)))
+
+    def scene_bin(self):
+        """Stage (a) on the device; the frame is then ready for frame_render."""
+        self._check(self.L.gudni_scene_bin(self.ctx))
+
+    def scene_counts(self):
+        c = np.zeros(3, np.int32)
+        self._check(self.L.gudni_scene_counts(self.ctx, _ptr(c)))
+        return int(c[0]), int(c[1]), int(c[2])
+
+    def scene_read(self):
+        """The device-built tiles (shapeStart absolute), reversed job index per
+        tile, shape references, their extents and the work list."""
+        nt, nr, nw = self.scene_counts()
+        out = {"tiles": np.zeros(nt * 32, np.uint8), "tile_job": np.zeros(nt, np.int32),
+               "refs": np.zeros(nr * 16, np.uint8), "ref_bounds": np.zeros((nr, 4), np.float32),
+               "work": np.zeros(nw, np.uint32)}
+        self._check(self.L.gudni_scene_read(
+            self.ctx, *[_ptr(out[k]) if out[k].size else None
+                        for k in ("tiles", "tile_job", "refs", "ref_bounds", "work")]))
+        return out
+
+    def last_bin_ms(self) -> float:
+        return float(self.L.gudni_last_bin_ms(self.ctx))
+
+    def raster_scene(self, geometry: Geometry, scene, rnd: np.ndarray | None = None) -> np.ndarray:
+        """Upload, bin on the device, render, read back: W*H BGRA words."""
+        d, arrs = self._scene_desc(geometry, scene, rnd)
+        out = np.zeros(scene.width * scene.height, np.uint32)
+        self._check(self.L.gudni_raster_scene(self.ctx, C.byref(d), _ptr(out)))
+        return out
+
     def enable_timing(self, on=True):
         self._check(self.L.gudni_enable_timing(self.ctx, 1 if on else 0))
 

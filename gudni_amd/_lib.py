@@ -50,6 +50,15 @@ class BatchScene(C.Structure):
                 ("width", C.c_int32), ("height", C.c_int32), ("background", C.c_float * 4)]
 
 
+class SceneDesc(C.Structure):
+    _fields_ = [("geo", C.c_void_p), ("geo_bytes", C.c_size_t), ("shapes", C.c_void_p),
+                ("n_shapes", C.c_int32), ("bbox", C.c_void_p), ("substances", C.c_void_p),
+                ("n_substances", C.c_int32), ("pict_data", C.c_void_p), ("pict_bytes", C.c_size_t),
+                ("pict_uses", C.c_void_p), ("n_pict_uses", C.c_int32), ("random_field", C.c_void_p),
+                ("n_random", C.c_int32), ("width", C.c_int32), ("height", C.c_int32),
+                ("background", C.c_float * 4)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "gudni_default_spec": (None, [C.POINTER(Spec)]),
@@ -87,6 +96,13 @@ SIGNATURES = {
     "gudni_batch_check": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "gudni_batch_frame": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "gudni_batch_free": (None, [C.c_void_p]),
+    "gudni_scene_upload": (C.c_int, [C.c_void_p, C.POINTER(SceneDesc)]),
+    "gudni_scene_bin": (C.c_int, [C.c_void_p]),
+    "gudni_scene_counts": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "gudni_scene_read": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_void_p]),
+    "gudni_last_bin_ms": (C.c_float, [C.c_void_p]),
+    "gudni_raster_scene": (C.c_int, [C.c_void_p, C.POINTER(SceneDesc), C.c_void_p]),
 }
 
 _lib = None

@@ -1,0 +1,125 @@
+// context.h -- host-side state of a libgudni_hip context, shared by the
+// translation units that implement the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "gudni_hip.h"
+#include "internal.h"
+#include "device.h"
+
+namespace gudni {
+
+// A device allocation that only grows.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t n)
+    {
+        if (n <= bytes && p) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        const size_t want = n ? n : 16;
+        hipError_t e = hipMalloc(&p, want);
+        if (e == hipSuccess) bytes = want;
+        return e;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+} // namespace gudni
+
+struct gudni_adjoint;
+
+struct gudni_ctx {
+    gudni_spec spec;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string error;
+    int cus = 0;
+    int fast_blocks = 0, slow_blocks = 0;
+    // common buffers
+    gudni::DevBuf geo, subs, pict, uses, random;
+    int32_t n_substances = 0, n_uses = 0;
+    // host copies of the common buffers, used to bounds-check job records
+    std::vector<uint8_t> geo_host;
+    std::vector<float4> subs_host;
+    // per launch buffers
+    gudni::DevBuf tiles, tile_job, shapes, xrange, work, out, out_f32;
+    gudni::DevBuf fast_thr, fast_hdr, fast_sidx, slow_thr, slow_hdr, slow_sidx;
+    gudni::DevBuf control;   // [0] fast counter [1] slow counter [2] status [3..] retry
+    gudni::DevBuf scene_table;           // single-frame scene descriptor
+    gudni::dev::SceneDesc single_scene;  // host copy of what scene_table holds
+    gudni::DevBuf cycles;    // stage cycle counters of the fast pass (timing only)
+    gudni::DevBuf dump;      // dump staging
+    int32_t retry_capacity = 0;
+    // uploaded frame (Level 2)
+    int32_t frame_w = 0, frame_h = 0, frame_n_work = 0, frame_n_tiles = 0, frame_index = 0;
+    float background[4] = {1, 1, 1, 1};
+    bool frame_ready = false;
+    // Level 2 scene: shapes in traversal order, binned on the device
+    gudni::DevBuf scene_shapes, scene_bbox, scene_bounds, bin_ws;
+    int32_t scene_n_shapes = 0, scene_w = 0, scene_h = 0;
+    float scene_bg[4] = {1, 1, 1, 1};
+    int32_t bin_cap_pairs = 0, bin_cap_nodes = 0, scene_n_refs = 0;
+    bool scene_ready = false;
+    hipEvent_t bin_ev[2] = {nullptr, nullptr};
+    float last_bin_ms = 0;
+    // timing
+    bool timing = false, timed_render = false;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    float last_ms[2] = {0, 0};
+    unsigned long long last_cycles[4] = {0, 0, 0, 0};
+};
+
+#define CHECK_HIP(ctx, expr)                                                        \
+    do {                                                                            \
+        hipError_t e_ = (expr);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            (ctx)->error = std::string(#expr) + ": " + hipGetErrorString(e_);      \
+            return GUDNI_ERR_HIP;                                                   \
+        }                                                                           \
+    } while (0)
+
+namespace gudni {
+
+inline int set_error(gudni_ctx *ctx, int code, const std::string &msg)
+{
+    ctx->error = msg;
+    return code;
+}
+
+template <typename T>
+inline hipError_t upload(gudni_ctx *ctx, DevBuf &b, const T *src, size_t bytes)
+{
+    hipError_t e = b.ensure(bytes);
+    if (e != hipSuccess) return e;
+    if (bytes && src) return hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+    return hipSuccess;
+}
+
+// Host copies of the common buffers, for record validation.
+struct CommonView {
+    const uint8_t *geo;
+    size_t geo_bytes;
+    const float4 *subs;
+    int32_t n_subs, n_uses;
+};
+
+CommonView ctx_view(gudni_ctx *ctx);
+// Bounds-checks shape records (and tiles) against the common buffers and
+// fills each shape's strand extent (xmin, ymin, xmax, ymax).
+int validate_job(gudni_ctx *ctx, const CommonView &cv, const ShapeRec *shapes, int32_t n_shapes,
+                 const TileRec *tiles, int32_t n_tiles, int32_t w, int32_t h,
+                 std::vector<float> &xrange);
+
+} // namespace gudni

@@ -175,6 +175,46 @@ int gudni_frame_render(gudni_ctx *ctx, uint32_t *out_dev, float *out_f32_dev);
 /* Waits for the stream and reports a queue overflow or guard hit seen by the
  * last render. */
 int gudni_frame_check(gudni_ctx *ctx);
+
+/* ---- Level 2: a whole scene, binned into tiles on the device --------------
+ * Replaces buildRasterJobs + queueRasterJobs (OpenCL/CallKernels.hs:218-255)
+ * and the tile tree they build (Raster/TileTree.hs:81-204, Raster/Job.hs:
+ * 132-179).  The scene holds what the serialiser produces (Raster/
+ * Serialize.hs:148-177): the strand heap, one 16-byte Shape record per
+ * ShapeEntry in traversal order, and each entry's box (left, top, right,
+ * bottom; Raster/Enclosure.hs:62-73).  The device build yields the same
+ * tiles, shape references (newest first), job numbering and column
+ * allocations as the host build (gudni_jobs_build + gudni_frame_upload). */
+typedef struct gudni_scene {
+    const void *geo;            size_t geo_bytes;
+    const void *shapes;         int32_t n_shapes;      /* 16 B Shape records   */
+    const float *bbox;                                 /* 4 floats per shape   */
+    const void *substances;     int32_t n_substances;
+    const void *pict_data;      size_t pict_bytes;
+    const void *pict_uses;      int32_t n_pict_uses;
+    const float *random_field;  int32_t n_random;
+    int32_t width, height;
+    float background[4];
+} gudni_scene;
+/* Uploads and validates the scene (the common buffers of gudni_upload_common
+ * plus the shape records and boxes). */
+int   gudni_scene_upload(gudni_ctx *ctx, const gudni_scene *scene);
+/* Stage (a) on the device: tile tree, per-tile shape references, job fields
+ * and the work list, then the frame is ready for gudni_frame_render. */
+int   gudni_scene_bin(gudni_ctx *ctx);
+/* counts[0] tiles, counts[1] shape references, counts[2] work units. */
+int   gudni_scene_counts(gudni_ctx *ctx, int32_t counts[3]);
+/* Reads back the device build (any pointer may be NULL): tiles (32 B each,
+ * shapeStart absolute into the references), tile_job (reversed job index per
+ * tile), references (16 B each), their strand extents (4 floats) and the work
+ * list (tile << 2 | quarter). */
+int   gudni_scene_read(gudni_ctx *ctx, void *tiles, int32_t *tile_job, void *refs,
+                       float *ref_bounds, uint32_t *work);
+/* Milliseconds of the last gudni_scene_bin on the device (timing enabled). */
+float gudni_last_bin_ms(const gudni_ctx *ctx);
+/* Upload + bin + render + read back, synchronously: the scene's W*H BGRA
+ * words into host_out. */
+int   gudni_raster_scene(gudni_ctx *ctx, const gudni_scene *scene, uint32_t *host_out);
 /* Pass timings (ms) of the last render, recorded with hipEvents on the
  * launch stream when enabled and read by gudni_frame_check: [0] counter
  * reset + fast pass, [1] retry pass. */

@@ -63,3 +63,33 @@ def tiny_square():
     (0.1, 0.1) over white."""
     return S.build_scene(8, 8, [(S.solid_tag(0), [S.rectangle_pairs(0.1, 0.1, 2.1, 2.1)])],
                          [[1, 0, 0, 1]], name="tinySquare")
+
+
+TILE_DT = np.dtype([("box", "<i4", 4), ("hd", "<i2"), ("vd", "<i2"), ("col", "<i4"),
+                    ("start", "<i4"), ("count", "<i4")])
+
+
+def host_frame_layout(spec, jobs, width, height):
+    """The host build laid out as a whole frame, the way gudni_frame_upload
+    does it: shapeStart absolute into the concatenated references, reversed job
+    index per tile, and the work list of active tile quarters."""
+    tiles = jobs.tiles.view(TILE_DT).copy()
+    tile_job = np.zeros(jobs.n_tiles, np.int32)
+    for j in range(jobs.n_jobs):
+        t0, t1 = jobs.job_tile_offset[j], jobs.job_tile_offset[j + 1]
+        tiles["start"][t0:t1] += jobs.job_shape_offset[j]
+        tile_job[t0:t1] = jobs.n_jobs - 1 - j
+    cd = int(spec.threads_per_tile).bit_length() - 1
+    work = []
+    lanes = np.arange(64)
+    for t, r in enumerate(tiles):
+        hd, vd = int(r["hd"]), int(r["vd"])
+        dd = max(0, vd - (cd - hd))
+        for q in range(spec.threads_per_tile // 64):
+            col = q * 64 + lanes
+            ix = ((1 << hd) - 1) & col
+            iy = (col >> hd) << dd
+            box = r["box"]
+            if np.any((iy < box[3] - box[1]) & (ix + box[0] < width) & (iy + box[1] < height)):
+                work.append((t << 2) | q)
+    return tiles, tile_job, jobs.shape_refs.copy(), np.array(work, np.uint32)
