@@ -71,6 +71,49 @@ def cpu_baseline(scene, seconds_target=15.0):
                       f"{dt:.1f} s"}
 
 
+def measured_traffic(kernel_ms):
+    """HBM bytes per fast-pass launch from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the
+    counter reads for a plain copy of known size)."""
+    path = os.path.join(ROOT, "profiles", "raster_pass_cfg2_4096_traffic.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as fh:
+        t = json.load(fh)
+    return float(t["traffic_bytes"]), {
+        "source": "profiles/raster_pass_cfg2_4096_counters.md",
+        "fetch_bytes": t["fetch_kib"] * 1024 * t["fetch_factor"],
+        "write_bytes": t["write_kib"] * 1024 * t["write_factor"],
+        "profiled_kernel_ms": t["fast_pass_avg_ms"], "this_run_kernel_ms": kernel_ms,
+        "lds_bank_conflict_rate": t["lds_bank_conflict_rate"]}
+
+
+def device_binning_leg(r, scene, geo, stream, steps):
+    """Stage (a) on the device for the same frame: gudni_scene_bin (tile tree,
+    references, jobs, work list) timed with HIP events, then bin + render."""
+    import torch
+    r.scene_upload(geo, scene)
+    r.enable_timing(True)
+    bin_ms = []
+    for _ in range(max(3, steps)):
+        r.scene_bin()
+        bin_ms.append(r.last_bin_ms())
+    r.enable_timing(False)
+    out = torch.zeros(scene.width * scene.height, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        r.scene_bin()
+        r.frame_render(out.data_ptr())
+    r.frame_check()
+    torch.cuda.synchronize()
+    both = (time.perf_counter() - t) / steps * 1e3
+    nt, nr, nw = r.scene_counts()
+    return {"bin_ms": float(np.median(bin_ms)), "bin_plus_render_ms": both,
+            "bin_plus_render_gpixels_per_s": scene.width * scene.height / (both * 1e-3) / 1e9,
+            "tiles": nt, "refs": nr, "work_units": nw}
+
+
 def batched_leg(spec, steps, log_fn):
     """The batched workload: 256 scenes of 256^2..2048^2 with 100..50k shapes,
     tile size per scene-size group chosen by the autotuner, all scenes in one
@@ -226,6 +269,10 @@ def main():
     gpix = npix / (ms * 1e-3) / 1e9
     mcurves = scene.n_curves / (ms * 1e-3) / 1e6
 
+    log("device binning (stage a) of the same frame")
+    binning = device_binning_leg(r, scene, geo, stream, max(3, min(args.steps, 10)))
+    traffic, traffic_info = measured_traffic(fast_ms)
+
     batched = None
     if not args.no_batch:
         log("batched workload (256 scenes)")
@@ -247,10 +294,12 @@ def main():
                    "curves": scene.n_curves, "tiles": jobs.n_tiles, "tile_shape_refs": jobs.n_refs,
                    "coverage_mode": "accurate"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_detail": traffic_info,
                      "kernel": "raster_pass (fast pass)", "kernel_ms": fast_ms,
                      "algorithmic_bytes": b_alg, "retry_pass_ms": float(np.mean(retry)),
                      "stage_cycle_share": stage_share},
+        "device_binning": binning,
         "batched": batched,
         "cpu_baseline": cpu,
         "host_stages_s": {"serialize_and_bin": t_host, "upload": t_upload, "readback": t_read},

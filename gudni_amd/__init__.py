@@ -280,6 +280,10 @@ class Rasterizer:
         self._check(self.L.gudni_raster_scene(self.ctx, C.byref(d), _ptr(out)))
         return out
 
+    def set_blocks_per_cu(self, n: int):
+        """Fast-pass blocks per CU (0 = as many as fit)."""
+        self._check(self.L.gudni_set_blocks_per_cu(self.ctx, int(n)))
+
     def enable_timing(self, on=True):
         self._check(self.L.gudni_enable_timing(self.ctx, 1 if on else 0))
 
@@ -357,6 +361,75 @@ class Batch:
     def close(self):
         if self.handle:
             self.r.L.gudni_batch_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Adjoint:
+    """Gradients of the L2 image loss for one or more scenes (one launch runs
+    them all).  items: (scene, target) pairs, target an (H, W, 4) float32
+    image; the scene's outline pairs and substance colours are the
+    parameters."""
+
+    def __init__(self, rasterizer: Rasterizer, items):
+        self.r = rasterizer
+        self.L = rasterizer.L
+        self.scenes = [sc for sc, _ in items]
+        self._keep = []
+        descs = (_lib.AdjointScene * len(items))()
+        for i, (sc, target) in enumerate(items):
+            arrs = [np.ascontiguousarray(sc.tags, np.uint64),
+                    np.ascontiguousarray(sc.shape_outline_offsets, np.int32),
+                    np.ascontiguousarray(sc.outline_pair_offsets, np.int32),
+                    np.ascontiguousarray(sc.pairs, np.float32),
+                    np.ascontiguousarray(sc.substances, np.float32),
+                    np.ascontiguousarray(sc.pict_data, np.uint8),
+                    np.ascontiguousarray(sc.pict_uses, np.uint8),
+                    np.ascontiguousarray(target, np.float32)]
+            assert arrs[7].shape == (sc.height, sc.width, 4)
+            self._keep.append(arrs)
+            tags, soff, ooff, pairs, subs, pict, uses, tgt = arrs
+            o = _lib.Outlines(sc.n_shapes, *[a.ctypes.data for a in (tags, soff, ooff, pairs)])
+            descs[i] = _lib.AdjointScene(o, _ptr(subs) if subs.size else None, subs.shape[0],
+                                         _ptr(pict) if pict.size else None, pict.size,
+                                         _ptr(uses) if uses.size else None, uses.size // 24,
+                                         _ptr(tgt), sc.width, sc.height,
+                                         (C.c_float * 4)(*[float(x) for x in sc.background]))
+        self.handle = C.c_void_p()
+        self.r._check(self.L.gudni_adjoint_upload(self.r.ctx, len(items), descs, C.byref(self.handle)))
+
+    def update(self, i, pairs=None, rgba=None):
+        """New parameters for scene i (re-serialised and re-binned)."""
+        p = None if pairs is None else np.ascontiguousarray(pairs, np.float32)
+        c = None if rgba is None else np.ascontiguousarray(rgba, np.float32)
+        self.r._check(self.L.gudni_adjoint_update(self.r.ctx, self.handle, i, _ptr(p), _ptr(c)))
+
+    def run(self):
+        self.r._check(self.L.gudni_adjoint_run(self.r.ctx, self.handle))
+
+    def run_scene(self, i):
+        self.r._check(self.L.gudni_adjoint_run_scene(self.r.ctx, self.handle, i))
+
+    def result(self, i, image=False):
+        """(loss, d_pairs (n_pairs, 4), d_rgba (n_substances, 4), image or None)."""
+        sc = self.scenes[i]
+        loss = C.c_double()
+        dp = np.zeros((sc.pairs.size // 4, 4), np.float32)
+        dc = np.zeros((len(sc.substances), 4), np.float32)
+        img = np.zeros((sc.height, sc.width, 4), np.float32) if image else None
+        self.r._check(self.L.gudni_adjoint_result(self.r.ctx, self.handle, i, C.byref(loss),
+                                                  _ptr(dp) if dp.size else None,
+                                                  _ptr(dc) if dc.size else None, _ptr(img)))
+        return loss.value, dp, dc, img
+
+    def close(self):
+        if self.handle:
+            self.L.gudni_adjoint_free(self.handle)
             self.handle = None
 
     def __del__(self):

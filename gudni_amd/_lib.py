@@ -59,6 +59,13 @@ class SceneDesc(C.Structure):
                 ("background", C.c_float * 4)]
 
 
+class AdjointScene(C.Structure):
+    _fields_ = [("outlines", Outlines), ("rgba", C.c_void_p), ("n_substances", C.c_int32),
+                ("pict_data", C.c_void_p), ("pict_bytes", C.c_size_t), ("pict_uses", C.c_void_p),
+                ("n_pict_uses", C.c_int32), ("target", C.c_void_p), ("width", C.c_int32),
+                ("height", C.c_int32), ("background", C.c_float * 4)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "gudni_default_spec": (None, [C.POINTER(Spec)]),
@@ -85,6 +92,7 @@ SIGNATURES = {
     "gudni_frame_render": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "gudni_frame_check": (C.c_int, [C.c_void_p]),
     "gudni_enable_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "gudni_set_blocks_per_cu": (C.c_int, [C.c_void_p, C.c_int32]),
     "gudni_last_timings": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
     "gudni_last_stage_cycles": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
     "gudni_dump_job": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
@@ -103,6 +111,16 @@ SIGNATURES = {
                                    C.c_void_p]),
     "gudni_last_bin_ms": (C.c_float, [C.c_void_p]),
     "gudni_raster_scene": (C.c_int, [C.c_void_p, C.POINTER(SceneDesc), C.c_void_p]),
+    "gudni_adjoint_upload": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(AdjointScene),
+                                       C.POINTER(C.c_void_p)]),
+    "gudni_adjoint_update": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    "gudni_adjoint_run": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "gudni_adjoint_run_scene": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
+    "gudni_adjoint_result": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_void_p]),
+    "gudni_adjoint_free": (None, [C.c_void_p]),
+    "gudni_adjoint": (C.c_int, [C.c_void_p, C.POINTER(AdjointScene), C.c_void_p, C.c_void_p,
+                                C.c_void_p, C.c_void_p]),
 }
 
 _lib = None

@@ -38,7 +38,6 @@ struct DevBuf {
 
 } // namespace gudni
 
-struct gudni_adjoint;
 
 struct gudni_ctx {
     gudni_spec spec;
@@ -47,6 +46,7 @@ struct gudni_ctx {
     std::string error;
     int cus = 0;
     int fast_blocks = 0, slow_blocks = 0;
+    int blocks_per_cu = 0;        // fast-pass blocks per CU (0: as many as fit, at most 4)
     // common buffers
     gudni::DevBuf geo, subs, pict, uses, random;
     int32_t n_substances = 0, n_uses = 0;
@@ -116,6 +116,18 @@ struct CommonView {
 };
 
 CommonView ctx_view(gudni_ctx *ctx);
+// Active (tile, quarter) work units of a tile list (isActiveThread).
+void build_work(const gudni_spec &spec, const TileRec *tiles, int n_tiles, int width, int height,
+                std::vector<uint32_t> &work);
+// Scratch rings of the raster passes, sized on first use.
+int ensure_scratch(gudni_ctx *ctx);
+// Pass counters, status word and retry list for n_work units.
+int ensure_control(gudni_ctx *ctx, int n_work);
+// A Frame with the context's spec and buffers; the caller sets the rest.
+dev::Frame base_frame(gudni_ctx *ctx);
+// Waits for the stream and turns the status word into an error code.
+int read_status(gudni_ctx *ctx);
+int validate_uses(gudni_ctx *ctx, const void *pict_uses, int32_t n_pict_uses, size_t pict_bytes);
 // Bounds-checks shape records (and tiles) against the common buffers and
 // fills each shape's strand extent (xmin, ymin, xmax, ymax).
 int validate_job(gudni_ctx *ctx, const CommonView &cv, const ShapeRec *shapes, int32_t n_shapes,

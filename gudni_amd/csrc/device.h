@@ -28,6 +28,12 @@ constexpr uint32_t kPersistTop = 0xC0000000u;
 constexpr uint32_t kPersistBottom = 0x40000000u;
 constexpr uint32_t kUnpersist = 0xBFFFFFFFu;
 constexpr uint32_t kShapeBitMask = 0x0FFFFFFFu;
+// Shape labels are below 128 (maxShapes <= 127), so only the low seven bits
+// of the shape-bit field carry them; the adjoint keeps a record's origin in
+// the bits above.
+constexpr uint32_t kLabelMask = 0x7Fu;
+constexpr int kOriginShift = 7;
+constexpr uint32_t kOriginMask = 0x3FFFFFu;   // 22 bits: bits 7..28
 constexpr float kMinCrop = 0.2f;          // MINCROP (Kernels.cl:47)
 constexpr float kFlatness = 0.25f;        // TAXICAB_FLATNESS (Kernels.cl:250)
 
@@ -46,6 +52,11 @@ constexpr long kSectionGuard = 1L << 22;  // sweep-length bound per column: the
 constexpr int kMaxBisect = 64;            // bifurcation bound per intersection
 
 // status codes a lane can raise (bit flags in the launch's status word)
+// Control words of a launch: per-part work counters of the fast pass, the
+// retry pass's counter, the status word, the retry list (count, units).
+constexpr int kParts = 8;                  // one work range per XCD
+constexpr int kCtlFast = 0, kCtlSlow = kParts, kCtlStatus = kParts + 1, kCtlRetry = kParts + 2;
+constexpr int kCtlWords = kCtlRetry + 1;   // words before the retry entries
 constexpr uint32_t kStatusOverflow = 1u;   // live queue > maxThresholds
 constexpr uint32_t kStatusGuard = 2u;      // section guard hit
 
@@ -141,7 +152,8 @@ struct Frame {
     int32_t *shape_idx;           // [slots][max_shapes][64]
     int32_t cap_log2;
     // control
-    uint32_t *counter;            // work counter for this pass
+    uint32_t *counter;            // work counters of this pass, one per part
+    int32_t n_parts;              // parts the work list is cut into
     uint32_t *status;             // OR of lane status flags
     uint32_t *retry;              // [0] = count, [1..] = abandoned work units
     int32_t retry_capacity;
@@ -187,6 +199,44 @@ struct Queue {
         return true;
     }
     __device__ __forceinline__ void pop() { head = (head + 1) & mask; --len; }
+};
+
+// Work claiming.  The list is cut into n_parts contiguous ranges; block b
+// starts on range b % n_parts.  Workgroups are dispatched to the XCDs round
+// robin, so with one range per XCD a wave first drains the range its own
+// XCD owns -- neighbouring tiles share shapes, whose records and strands are
+// then read through one L2 -- and afterwards helps with the others.  Every
+// wave visits every range, so the grid drains completely.
+struct Claimer {
+    uint32_t *counters;
+    int parts, n_work, part0, k, begin, end;
+    __device__ void init(uint32_t *c, int n_parts, int n, int first)
+    {
+        counters = c;
+        parts = n_parts;
+        n_work = n;
+        part0 = first % n_parts;
+        k = 0;
+        range();
+    }
+    __device__ void range()
+    {
+        const int part = (part0 + k) % parts;
+        begin = (int)(((long long)n_work * part) / parts);
+        end = (int)(((long long)n_work * (part + 1)) / parts);
+    }
+    // next work index (wave-uniform), or -1 once every range is drained
+    __device__ int next(int lane)
+    {
+        while (k < parts) {
+            uint32_t w = 0;
+            if (lane == 0) w = atomicAdd(counters + (part0 + k) % parts, 1u);
+            w = __builtin_amdgcn_readfirstlane(w);
+            if ((int)w < end - begin) return begin + (int)w;
+            if (++k < parts) range();
+        }
+        return -1;
+    }
 };
 
 struct Column {
@@ -451,10 +501,10 @@ struct Stack {
     uint32_t dirty;   // groups of 32 bits touched since their composite was cached
     __device__ __forceinline__ void flip(uint32_t h)
     {
-        const uint32_t b = h & kShapeBitMask;
+        const uint32_t b = h & kLabelMask;
         if (b < 64u) s0 ^= 1ull << b;
-        else if (b < 128u) s1 ^= 1ull << (b - 64u);
-        dirty |= 1u << ((b >> 5) & 3u);
+        else s1 ^= 1ull << (b - 64u);
+        dirty |= 1u << (b >> 5);
     }
 };
 

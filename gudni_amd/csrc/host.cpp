@@ -58,10 +58,11 @@ static inline Pt lerp_t(float t, Pt a, Pt b)
 // Split a quadratic whose control point lies strictly outside the x-range of
 // its end points ("knob") at the parameter where dx/dt changes sign, found by
 // the bisection of Deknob.hs:61-79 on FP32 parameters.
-static int split_knob(const Quad &q, Quad *out)
+static int split_knob(const Quad &q, Quad *out, float *split_t)
 {
     const bool bulge_left = q.c.x < q.p0.x && q.c.x < q.p1.x;
     const bool bulge_right = q.c.x > q.p0.x && q.c.x > q.p1.x;
+    *split_t = 0.0f;
     if (!bulge_left && !bulge_right) {
         out[0] = q;
         return 1;
@@ -88,6 +89,7 @@ static int split_knob(const Quad &q, Quad *out)
     }
     out[0] = Quad{q.p0, m0, on};
     out[1] = Quad{on, m1, q.p1};
+    *split_t = t;
     return 2;
 }
 
@@ -99,7 +101,7 @@ static inline int x_direction(const Quad &q)
 // Append one oriented strand of m <= 16 curves (Strand.hs:180-206):
 // header {u16 size in 8-byte units, u16 0, u32 0}, rightmost point, leftmost
 // point + its control, then the interior (point, control) pairs in tree order.
-static void emit_strand(const Quad *qs, int m, std::vector<uint8_t> &heap)
+static void emit_strand(const Quad *qs, int m, std::vector<uint8_t> &heap, uint32_t curve_base = 0)
 {
     const size_t base = heap.size();
     const size_t bytes = 16u * (size_t)(m + 1);
@@ -108,6 +110,7 @@ static void emit_strand(const Quad *qs, int m, std::vector<uint8_t> &heap)
     const uint16_t size = (uint16_t)(bytes / 8);
     std::memset(w, 0, 8);
     std::memcpy(w, &size, 2);
+    std::memcpy(w + 4, &curve_base, 4);   // zero unless curves are tracked
     float *f = reinterpret_cast<float *>(w + 8);
     f[0] = qs[m - 1].p1.x; f[1] = qs[m - 1].p1.y;
     f[2] = qs[0].p0.x;     f[3] = qs[0].p0.y;
@@ -121,19 +124,29 @@ static void emit_strand(const Quad *qs, int m, std::vector<uint8_t> &heap)
 }
 
 // One closed outline -> strands (Strand.hs:153-178).  Returns strand count.
+// curves (optional): records each strand curve's origin; pair_base is the
+// global index of the outline's first pair.
 static int outline_strands(const float *pairs, int k, std::vector<Quad> &work,
-                           std::vector<uint8_t> &heap)
+                           std::vector<uint8_t> &heap, std::vector<CurveSrc> *curves = nullptr,
+                           int pair_base = 0)
 {
     if (k < 2) return 0;
     work.clear();
+    std::vector<CurveSrc> wsrc;
     for (int i = 0; i < k; ++i) {
         const int j = (i + 1 == k) ? 0 : i + 1;
         Quad q{{pairs[4 * i], pairs[4 * i + 1]}, {pairs[4 * i + 2], pairs[4 * i + 3]},
                {pairs[4 * j], pairs[4 * j + 1]}};
         Quad two[2];
-        const int n = split_knob(q, two);
+        float t = 0.0f;
+        const int n = split_knob(q, two, &t);
         work.push_back(two[0]);
         if (n == 2) work.push_back(two[1]);
+        if (curves) {
+            const int32_t a = 2 * (pair_base + i), b = 2 * (pair_base + j);
+            wsrc.push_back(CurveSrc{a, a + 1, b, t, n == 2 ? kCurveLowerHalf : 0u});
+            if (n == 2) wsrc.push_back(CurveSrc{a, a + 1, b, t, kCurveUpperHalf});
+        }
     }
     // Maximal runs of curves that share a non-zero x direction; the run that
     // is still open when the loop ends is emitted first (Strand.hs:77-102).
@@ -152,7 +165,8 @@ static int outline_strands(const float *pairs, int k, std::vector<Quad> &work,
         for (int s = cuts[run]; s < cuts[run + 1]; s += kMaxStrandCurves) {
             const int m = std::min(kMaxStrandCurves, cuts[run + 1] - s);
             const Quad *src = &work[s];
-            if (src[0].p0.x > src[m - 1].p1.x) {   // run right-to-left: reverse
+            const bool reversed = src[0].p0.x > src[m - 1].p1.x;
+            if (reversed) {   // run right-to-left: reverse
                 for (int q = 0; q < m; ++q) {
                     const Quad &o = src[m - 1 - q];
                     oriented[q] = Quad{o.p1, o.c, o.p0};
@@ -160,16 +174,26 @@ static int outline_strands(const float *pairs, int k, std::vector<Quad> &work,
             } else {
                 std::copy(src, src + m, oriented);
             }
-            emit_strand(oriented, m, heap);
+            if (curves) {
+                emit_strand(oriented, m, heap, (uint32_t)curves->size());
+                for (int q = 0; q < m; ++q) {
+                    CurveSrc c = wsrc[(size_t)s + (size_t)(reversed ? m - 1 - q : q)];
+                    if (reversed) c.flags |= kCurveReversed;
+                    curves->push_back(c);
+                }
+            } else {
+                emit_strand(oriented, m, heap);
+            }
             ++strands;
         }
     }
     return strands;
 }
 
-int build_geometry(const gudni_outlines *sc, int32_t cw, int32_t ch, Geometry &g)
+int build_geometry(const gudni_outlines *sc, int32_t cw, int32_t ch, Geometry &g, bool track_curves)
 {
     g.heap.clear(); g.shapes.clear(); g.bbox.clear(); g.strands.clear(); g.source.clear();
+    g.curves.clear();
     const float fw = (float)cw, fh = (float)ch;
     std::vector<Quad> work;
     for (int32_t i = 0; i < sc->n_shapes; ++i) {
@@ -193,7 +217,8 @@ int build_geometry(const gudni_outlines *sc, int32_t cw, int32_t ch, Geometry &g
         uint32_t n = 0;
         for (int o = o0; o < o1; ++o) {
             const int a = sc->outline_pair_offsets[o], e = sc->outline_pair_offsets[o + 1];
-            n += (uint32_t)outline_strands(sc->pairs + 4 * (size_t)a, e - a, work, g.heap);
+            n += (uint32_t)outline_strands(sc->pairs + 4 * (size_t)a, e - a, work, g.heap,
+                                           track_curves ? &g.curves : nullptr, a);
         }
         ShapeRec rec;
         rec.tag = sc->tags[i];
@@ -387,7 +412,7 @@ int gudni_geometry_build(const gudni_outlines *scene, int32_t cw, int32_t ch, gu
     }
     try {
         auto *g = new gudni_geometry();
-        int rc = build_geometry(scene, cw, ch, g->g);
+        int rc = build_geometry(scene, cw, ch, g->g, false);
         if (rc) { delete g; return rc; }
         *out = g;
         return GUDNI_OK;

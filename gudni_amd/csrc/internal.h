@@ -4,6 +4,8 @@
 #include <string>
 #include <vector>
 
+#include "gudni_hip.h"
+
 namespace gudni {
 
 constexpr int kMaxStrandCurves = 16;   // splitTooLarge (Strand.hs:171-178)
@@ -37,12 +39,29 @@ static_assert(sizeof(PictUse) == 24, "PictureUse is 24 bytes");
 
 struct BBox { float l, t, r, b; };
 
+// Where a strand curve comes from, for gradients with respect to the outline
+// points: the outline curve's on-curve start, control and on-curve end (point
+// indices: pair q's on-curve point is 2q, its control 2q+1), the knob-split
+// parameter and which half this is, and whether the strand runs it backwards.
+struct CurveSrc {
+    int32_t p0, c, p1;
+    float t;
+    uint32_t flags;
+};
+constexpr uint32_t kCurveLowerHalf = 1u;   // [0, t] of a knob split
+constexpr uint32_t kCurveUpperHalf = 2u;   // [t, 1]
+constexpr uint32_t kCurveReversed = 4u;    // strand order is p1 -> p0
+static_assert(sizeof(CurveSrc) == 20, "CurveSrc is 20 bytes");
+
 struct Geometry {
     std::vector<uint8_t> heap;
     std::vector<ShapeRec> shapes;
     std::vector<BBox> bbox;
     std::vector<uint32_t> strands;
     std::vector<int32_t> source;
+    // with curve tracking only: one entry per strand curve, in heap order;
+    // each strand header's second dword holds its first curve's index
+    std::vector<CurveSrc> curves;
 };
 
 struct Jobs {
@@ -61,5 +80,11 @@ inline int ceil_log2(int x)
 }
 
 extern thread_local std::string g_host_error;
+
+// Serialisation (track_curves: also record each strand curve's origin and
+// write strand curve indices into the strand headers' second dword).
+int build_geometry(const gudni_outlines *sc, int32_t cw, int32_t ch, Geometry &g,
+                   bool track_curves);
+int build_jobs(const gudni_spec &spec, const Geometry &g, int32_t cw, int32_t ch, Jobs &out);
 
 } // namespace gudni

@@ -44,11 +44,11 @@ __global__ void __launch_bounds__(256) raster_pass(Frame f, int from_retry)
     int32_t *sidx = f.shape_idx + (size_t)slot * f.max_shapes * kWave + lane;
     TileTable *tt = &tables[wave];
     const int n_work = from_retry ? (int)min((uint32_t)f.retry_capacity, f.retry[0]) : f.n_work;
+    Claimer claim;
+    claim.init(f.counter, f.n_parts, n_work, (int)blockIdx.x);
     for (;;) {
-        uint32_t w = 0;
-        if (lane == 0) w = atomicAdd(f.counter, 1u);
-        w = wave_uniform(w);
-        if ((int)w >= n_work) break;
+        const int w = claim.next(lane);
+        if (w < 0) break;
         const uint32_t unit = from_retry ? f.retry[1 + w] : f.work[w];
         const int count = f.tiles[unit >> 2].shape_count;   // wave-uniform
         const int scene = f.tile_scene
@@ -109,7 +109,7 @@ namespace gudni {
 
 // Which quarters of a tile have an active column (isActiveThread,
 // Kernels.cl:1720-1722), evaluated for every column on the host.
-static void build_work(const gudni_spec &spec, const TileRec *tiles, int n_tiles, int width,
+void build_work(const gudni_spec &spec, const TileRec *tiles, int n_tiles, int width,
                        int height, std::vector<uint32_t> &work)
 {
     const int cd = ceil_log2(spec.threads_per_tile);
@@ -130,13 +130,14 @@ static void build_work(const gudni_spec &spec, const TileRec *tiles, int n_tiles
     }
 }
 
-static int ensure_scratch(gudni_ctx *ctx)
+int ensure_scratch(gudni_ctx *ctx)
 {
     if (ctx->fast_blocks) return GUDNI_OK;
     const int waves_per_block = 4;
     int per_cu = 0;
     CHECK_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::raster_pass, 256, 0));
     if (per_cu < 1) per_cu = 1;
+    if (ctx->blocks_per_cu > 0) per_cu = std::min(per_cu, ctx->blocks_per_cu);
     ctx->fast_blocks = ctx->cus * per_cu;
     ctx->fast_blocks = std::min(ctx->fast_blocks, ctx->cus * 4);
     ctx->slow_blocks = std::max(1, ctx->cus / 8);
@@ -156,18 +157,18 @@ static int ensure_scratch(gudni_ctx *ctx)
     return GUDNI_OK;
 }
 
-static int ensure_control(gudni_ctx *ctx, int n_work)
+int ensure_control(gudni_ctx *ctx, int n_work)
 {
     const int want = std::max(1024, n_work);
     if (ctx->retry_capacity < want) {
-        CHECK_HIP(ctx, ctx->control.ensure(sizeof(uint32_t) * (size_t)(4 + want)));
+        CHECK_HIP(ctx, ctx->control.ensure(sizeof(uint32_t) * (size_t)(dev::kCtlWords + want)));
         ctx->retry_capacity = want;
     }
     return GUDNI_OK;
 }
 
 // Fill a Frame with the context's buffers; caller sets the per-launch parts.
-static dev::Frame base_frame(gudni_ctx *ctx)
+dev::Frame base_frame(gudni_ctx *ctx)
 {
     dev::Frame f;
     std::memset(&f, 0, sizeof(f));
@@ -222,13 +223,14 @@ static dev::SceneDesc single_desc(gudni_ctx *ctx, const float bg[4], int32_t w, 
 static int launch_passes(gudni_ctx *ctx, dev::Frame f)
 {
     uint32_t *ctl = (uint32_t *)ctx->control.p;
-    hipLaunchKernelGGL(dev::reset_counters, dim3(1), dim3(64), 0, ctx->stream, ctl, 4);
+    hipLaunchKernelGGL(dev::reset_counters, dim3(1), dim3(64), 0, ctx->stream, ctl, dev::kCtlWords);
     CHECK_HIP(ctx, hipGetLastError());
-    f.status = ctl + 2;
-    f.retry = ctl + 3;
+    f.status = ctl + dev::kCtlStatus;
+    f.retry = ctl + dev::kCtlRetry;
     f.retry_capacity = ctx->retry_capacity;
     // fast pass: small rings, as many waves as fit
-    f.counter = ctl + 0;
+    f.counter = ctl + dev::kCtlFast;
+    f.n_parts = dev::kParts;
     f.q_thr = (float4 *)ctx->fast_thr.p;
     f.q_hdr = (uint32_t *)ctx->fast_hdr.p;
     f.shape_idx = (int32_t *)ctx->fast_sidx.p;
@@ -240,7 +242,8 @@ static int launch_passes(gudni_ctx *ctx, dev::Frame f)
     // retry pass: rings of maxThresholds records
     int slog = 0;
     while ((1 << slog) < ctx->spec.max_thresholds) ++slog;
-    f.counter = ctl + 1;
+    f.counter = ctl + dev::kCtlSlow;
+    f.n_parts = 1;
     f.q_thr = (float4 *)ctx->slow_thr.p;
     f.q_hdr = (uint32_t *)ctx->slow_hdr.p;
     f.shape_idx = (int32_t *)ctx->slow_sidx.p;
@@ -250,10 +253,10 @@ static int launch_passes(gudni_ctx *ctx, dev::Frame f)
     return GUDNI_OK;
 }
 
-static int read_status(gudni_ctx *ctx)
+int read_status(gudni_ctx *ctx)
 {
     uint32_t st = 0;
-    CHECK_HIP(ctx, hipMemcpyAsync(&st, (uint32_t *)ctx->control.p + 2, 4, hipMemcpyDeviceToHost,
+    CHECK_HIP(ctx, hipMemcpyAsync(&st, (uint32_t *)ctx->control.p + dev::kCtlStatus, 4, hipMemcpyDeviceToHost,
                                   ctx->stream));
     CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (st & dev::kStatusOverflow)
@@ -269,7 +272,7 @@ CommonView ctx_view(gudni_ctx *ctx)
                       ctx->n_substances, ctx->n_uses};
 }
 
-static int validate_uses(gudni_ctx *ctx, const void *pict_uses, int32_t n_pict_uses,
+int validate_uses(gudni_ctx *ctx, const void *pict_uses, int32_t n_pict_uses,
                          size_t pict_bytes)
 {
     for (int32_t i = 0; i < n_pict_uses; ++i) {
@@ -796,6 +799,15 @@ int gudni_raster_scene(gudni_ctx *ctx, const gudni_scene *sc, uint32_t *host_out
     if (rc) return rc;
     CHECK_HIP(ctx, hipMemcpy(host_out, ctx->out.p, 4 * npix, hipMemcpyDeviceToHost));
     return GUDNI_OK;
+}
+
+int gudni_set_blocks_per_cu(gudni_ctx *ctx, int32_t blocks_per_cu)
+{
+    if (!ctx || blocks_per_cu < 0 || blocks_per_cu > 4) return GUDNI_ERR_ARG;
+    CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->blocks_per_cu = blocks_per_cu;
+    ctx->fast_blocks = 0;   // resized on the next launch
+    return ensure_scratch(ctx);
 }
 
 int gudni_enable_timing(gudni_ctx *ctx, int enable)

@@ -215,9 +215,57 @@ float gudni_last_bin_ms(const gudni_ctx *ctx);
 /* Upload + bin + render + read back, synchronously: the scene's W*H BGRA
  * words into host_out. */
 int   gudni_raster_scene(gudni_ctx *ctx, const gudni_scene *scene, uint32_t *host_out);
+
+/* ---- Adjoint: gradients of an image-space L2 loss --------------------------
+ * No counterpart in the reference (an extension on the same raster path).
+ * Forward: the raster path with GUDNI_COVERAGE_ACCURATE section mid-points and
+ * a premultiplied front-to-back composite per section; the image is the
+ * pre-quantisation RGBA.  Loss: L = sum over pixels and RGBA channels of
+ * (image - target)^2.  Gradients: dL/d(outline points) in the layout of
+ * gudni_outlines.pairs (on.x on.y off.x off.y per pair) and dL/d(substance
+ * RGBA) for solid substances (zero for pictures).  Discrete decisions (knob
+ * splits, strand runs, tiles, curve subdivision, record drops, section
+ * structure) are held; the knob-split parameter follows the analytic vertical
+ * tangent.  Requirements: every shape is an added shape with a substance of
+ * its own.  One persistent launch runs the forward and, recomputing each
+ * tile's records, the backward, for all scenes of the handle. */
+typedef struct gudni_adjoint_scene {
+    gudni_outlines outlines;            /* the geometry parameters           */
+    const float *rgba;  int32_t n_substances;   /* 16 B substance records    */
+    const void *pict_data;  size_t pict_bytes;
+    const void *pict_uses;  int32_t n_pict_uses;
+    const float *target;                /* width*height RGBA floats          */
+    int32_t width, height;
+    float background[4];
+} gudni_adjoint_scene;
+typedef struct gudni_adjoint_batch gudni_adjoint_batch;
+/* Serialises, bins and uploads n_scenes scenes with their targets. */
+int  gudni_adjoint_upload(gudni_ctx *ctx, int32_t n_scenes, const gudni_adjoint_scene *scenes,
+                          gudni_adjoint_batch **out);
+/* New parameters for one scene (pairs and/or rgba may be NULL = unchanged):
+ * re-serialises and re-bins on the host and re-uploads. */
+int  gudni_adjoint_update(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t scene, const float *pairs,
+                          const float *rgba);
+/* Forward + backward of every scene in one launch (stream-ordered). */
+int  gudni_adjoint_run(gudni_ctx *ctx, gudni_adjoint_batch *a);
+/* The same for one scene alone. */
+int  gudni_adjoint_run_scene(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t scene);
+/* Waits and copies one scene's loss, gradients and image (any may be NULL). */
+int  gudni_adjoint_result(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t scene, double *loss,
+                          float *d_pairs, float *d_rgba, float *image);
+void gudni_adjoint_free(gudni_adjoint_batch *a);
+/* One scene, synchronously: upload, run, result, free. */
+int  gudni_adjoint(gudni_ctx *ctx, const gudni_adjoint_scene *scene, double *loss, float *d_pairs,
+                   float *d_rgba, float *image);
 /* Pass timings (ms) of the last render, recorded with hipEvents on the
  * launch stream when enabled and read by gudni_frame_check: [0] counter
  * reset + fast pass, [1] retry pass. */
+/* Launch shape of the persistent fast pass: blocks (of four waves) per
+ * compute unit, 1..4, or 0 for as many as fit (at most 4).  Fewer resident
+ * waves keep more of each wave's threshold rings in L2; more hide more
+ * latency.  No counterpart in the reference (its launch shape is fixed by
+ * the OpenCL runtime). */
+int gudni_set_blocks_per_cu(gudni_ctx *ctx, int32_t blocks_per_cu);
 int gudni_enable_timing(gudni_ctx *ctx, int enable);
 int gudni_last_timings(gudni_ctx *ctx, float *ms, int32_t n);
 /* Shader-clock cycles of the last timed render's fast pass, summed over

@@ -18,9 +18,9 @@ LIB = os.path.join(HERE, "liboracle_gudni.so")
 
 
 def build(verbose: bool = False) -> str:
-    src = os.path.join(HERE, "gudni_oracle.c")
+    srcs = [os.path.join(HERE, "gudni_oracle.c"), os.path.join(HERE, "gudni_diff.c")]
     cmd = ["gcc", "-O2", "-std=c11", "-Wall", "-Wextra", "-ffp-contract=off",
-           "-fno-fast-math", "-fPIC", "-shared", "-o", LIB + ".tmp", src, "-lm", "-lpthread"]
+           "-fno-fast-math", "-fPIC", "-shared", "-o", LIB + ".tmp"] + srcs + ["-lm", "-lpthread"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -127,6 +127,26 @@ int go_render_scene(const go_spec *spec, const go_serialized *scene,
 
 void go_default_spec(go_spec *spec);
 
+/* FP64 restatement of the differentiable forward (gudni_diff.c): the loss
+ * sum (pixel - target)^2 over the window [x0,x1) x [y0,y1) of a scene of
+ * solid substances (rgba: 4 doubles per substance), the window's pixels
+ * (optional, row-major RGBA doubles) and a signature of every discrete
+ * decision.  Returns 0, or < 0 on queue overflow / sweep guard. */
+int go_diff_loss(int32_t n_shapes, const uint64_t *tags, const int32_t *shape_outline_offsets,
+                 const int32_t *outline_pair_offsets, const double *pairs, const double *rgba,
+                 const double *background, int32_t width, int32_t height, const float *target,
+                 int32_t x0, int32_t y0, int32_t x1, int32_t y1, double *loss,
+                 double *image, uint64_t *signature);
+
+/* The adjoint's algorithm restated in FP64 over the whole canvas, columns
+ * cut into segments of `seg` rows: loss, dL/d(outline points) (4 doubles per
+ * pair) and dL/d(substance RGBA). */
+int go_diff_grad(int32_t n_shapes, const uint64_t *tags, const int32_t *shape_outline_offsets,
+                 const int32_t *outline_pair_offsets, const double *pairs, const double *rgba,
+                 const double *background, int32_t width, int32_t height, const float *target,
+                 int32_t seg, double *loss, double *d_pairs, double *d_rgba, int32_t n_pairs,
+                 int32_t n_substances);
+
 #ifdef __cplusplus
 }
 #endif

@@ -82,6 +82,16 @@ def lib():
                                       C.POINTER(JobIn), C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_int]
         L.go_default_spec.argtypes = [C.POINTER(Spec)]
+        L.go_diff_loss.restype = C.c_int
+        L.go_diff_loss.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
+                                   C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                   C.POINTER(C.c_double), C.c_void_p, C.POINTER(C.c_uint64)]
+        L.go_diff_grad.restype = C.c_int
+        L.go_diff_grad.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
+                                   C.c_int32, C.POINTER(C.c_double), C.c_void_p, C.c_void_p,
+                                   C.c_int32, C.c_int32]
         _lib = L
     return _lib
 
@@ -258,3 +268,47 @@ def render_jobs(scene, spec, geometry, jobs, job_ids, out, nthreads=1, rnd=None)
         h = np.clip(np.minimum(tr[:, 3], scene.height) - tr[:, 1], 0, None)
         pixels += int((w * h).sum())
     return pixels
+
+
+def diff_loss(scene, pairs, rgba, target, window, want_image=False):
+    """FP64 forward of the differentiable path (gudni_diff.c) over a window
+    (x0, y0, x1, y1) of pixel columns: (loss, image or None, signature).
+    pairs: (n_pairs, 4) outline points, rgba: (n_substances, 4) solid
+    colours, target: (H, W, 4) float32."""
+    x0, y0, x1, y1 = [int(v) for v in window]
+    tags = np.ascontiguousarray(scene.tags, np.uint64)
+    soff = np.ascontiguousarray(scene.shape_outline_offsets, np.int32)
+    ooff = np.ascontiguousarray(scene.outline_pair_offsets, np.int32)
+    pairs = np.ascontiguousarray(pairs, np.float64)
+    rgba = np.ascontiguousarray(rgba, np.float64)
+    bg = np.ascontiguousarray(scene.background, np.float64)
+    target = np.ascontiguousarray(target, np.float32)
+    img = np.zeros((y1 - y0, x1 - x0, 4), np.float64) if want_image else None
+    loss = C.c_double()
+    sig = C.c_uint64()
+    rc = lib().go_diff_loss(scene.n_shapes, _ptr(tags), _ptr(soff), _ptr(ooff), _ptr(pairs),
+                            _ptr(rgba), _ptr(bg), scene.width, scene.height, _ptr(target),
+                            x0, y0, x1, y1, C.byref(loss), _ptr(img), C.byref(sig))
+    if rc != 0:
+        raise RuntimeError(f"go_diff_loss failed ({rc})")
+    return loss.value, img, sig.value
+
+
+def diff_grad(scene, pairs, rgba, target, seg=256):
+    """The adjoint's algorithm in FP64 (gudni_diff.c): (loss, d_pairs, d_rgba)."""
+    tags = np.ascontiguousarray(scene.tags, np.uint64)
+    soff = np.ascontiguousarray(scene.shape_outline_offsets, np.int32)
+    ooff = np.ascontiguousarray(scene.outline_pair_offsets, np.int32)
+    pairs = np.ascontiguousarray(pairs, np.float64).reshape(-1, 4)
+    rgba = np.ascontiguousarray(rgba, np.float64).reshape(-1, 4)
+    bg = np.ascontiguousarray(scene.background, np.float64)
+    target = np.ascontiguousarray(target, np.float32)
+    dp = np.zeros_like(pairs)
+    dc = np.zeros_like(rgba)
+    loss = C.c_double()
+    rc = lib().go_diff_grad(scene.n_shapes, _ptr(tags), _ptr(soff), _ptr(ooff), _ptr(pairs), _ptr(rgba),
+                            _ptr(bg), scene.width, scene.height, _ptr(target), int(seg), C.byref(loss),
+                            _ptr(dp), _ptr(dc), len(pairs), len(rgba))
+    if rc != 0:
+        raise RuntimeError(f"go_diff_grad failed ({rc})")
+    return loss.value, dp, dc

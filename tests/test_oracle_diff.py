@@ -1,0 +1,136 @@
+"""The FP64 forward used for the adjoint's finite differences (oracle/
+gudni_diff.c) against the oracle's FP32 raster path and its FP64 section
+integration: the same pixels, so the finite differences differentiate the
+function the raster path computes."""
+import numpy as np
+
+from gudni_amd import scenes as S
+from oracle import oracle as O
+from tests import helpers as H
+from tests import adjoint_cases as A
+
+
+def fp64_image(scene):
+    target = np.zeros((scene.height, scene.width, 4), np.float32)
+    _, img, _ = O.diff_loss(scene, scene.pairs.reshape(-1, 4), np.asarray(scene.substances, np.float64),
+                            target, (0, 0, scene.width, scene.height), want_image=True)
+    return img
+
+
+def polygon_scene(n=120, size=192, seed=3):
+    rng = np.random.default_rng(seed)
+    shapes, subs = [], []
+    for i in range(n):
+        k = int(rng.integers(3, 9))
+        c = rng.uniform(0, size, 2)
+        r = rng.uniform(4, 40)
+        ang = np.sort(rng.uniform(0, 2 * np.pi, k))
+        pts = [(float(c[0] + r * np.cos(a)), float(c[1] + r * np.sin(a))) for a in ang]
+        shapes.append((S.solid_tag(i), [S.straight_pairs(pts)]))
+        subs.append([float(v) for v in rng.uniform(0, 1, 4)])
+    return S.build_scene(size, size, shapes, subs, name="polygons")
+
+
+def test_fp64_forward_matches_oracle_on_polygons():
+    """Straight edges (no knobs): only FP32 rounding separates the two."""
+    sc = polygon_scene()
+    ref = H.oracle_render(sc, want_f64=True)
+    d = np.abs(fp64_image(sc) - ref["f64"].reshape(sc.height, sc.width, 4)).max(axis=2)
+    assert np.median(d) < 1e-7
+    assert (d > 1e-5).mean() < 1e-3, f"{(d > 1e-5).sum()} pixels differ by more than 1e-5"
+
+
+def test_fp64_forward_matches_oracle_on_blobs():
+    """Blobs with knob splits: the FP64 forward splits at the exact tangent,
+    the serialiser at its FP32 bisection (within 1e-4 in t)."""
+    sc = S.batch_scene(192, 200, 5)
+    ref = H.oracle_render(sc, want_f64=True)
+    d = np.abs(fp64_image(sc) - ref["f64"].reshape(sc.height, sc.width, 4)).max(axis=2)
+    assert np.median(d) < 1e-7
+    assert (d > 1e-5).mean() < 2e-2
+    assert d.max() < 2e-2
+
+
+def test_window_equals_whole_frame():
+    """Cutting columns into segments changes nothing (tiles only do that)."""
+    sc = polygon_scene(60, 128, 9)
+    full = fp64_image(sc)
+    target = np.zeros((sc.height, sc.width, 4), np.float32)
+    _, win, _ = O.diff_loss(sc, sc.pairs.reshape(-1, 4), np.asarray(sc.substances, np.float64), target,
+                            (30, 41, 97, 100), want_image=True)
+    np.testing.assert_allclose(win, full[41:100, 30:97], atol=1e-12)
+
+
+def test_loss_is_sum_of_squares_and_signature_detects_flips():
+    sc = polygon_scene(40, 96, 11)
+    target = np.random.default_rng(0).uniform(0, 1, (sc.height, sc.width, 4)).astype(np.float32)
+    pairs = sc.pairs.reshape(-1, 4).astype(np.float64)
+    rgba = np.asarray(sc.substances, np.float64)
+    loss, img, sig = O.diff_loss(sc, pairs, rgba, target, (0, 0, 96, 96), want_image=True)
+    assert abs(loss - ((img - target) ** 2).sum()) < 1e-9 * max(1.0, loss)
+    # a colour change keeps every decision; moving a vertex by pixels does not
+    _, _, sig_c = O.diff_loss(sc, pairs, rgba + 0.01, target, (0, 0, 96, 96))
+    assert sig_c == sig
+    moved = pairs.copy()
+    moved[0, :2] += 5.0
+    _, _, sig_m = O.diff_loss(sc, moved, rgba, target, (0, 0, 96, 96))
+    assert sig_m != sig
+
+
+def test_colour_gradient_closed_form():
+    """One opaque-background pixel fully inside one translucent square: the
+    pixel is a*c + (1-a)*bg, so dL/dc = 2 (pix - t) a exactly; the central
+    difference of the FP64 forward reproduces it."""
+    sq = S.build_scene(16, 16, [(S.solid_tag(0), [S.rectangle_pairs(2.0, 2.0, 12.0, 12.0)])],
+                       [[0.3, 0.6, 0.9, 0.5]], background=(1.0, 1.0, 1.0, 1.0), name="square")
+    target = np.zeros((16, 16, 4), np.float32)
+    pairs = sq.pairs.reshape(-1, 4).astype(np.float64)
+    rgba = np.asarray(sq.substances, np.float64)
+    h = 1e-4
+    lp = O.diff_loss(sq, pairs, rgba + np.array([[h, 0, 0, 0]]), target, (5, 5, 6, 6))[0]
+    lm = O.diff_loss(sq, pairs, rgba - np.array([[h, 0, 0, 0]]), target, (5, 5, 6, 6))[0]
+    pix = 0.5 * 0.3 + 0.5 * 1.0
+    assert abs((lp - lm) / (2 * h) - 2 * pix * 0.5) < 1e-8
+
+
+def test_fd_samples_are_finite():
+    sc, target = A.case(96, 60, 21)
+    smp = A.fd_samples(sc, target, n=8, seed=1)
+    assert len(smp) == 8 and all(np.isfinite(v) for *_, v in smp)
+
+
+def test_fp64_adjoint_matches_finite_differences():
+    """The adjoint's algorithm restated in FP64 (go_diff_grad: line weights,
+    span-boundary terms, end points, knob splits, colour walk) against
+    central differences of the FP64 forward on 64 sampled parameters."""
+    sc, target = A.case(128, 40, 63)
+    _, dp, dc = O.diff_grad(sc, sc.pairs, sc.substances, target)
+    samples = A.fd_samples(sc, target, n=64, seed=3)
+    assert len(samples) == 64
+    got, fd, err, bad = A.compare(samples, dp, dc)
+    assert not bad.any(), [(s, g) for s, g, b in zip(samples, got, bad) if b]
+
+
+def test_fp64_adjoint_closed_forms():
+    """A translucent square over white: the colour gradient of one pixel is
+    2 (pix - t) a per channel; moving the right edge by dx changes the loss by
+    the edge column's colour difference times its height."""
+    sq = S.build_scene(16, 16, [(S.solid_tag(0), [S.rectangle_pairs(2.0, 2.0, 12.5, 12.0)])],
+                       [[0.3, 0.6, 0.9, 0.5]], background=(1.0, 1.0, 1.0, 1.0), name="square")
+    target = np.zeros((16, 16, 4), np.float32)
+    _, dp, dc = O.diff_grad(sq, sq.pairs, sq.substances, target)
+    # d/dr = sum over pixels of 2 pix_r * a * coverage: 100 full pixels with
+    # pix_r = 0.65 and the ten half-covered pixels of column 12, pix_r = 0.825
+    r = float(np.float32(0.3))   # the substance table is FP32
+    expect_r = 100 * 2 * (0.5 * r + 0.5) * 0.5 + 10 * 2 * (0.25 * r + 0.75) * 0.5 * 0.5
+    assert abs(dc[0, 0] - expect_r) < 1e-9
+    # the right edge x = 12.5 of rows 2..12: half-covered column 12
+    col = np.float32([0.3, 0.6, 0.9]).astype(np.float64)
+    inside = np.append(0.5 * col + 0.5, 1.0)
+    outside = np.ones(4)
+    g_col12 = 2 * (0.5 * inside + 0.5 * outside)          # dL/dpixel in column 12
+    expect = float(g_col12 @ (inside - outside)) * 10.0     # ten rows
+    pts = sq.pairs.reshape(-1, 4)
+    right = [i for i in range(len(pts)) if abs(pts[i, 0] - 12.5) < 1e-6]
+    got = sum(dp[i, 0] for i in right) + sum(dp[i, 2] for i in range(len(pts)) if abs(pts[i, 2] - 12.5) < 1e-6)
+    assert abs(got - expect) < 1e-9, (got, expect)
