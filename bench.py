@@ -45,12 +45,13 @@ def algorithmic_bytes(scene, geo, jobs):
     return per_ref + 32 * jobs.n_tiles + 16 * scene.substances.shape[0] + 4 * scene.width * scene.height
 
 
-def cpu_baseline(scene, seconds_target=15.0):
+def cpu_baseline(scene, seconds_target=15.0, texture_filter=1):
     """The oracle (CPU restatement, all host threads) on a bounded sample of
     the same frame: whole raster jobs, taken in order until ~seconds_target."""
     from oracle import oracle as O
     spec = O.default_spec()
     spec.coverage_mode = 1   # the reference's own arithmetic, as its CPU path would run
+    spec.texture_filter = texture_filter
     og = O.Geometry(scene)
     oj = O.JobList(spec, og, scene.width, scene.height)
     threads = min(16, os.cpu_count() or 1)
@@ -112,6 +113,107 @@ def device_binning_leg(r, scene, geo, stream, steps):
     return {"bin_ms": float(np.median(bin_ms)), "bin_plus_render_ms": both,
             "bin_plus_render_gpixels_per_s": scene.width * scene.height / (both * 1e-3) / 1e9,
             "tiles": nt, "refs": nr, "work_units": nw}
+
+
+def render_f32(r, scene, spec):
+    """The scene's pre-quantisation RGBA image from the raster path (H, W, 4)."""
+    import torch
+    import gudni_amd as G
+    _, geo, jobs = G.prepare(scene, spec)
+    r.upload_common(geo.geo, scene)
+    r.frame_upload(jobs, scene)
+    out = torch.zeros(scene.width * scene.height, dtype=torch.int32, device="cuda")
+    f32 = torch.zeros(scene.width * scene.height * 4, dtype=torch.float32, device="cuda")
+    r.frame_render(out.data_ptr(), f32.data_ptr())
+    r.frame_check()
+    return f32.cpu().numpy().reshape(scene.height, scene.width, 4)
+
+
+def adjoint_leg(spec, steps, log_fn):
+    """The differentiable workload: a 1024^2, 2,000-shape target and a
+    perturbed copy as parameters; one step = forward + backward (loss,
+    dL/d points, dL/d RGBA) in one launch.  Device time from events around the
+    launches; the full step adds re-serialising and re-binning the updated
+    parameters on the host, the upload and reading the gradients back."""
+    import torch
+    import gudni_amd as G
+    from gudni_amd import scenes as S
+    base = S.config4()
+    params = S.perturb(base, S.SEEDS[4] + 1)
+    r = G.Rasterizer(spec)
+    stream = torch.cuda.current_stream()
+    r.set_stream(stream.cuda_stream)
+    target = render_f32(r, base, spec)
+    adj = G.Adjoint(r, [(params, target)])
+    adj.run()
+    loss0, dp, dc, _ = adj.result(0)
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(steps):
+        adj.run()
+    stop.record(stream)
+    adj.result(0)
+    dev_ms = start.elapsed_time(stop) / steps
+    pairs = params.pairs.copy()
+    t = time.perf_counter()
+    for _ in range(steps):
+        pairs = pairs - 0.05 * dp.reshape(pairs.shape)
+        adj.update(0, pairs=pairs)
+        adj.run()
+        loss, dp, dc, _ = adj.result(0)
+    full_ms = (time.perf_counter() - t) / steps * 1e3
+    adj.close()
+    r.close()
+    log_fn(f"adjoint: {dev_ms:.2f} ms per step on the device, {full_ms:.2f} ms with host updates; "
+           f"loss {loss0:.2f} -> {loss:.2f}")
+    return {"workload": "1024x1024, 2000 shapes (17k quadratics), solid; params perturbed "
+                        "(points + N(0,4px), colours + N(0,0.1))",
+            "ms_per_step_device": dev_ms, "steps_per_s_device": 1e3 / dev_ms,
+            "ms_per_step_with_host_update": full_ms, "steps_per_s_with_host_update": 1e3 / full_ms,
+            "loss_first": loss0, "loss_after_steps": loss, "n_points": int(params.pairs.size // 2),
+            "n_substances": int(len(params.substances))}
+
+
+def batched_adjoint_leg(spec, steps, log_fn):
+    """64 scenes of 512^2 (500-5,000 shapes, solid + a 1024^2 texture): forward
+    and backward of all scenes in one persistent launch, against 64
+    per-scene adjoint launches."""
+    import torch
+    import gudni_amd as G
+    from gudni_amd import scenes as S
+    bases = S.config5()
+    r = G.Rasterizer(spec)
+    stream = torch.cuda.current_stream()
+    r.set_stream(stream.cuda_stream)
+    items = [(S.perturb(b, S.SEEDS[5] + i), render_f32(r, b, spec)) for i, b in enumerate(bases)]
+    adj = G.Adjoint(r, items)
+    adj.run()
+    adj.result(0)
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(steps):
+        adj.run()
+    stop.record(stream)
+    adj.result(0)
+    one_ms = start.elapsed_time(stop) / steps
+    start.record(stream)
+    for _ in range(steps):
+        for i in range(len(items)):
+            adj.run_scene(i)
+    stop.record(stream)
+    adj.result(0)
+    seq_ms = start.elapsed_time(stop) / steps
+    adj.close()
+    r.close()
+    log_fn(f"batched adjoint: {one_ms:.2f} ms per step in one launch, {seq_ms:.2f} ms as "
+           f"{len(items)} launches")
+    return {"workload": f"{len(items)} scenes of 512x512, 500-5000 shapes, solid + 1024^2 texture",
+            "ms_per_step": one_ms, "steps_per_s": 1e3 / one_ms,
+            "sequential_ms_per_step": seq_ms, "sequential_steps_per_s": 1e3 / seq_ms,
+            "speedup_vs_sequential": seq_ms / one_ms,
+            "shapes": int(sum(s.n_shapes for s, _ in items))}
 
 
 def batched_leg(spec, steps, log_fn):
@@ -182,6 +284,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable workloads")
+    ap.add_argument("--texture-filter", type=int, default=1,
+                    help="picture sampling of the benchmark scenes: 1 bilinear, 0 nearest")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -210,7 +315,7 @@ def main():
     scene = S.config2()
     t_gen = time.perf_counter() - t
     t = time.perf_counter()
-    spec, geo, jobs = G.prepare(scene)
+    spec, geo, jobs = G.prepare(scene, G.default_spec(texture_filter=args.texture_filter))
     t_host = time.perf_counter() - t
     log(f"scene {scene.width}x{scene.height}: {scene.n_shapes} shapes, {scene.n_curves} curves, "
         f"{jobs.n_tiles} tiles, {jobs.n_refs} tile-shape refs; gen {t_gen:.2f}s host prep {t_host:.2f}s")
@@ -273,6 +378,13 @@ def main():
     binning = device_binning_leg(r, scene, geo, stream, max(3, min(args.steps, 10)))
     traffic, traffic_info = measured_traffic(fast_ms)
 
+    adjoint = batched_adjoint = None
+    if not args.no_adjoint:
+        log("adjoint workload (1024^2, 2000 shapes)")
+        adjoint = adjoint_leg(spec, max(3, args.batch_steps), log)
+        log("batched adjoint workload (64 x 512^2)")
+        batched_adjoint = batched_adjoint_leg(spec, max(2, args.batch_steps), log)
+
     batched = None
     if not args.no_batch:
         log("batched workload (256 scenes)")
@@ -281,7 +393,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle on a bounded sample)")
-        cpu = cpu_baseline(scene, args.cpu_seconds)
+        cpu = cpu_baseline(scene, args.cpu_seconds, args.texture_filter)
 
     line = {
         "metric": METRIC, "value": gpix, "unit": "Gpixels/s", "n_gpus": 1, "steps": args.steps,
@@ -289,7 +401,9 @@ def main():
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "mcurves_per_s": mcurves,
         "config": {"workload": "plain raster, 4096x4096 canvas, 200k shapes (180k blobs, 20k glyph "
-                               "outlines), 2.14M quadratics, solid + nearest-texture substances",
+                               "outlines), 2.14M quadratics; substances 70% solid, 20% linear/radial "
+                               "gradients, 10% textures (" + ("bilinear" if args.texture_filter else "nearest")
+                               + " sampling)",
                    "canvas": [scene.width, scene.height], "shapes": scene.n_shapes,
                    "curves": scene.n_curves, "tiles": jobs.n_tiles, "tile_shape_refs": jobs.n_refs,
                    "coverage_mode": "accurate"},
@@ -301,6 +415,8 @@ def main():
                      "stage_cycle_share": stage_share},
         "device_binning": binning,
         "batched": batched,
+        "adjoint": adjoint,
+        "batched_adjoint": batched_adjoint,
         "cpu_baseline": cpu,
         "host_stages_s": {"serialize_and_bin": t_host, "upload": t_upload, "readback": t_read},
     }

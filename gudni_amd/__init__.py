@@ -40,6 +40,14 @@ def _ptr(a):
     return None if a is None else C.c_void_p(a.ctypes.data)
 
 
+def _gradients(scene) -> np.ndarray:
+    """The scene's gradient records (28 float32 words each), possibly none."""
+    g = getattr(scene, "gradients", None)
+    if g is None or len(g) == 0:
+        return np.zeros((0, 28), np.float32)
+    return np.ascontiguousarray(np.asarray(g, np.float32).reshape(-1, 28))
+
+
 def default_spec(**overrides) -> RasterSpec:
     s = RasterSpec()
     _lib.load().gudni_default_spec(C.byref(s))
@@ -170,6 +178,9 @@ class Rasterizer:
         self._check(self.L.gudni_upload_common(
             self.ctx, _ptr(geo), geo.size, _ptr(subs), subs.shape[0], _ptr(pict), pict.size,
             _ptr(uses) if uses.size else None, uses.shape[0], _ptr(rnd), rnd.size))
+        grads = _gradients(scene)
+        self._check(self.L.gudni_upload_gradients(self.ctx, _ptr(grads) if len(grads) else None,
+                                                   len(grads)))
 
     def raster_job(self, shapes: np.ndarray, tiles: np.ndarray, background, width, height,
                    frame=0, job_index=0, out: np.ndarray | None = None) -> np.ndarray:
@@ -233,15 +244,16 @@ class Rasterizer:
                 np.ascontiguousarray(scene.substances, np.float32),
                 np.ascontiguousarray(scene.pict_data, np.uint8),
                 np.ascontiguousarray(scene.pict_uses, np.uint8),
-                np.ascontiguousarray(rnd, np.float32)]
-        geo, shapes, bbox, subs, pict, uses, rnd = arrs
+                np.ascontiguousarray(rnd, np.float32), _gradients(scene)]
+        geo, shapes, bbox, subs, pict, uses, rnd, grads = arrs
         d = _lib.SceneDesc(_ptr(geo) if geo.size else None, geo.size,
                            _ptr(shapes) if shapes.size else None, shapes.size // 16,
                            _ptr(bbox) if bbox.size else None, _ptr(subs) if subs.size else None,
                            subs.shape[0], _ptr(pict) if pict.size else None, pict.size,
                            _ptr(uses) if uses.size else None, uses.shape[0], _ptr(rnd), rnd.size,
                            scene.width, scene.height,
-                           (C.c_float * 4)(*[float(x) for x in scene.background]))
+                           (C.c_float * 4)(*[float(x) for x in scene.background]),
+                           _ptr(grads) if len(grads) else None, len(grads))
         return d, arrs
 
     def scene_upload(self, geometry: Geometry, scene, rnd: np.ndarray | None = None):
@@ -322,7 +334,8 @@ class Batch:
             subs = np.ascontiguousarray(scene.substances, np.float32)
             pict = np.ascontiguousarray(scene.pict_data, np.uint8)
             uses = np.ascontiguousarray(scene.pict_uses, np.uint8)
-            self._keep += [geo, jobs, subs, pict, uses]
+            grads = _gradients(scene)
+            self._keep += [geo, jobs, subs, pict, uses, grads]
             a = arr[k]
             a.geo = geo.geo.ctypes.data if geo.geo.size else None
             a.geo_bytes = geo.geo.size
@@ -333,6 +346,8 @@ class Batch:
             a.pict_uses = uses.ctypes.data if uses.size else None
             a.n_pict_uses = uses.shape[0]
             a.jobs = jobs.handle
+            a.gradients = grads.ctypes.data if len(grads) else None
+            a.n_gradients = len(grads)
             a.width, a.height = scene.width, scene.height
             for c in range(4):
                 a.background[c] = float(scene.background[c])
@@ -390,16 +405,17 @@ class Adjoint:
                     np.ascontiguousarray(sc.substances, np.float32),
                     np.ascontiguousarray(sc.pict_data, np.uint8),
                     np.ascontiguousarray(sc.pict_uses, np.uint8),
-                    np.ascontiguousarray(target, np.float32)]
+                    np.ascontiguousarray(target, np.float32), _gradients(sc)]
             assert arrs[7].shape == (sc.height, sc.width, 4)
             self._keep.append(arrs)
-            tags, soff, ooff, pairs, subs, pict, uses, tgt = arrs
+            tags, soff, ooff, pairs, subs, pict, uses, tgt, grads = arrs
             o = _lib.Outlines(sc.n_shapes, *[a.ctypes.data for a in (tags, soff, ooff, pairs)])
             descs[i] = _lib.AdjointScene(o, _ptr(subs) if subs.size else None, subs.shape[0],
                                          _ptr(pict) if pict.size else None, pict.size,
                                          _ptr(uses) if uses.size else None, uses.size // 24,
                                          _ptr(tgt), sc.width, sc.height,
-                                         (C.c_float * 4)(*[float(x) for x in sc.background]))
+                                         (C.c_float * 4)(*[float(x) for x in sc.background]),
+                                         _ptr(grads) if len(grads) else None, len(grads))
         self.handle = C.c_void_p()
         self.r._check(self.L.gudni_adjoint_upload(self.r.ctx, len(items), descs, C.byref(self.handle)))
 

@@ -13,7 +13,7 @@ class Spec(C.Structure):
                 ("max_tiles_per_call", C.c_int32), ("max_thresholds", C.c_int32),
                 ("max_strands_per_tile", C.c_int32), ("max_shapes", C.c_int32),
                 ("random_field_size", C.c_int32), ("stochastic_factor", C.c_float),
-                ("coverage_mode", C.c_int32)]
+                ("coverage_mode", C.c_int32), ("texture_filter", C.c_int32)]
 
 
 class Outlines(C.Structure):
@@ -47,7 +47,8 @@ class BatchScene(C.Structure):
     _fields_ = [("geo", C.c_void_p), ("geo_bytes", C.c_size_t), ("substances", C.c_void_p),
                 ("n_substances", C.c_int32), ("pict_data", C.c_void_p), ("pict_bytes", C.c_size_t),
                 ("pict_uses", C.c_void_p), ("n_pict_uses", C.c_int32), ("jobs", C.c_void_p),
-                ("width", C.c_int32), ("height", C.c_int32), ("background", C.c_float * 4)]
+                ("width", C.c_int32), ("height", C.c_int32), ("background", C.c_float * 4),
+                ("gradients", C.c_void_p), ("n_gradients", C.c_int32)]
 
 
 class SceneDesc(C.Structure):
@@ -56,14 +57,15 @@ class SceneDesc(C.Structure):
                 ("n_substances", C.c_int32), ("pict_data", C.c_void_p), ("pict_bytes", C.c_size_t),
                 ("pict_uses", C.c_void_p), ("n_pict_uses", C.c_int32), ("random_field", C.c_void_p),
                 ("n_random", C.c_int32), ("width", C.c_int32), ("height", C.c_int32),
-                ("background", C.c_float * 4)]
+                ("background", C.c_float * 4), ("gradients", C.c_void_p), ("n_gradients", C.c_int32)]
 
 
 class AdjointScene(C.Structure):
     _fields_ = [("outlines", Outlines), ("rgba", C.c_void_p), ("n_substances", C.c_int32),
                 ("pict_data", C.c_void_p), ("pict_bytes", C.c_size_t), ("pict_uses", C.c_void_p),
                 ("n_pict_uses", C.c_int32), ("target", C.c_void_p), ("width", C.c_int32),
-                ("height", C.c_int32), ("background", C.c_float * 4)]
+                ("height", C.c_int32), ("background", C.c_float * 4), ("gradients", C.c_void_p),
+                ("n_gradients", C.c_int32)]
 
 
 # name -> (restype, argtypes)
@@ -76,6 +78,7 @@ SIGNATURES = {
     "gudni_upload_common": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_int32,
                                       C.c_void_p, C.c_size_t, C.c_void_p, C.c_int32, C.c_void_p,
                                       C.c_int32]),
+    "gudni_upload_gradients": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
     "gudni_raster_job": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
                                    C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                    C.c_void_p]),

@@ -417,7 +417,7 @@ __device__ __forceinline__ uint64_t label_tag(const SceneDesc &s, const int32_t 
 // Straight (non-premultiplied) colour of a shape at pixel (ax, ay).
 __device__ __forceinline__ float4 shape_color(const SceneDesc &s, uint64_t tag, int ax, int ay)
 {
-    return read_color(s, tag & kTagIdMask, (tag & kTagTypeMask) == kTagSolid, ax, ay);
+    return read_color(s, tag, ax, ay);
 }
 
 // Colour gradient of one section: gc = dL/dC for C = (P/A, A), S = (P, A)
@@ -880,11 +880,12 @@ struct gudni_adjoint_batch {
         std::vector<float> pairs;
         std::vector<float4> subs;
         std::vector<uint8_t> pict, uses;
+        std::vector<gudni_gradient> grads;
     };
     int device = 0;
     std::vector<Scene> scenes;
     DevBuf geo, subs, pict, uses, shapes, bounds, curves, points, target, image, gpix, d_points, d_rgba,
-        loss, tiles, tile_job, tile_scene, work, descs, orig_fast, orig_slow;
+        loss, tiles, tile_job, tile_scene, work, descs, orig_fast, orig_slow, grads;
     std::vector<size_t> points_off, subs_off, pix_off;
     std::vector<int32_t> work_begin, work_count;
     int32_t n_work = 0;
@@ -924,7 +925,8 @@ int build(gudni_ctx *ctx, gudni_adjoint_batch *a)
     std::vector<TileRec> tiles;
     std::vector<int32_t> tile_job, tile_scene;
     std::vector<uint32_t> work;
-    std::vector<size_t> geo_off, pict_off, uses_off, refs_off, curves_off;
+    std::vector<size_t> geo_off, pict_off, uses_off, refs_off, curves_off, grads_off;
+    std::vector<gudni_gradient> grads;
     a->points_off.clear(); a->subs_off.clear(); a->pix_off.clear();
     a->work_begin.clear(); a->work_count.clear();
     size_t pix = 0;
@@ -951,7 +953,7 @@ int build(gudni_ctx *ctx, gudni_adjoint_batch *a)
                 tj[i] = n_jobs - 1 - jb;
             }
         const CommonView cv{g.heap.data(), g.heap.size(), sc.subs.data(), (int32_t)sc.subs.size(),
-                            (int32_t)(sc.uses.size() / 24)};
+                            (int32_t)(sc.uses.size() / 24), (int32_t)sc.grads.size()};
         std::vector<float> xr(4 * j.refs.size() + 4);
         rc = validate_job(ctx, cv, j.refs.data(), (int32_t)j.refs.size(), j.tiles.data(),
                           (int32_t)j.tiles.size(), sc.width, sc.height, xr);
@@ -966,6 +968,8 @@ int build(gudni_ctx *ctx, gudni_adjoint_batch *a)
         pict.resize((pict.size() + 15) & ~(size_t)15);
         uses_off.push_back(uses.size() / 24);
         uses.insert(uses.end(), sc.uses.begin(), sc.uses.end());
+        grads_off.push_back(grads.size());
+        grads.insert(grads.end(), sc.grads.begin(), sc.grads.end());
         refs_off.push_back(refs.size());
         refs.insert(refs.end(), j.refs.begin(), j.refs.end());
         bounds.insert(bounds.end(), xr.begin(), xr.begin() + 4 * j.refs.size());
@@ -1007,6 +1011,7 @@ int build(gudni_ctx *ctx, gudni_adjoint_batch *a)
     CHECK_HIP(ctx, upload(ctx, a->tile_job, tile_job.data(), 4 * tile_job.size()));
     CHECK_HIP(ctx, upload(ctx, a->tile_scene, tile_scene.data(), 4 * tile_scene.size()));
     CHECK_HIP(ctx, upload(ctx, a->work, work.data(), 4 * work.size()));
+    CHECK_HIP(ctx, upload(ctx, a->grads, grads.data(), sizeof(gudni_gradient) * grads.size()));
     CHECK_HIP(ctx, a->image.ensure(16 * pix));
     CHECK_HIP(ctx, a->gpix.ensure(16 * pix));
     CHECK_HIP(ctx, a->d_points.ensure(8 * a->n_points));
@@ -1025,6 +1030,8 @@ int build(gudni_ctx *ctx, gudni_adjoint_batch *a)
         d.s.uses = (const PictUse *)a->uses.p + uses_off[s];
         d.s.out = nullptr;
         d.s.out_f32 = (float4 *)a->image.p + a->pix_off[s];
+        d.s.grads = (const float4 *)((const gudni_gradient *)a->grads.p + grads_off[s]);
+        d.s.filter = ctx->spec.texture_filter;
         for (int i = 0; i < 4; ++i) d.s.bg[i] = sc.bg[i];
         d.s.width = sc.width;
         d.s.height = sc.height;
@@ -1061,7 +1068,9 @@ int copy_scene(gudni_ctx *ctx, const gudni_adjoint_scene &in, gudni_adjoint_batc
         (in.n_substances && !in.rgba) || (in.pict_bytes && !in.pict_data) ||
         (in.n_pict_uses && !in.pict_uses))
         return set_error(ctx, GUDNI_ERR_ARG, "adjoint scene " + std::to_string(s) + ": bad argument");
-    const int rc = validate_uses(ctx, in.pict_uses, in.n_pict_uses, in.pict_bytes);
+    int rc = validate_uses(ctx, in.pict_uses, in.n_pict_uses, in.pict_bytes);
+    if (rc) return rc;
+    rc = validate_gradients(ctx, in.gradients, in.n_gradients);
     if (rc) return rc;
     sc.width = in.width;
     sc.height = in.height;
@@ -1078,6 +1087,8 @@ int copy_scene(gudni_ctx *ctx, const gudni_adjoint_scene &in, gudni_adjoint_batc
     if (in.n_substances) std::memcpy(sc.subs.data(), in.rgba, 16 * (size_t)in.n_substances);
     sc.pict.assign((const uint8_t *)in.pict_data, (const uint8_t *)in.pict_data + in.pict_bytes);
     sc.uses.assign((const uint8_t *)in.pict_uses, (const uint8_t *)in.pict_uses + 24 * (size_t)in.n_pict_uses);
+    sc.grads.assign((const gudni_gradient *)in.gradients,
+                    (const gudni_gradient *)in.gradients + (in.gradients ? in.n_gradients : 0));
     return check_scene(ctx, sc, s);
 }
 
@@ -1228,7 +1239,7 @@ void gudni_adjoint_free(gudni_adjoint_batch *a)
     DevBuf *bufs[] = {&a->geo, &a->subs, &a->pict, &a->uses, &a->shapes, &a->bounds, &a->curves,
                       &a->points, &a->target, &a->image, &a->gpix, &a->d_points, &a->d_rgba, &a->loss,
                       &a->tiles, &a->tile_job, &a->tile_scene, &a->work, &a->descs, &a->orig_fast,
-                      &a->orig_slow};
+                      &a->orig_slow, &a->grads};
     for (DevBuf *b : bufs) b->release();
     delete a;
 }

@@ -48,8 +48,8 @@ struct gudni_ctx {
     int fast_blocks = 0, slow_blocks = 0;
     int blocks_per_cu = 0;        // fast-pass blocks per CU (0: as many as fit, at most 4)
     // common buffers
-    gudni::DevBuf geo, subs, pict, uses, random;
-    int32_t n_substances = 0, n_uses = 0;
+    gudni::DevBuf geo, subs, pict, uses, random, grads;
+    int32_t n_substances = 0, n_uses = 0, n_grads = 0;
     // host copies of the common buffers, used to bounds-check job records
     std::vector<uint8_t> geo_host;
     std::vector<float4> subs_host;
@@ -112,8 +112,11 @@ struct CommonView {
     const uint8_t *geo;
     size_t geo_bytes;
     const float4 *subs;
-    int32_t n_subs, n_uses;
+    int32_t n_subs, n_uses, n_grads;
 };
+
+// Checks gradient records (kind 0 or 1, 1..4 stops).
+int validate_gradients(gudni_ctx *ctx, const void *grads, int32_t n);
 
 CommonView ctx_view(gudni_ctx *ctx);
 // Active (tile, quarter) work units of a tile list (isActiveThread).

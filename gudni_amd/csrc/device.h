@@ -43,6 +43,9 @@ constexpr uint64_t kTagCompoundMask = 0x3000000000000000ull;
 constexpr uint64_t kTagContinue = 0x1000000000000000ull;
 constexpr uint64_t kTagAdd = 0x2000000000000000ull;
 constexpr uint64_t kTagIdMask = 0x0FFFFFFFFFFFFFFFull;
+// Both type bits set: a gradient substance (an extension; the reference
+// knows solids and pictures only).
+constexpr uint64_t kTagGradient = 0xC000000000000000ull;
 
 constexpr int kWave = 64;
 constexpr int kFastCapLog2 = 7;           // 128-record rings in the fast pass
@@ -129,9 +132,11 @@ struct alignas(16) SceneDesc {
     const PictUse *uses;
     uint32_t *out;                // BGRA words
     float4 *out_f32;              // optional pre-quantisation colours
+    const float4 *grads;          // gradient records, 7 float4 each
     float bg[4];                  // background RGBA
     int32_t width, height;
-    int32_t pad0, pad1;
+    int32_t filter;               // picture sampling: 0 nearest, 1 bilinear
+    int32_t pad0;
 };
 
 struct Frame {
@@ -621,17 +626,84 @@ __device__ __forceinline__ int f2i_rtz(float v)
     return (int)v;
 }
 
-// readColor (Kernels.cl:1411-1442): solid colour or nearest texel
-__device__ float4 read_color(const SceneDesc &s, uint64_t sid, bool solid, int ax, int ay)
+__device__ __forceinline__ int f2i_floor(float v)
 {
-    const float4 sub = s.substances[sid];
-    if (solid) return sub;
-    const uint32_t pict_id = __float_as_uint(sub.x);
-    const PictUse u = s.uses[pict_id];
+    if (!(v == v)) return 0;
+    if (v >= 2147483647.0f) return 2147483647;
+    if (v <= -2147483648.0f) return (-2147483647 - 1);
+    return (int)floorf(v);
+}
+
+__device__ __forceinline__ float4 texel(const SceneDesc &s, const PictUse &u, long long x, long long y)
+{
+    if (x < 0 || y < 0 || x >= u.w || y >= u.h) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const uchar4 px = *reinterpret_cast<const uchar4 *>(s.pict + u.mem_offset + 4ll * (y * u.w + x));
+    return make_float4((float)px.x / 255.0f, (float)px.y / 255.0f, (float)px.z / 255.0f,
+                       (float)px.w / 255.0f);
+}
+
+// Gradient substance at the pixel's integer position (where pictures are
+// sampled too): linear from (x0,y0) to (x1,y1) or radial around (x0,y0) with
+// radius x1, stops interpolated piecewise linearly.
+__device__ float4 eval_gradient(const float4 *g, int ax, int ay)
+{
+    const float4 h0 = g[0], h1 = g[1], off = g[2];
+    const int kind = (int)__float_as_uint(h0.x);
+    int n = (int)__float_as_uint(h0.y);
+    const float x0 = h0.z, y0 = h0.w, x1 = h1.x, y1 = h1.y;
+    const float px = (float)ax, py = (float)ay;
+    float t;
+    if (kind == 1) {
+        const float ex = px - x0, ey = py - y0;
+        t = x1 > 0.0f ? sqrtf((ex * ex) + (ey * ey)) / x1 : 0.0f;
+    } else {
+        const float dx = x1 - x0, dy = y1 - y0;
+        const float den = (dx * dx) + (dy * dy);
+        t = den > 0.0f ? (((px - x0) * dx) + ((py - y0) * dy)) / den : 0.0f;
+    }
+    t = cl_min(cl_max(t, 0.0f), 1.0f);
+    n = n < 1 ? 1 : (n > 4 ? 4 : n);
+    const float o[4] = {off.x, off.y, off.z, off.w};
+    if (!(t > o[0])) return g[3];
+    int i = 1;
+    while (i < n && t > o[i]) ++i;
+    if (i == n) return g[3 + n - 1];
+    const float span = o[i] - o[i - 1];
+    const float f = span > 0.0f ? (t - o[i - 1]) / span : 0.0f;
+    const float4 a = g[3 + i - 1], b = g[3 + i];
+    return make_float4(a.x + ((b.x - a.x) * f), a.y + ((b.y - a.y) * f), a.z + ((b.z - a.z) * f),
+                       a.w + ((b.w - a.w) * f));
+}
+
+// readColor (Kernels.cl:1411-1442): solid colour, picture texel (nearest, as
+// the reference samples, or bilinear) or gradient
+__device__ float4 read_color(const SceneDesc &s, uint64_t tag, int ax, int ay)
+{
+    const uint64_t type = tag & kTagTypeMask;
+    const float4 sub = s.substances[tag & kTagIdMask];
+    if (type == kTagSolid) return sub;
+    const uint32_t id = __float_as_uint(sub.x);
+    if (type == kTagGradient) return eval_gradient(s.grads + 7 * (size_t)id, ax, ay);
+    const PictUse u = s.uses[id];
     float scale = u.scale;
     scale = scale < 0.0000001f ? 0.0000001f : scale;
-    const int rx = f2i_rtz(((float)ax / scale) - u.tx);
-    const int ry = f2i_rtz(((float)ay / scale) - u.ty);
+    const float sx = ((float)ax / scale) - u.tx, sy = ((float)ay / scale) - u.ty;
+    if (s.filter == 1) {
+        const float fx = sx - 0.5f, fy = sy - 0.5f;
+        const int ix = f2i_floor(fx), iy = f2i_floor(fy);
+        const float wx = fx - (float)ix, wy = fy - (float)iy;
+        const float4 t0 = texel(s, u, ix, iy), t1 = texel(s, u, (long long)ix + 1, iy);
+        const float4 t2 = texel(s, u, ix, (long long)iy + 1), t3 = texel(s, u, (long long)ix + 1, (long long)iy + 1);
+        const float ax0 = 1.0f - wx, ay0 = 1.0f - wy;
+        float4 r;
+        r.x = (((t0.x * ax0) + (t1.x * wx)) * ay0) + (((t2.x * ax0) + (t3.x * wx)) * wy);
+        r.y = (((t0.y * ax0) + (t1.y * wx)) * ay0) + (((t2.y * ax0) + (t3.y * wx)) * wy);
+        r.z = (((t0.z * ax0) + (t1.z * wx)) * ay0) + (((t2.z * ax0) + (t3.z * wx)) * wy);
+        r.w = (((t0.w * ax0) + (t1.w * wx)) * ay0) + (((t2.w * ax0) + (t3.w * wx)) * wy);
+        return r;
+    }
+    const int rx = f2i_rtz(sx);
+    const int ry = f2i_rtz(sy);
     if (rx >= 0 && ry >= 0 && rx < u.w && ry < u.h) {
         const uchar4 px = *reinterpret_cast<const uchar4 *>(
             s.pict + u.mem_offset + 4ll * ((long long)ry * u.w + rx));
@@ -667,7 +739,7 @@ __device__ __forceinline__ float4 group_composite(const SceneDesc &s, const Tile
         if ((tag & kTagTypeMask) == kTagSolid) {
             pm = tt->color[n];
         } else {
-            const float4 c = read_color(s, tag & kTagIdMask, false, ax, ay);
+            const float4 c = read_color(s, tag, ax, ay);
             pm = make_float4(c.x * c.w, c.y * c.w, c.z * c.w, c.w);
         }
         const float t = 1.0f - acc.w;
@@ -764,7 +836,7 @@ __device__ float4 determine_color(const Frame &f, const SceneDesc &s, const Stac
                 if (kCompact && solid) {
                     next = tt->color[top_bit];          // already in the mode's form
                 } else {
-                    next = read_color(s, sid, solid, ax, ay);
+                    next = read_color(s, tag, ax, ay);
                     positional = positional || !solid;
                     if (!ref_mode)
                         next = make_float4(next.x * next.w, next.y * next.w, next.z * next.w, next.w);

@@ -402,6 +402,7 @@ void gudni_default_spec(gudni_spec *s)
     s->random_field_size = 4096;
     s->stochastic_factor = 0.0f;
     s->coverage_mode = GUDNI_COVERAGE_ACCURATE;
+    s->texture_filter = GUDNI_FILTER_NEAREST;
 }
 
 int gudni_geometry_build(const gudni_outlines *scene, int32_t cw, int32_t ch, gudni_geometry **out)

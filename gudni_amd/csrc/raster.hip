@@ -212,9 +212,11 @@ static dev::SceneDesc single_desc(gudni_ctx *ctx, const float bg[4], int32_t w, 
     sd.uses = (const PictUse *)ctx->uses.p;
     sd.out = out;
     sd.out_f32 = (float4 *)out_f32;
+    sd.grads = (const float4 *)ctx->grads.p;
     for (int i = 0; i < 4; ++i) sd.bg[i] = bg[i];
     sd.width = w;
     sd.height = h;
+    sd.filter = ctx->spec.texture_filter;
     return sd;
 }
 
@@ -269,7 +271,19 @@ int read_status(gudni_ctx *ctx)
 CommonView ctx_view(gudni_ctx *ctx)
 {
     return CommonView{ctx->geo_host.data(), ctx->geo_host.size(), ctx->subs_host.data(),
-                      ctx->n_substances, ctx->n_uses};
+                      ctx->n_substances, ctx->n_uses, ctx->n_grads};
+}
+
+int validate_gradients(gudni_ctx *ctx, const void *grads, int32_t n)
+{
+    if (n < 0 || (n && !grads)) return set_error(ctx, GUDNI_ERR_ARG, "bad gradient table");
+    const gudni_gradient *g = (const gudni_gradient *)grads;
+    for (int32_t i = 0; i < n; ++i)
+        if ((g[i].kind != GUDNI_GRADIENT_LINEAR && g[i].kind != GUDNI_GRADIENT_RADIAL) ||
+            g[i].n_stops < 1 || g[i].n_stops > 4)
+            return set_error(ctx, GUDNI_ERR_ARG, "gradient " + std::to_string(i) +
+                                                     ": kind must be 0 or 1 with 1..4 stops");
+    return GUDNI_OK;
 }
 
 int validate_uses(gudni_ctx *ctx, const void *pict_uses, int32_t n_pict_uses,
@@ -313,10 +327,15 @@ int validate_job(gudni_ctx *ctx, const CommonView &cv, const ShapeRec *shapes,
         const uint64_t sid = r.tag & dev::kTagIdMask;
         if (sid >= (uint64_t)cv.n_subs)
             return set_error(ctx, GUDNI_ERR_ARG, "shape substance id outside the substance table");
-        if ((r.tag & dev::kTagTypeMask) != dev::kTagSolid) {
-            const uint32_t pid = __builtin_bit_cast(uint32_t, cv.subs[sid].x);
-            if (pid >= (uint32_t)cv.n_uses)
+        const uint64_t type = r.tag & dev::kTagTypeMask;
+        if (type != dev::kTagSolid) {
+            const uint32_t id = __builtin_bit_cast(uint32_t, cv.subs[sid].x);
+            if (type == dev::kTagGradient) {
+                if (id >= (uint32_t)cv.n_grads)
+                    return set_error(ctx, GUDNI_ERR_ARG, "gradient substance refers to a missing gradient");
+            } else if (id >= (uint32_t)cv.n_uses) {
                 return set_error(ctx, GUDNI_ERR_ARG, "picture substance refers to a missing usage");
+            }
         }
         size_t pos = 16u * (size_t)r.geo_start;
         // Extent of the shape's strands: x from the end points, y over every
@@ -373,6 +392,7 @@ int gudni_create(const gudni_spec *spec, int device, gudni_ctx **out)
     if (s.threads_per_tile != 256 || s.max_shapes < 1 || s.max_shapes > 127 ||
         s.max_thresholds < 1 || s.random_field_size < 1 ||
         (s.coverage_mode != GUDNI_COVERAGE_ACCURATE && s.coverage_mode != GUDNI_COVERAGE_REFERENCE) ||
+        (s.texture_filter != GUDNI_FILTER_NEAREST && s.texture_filter != GUDNI_FILTER_BILINEAR) ||
         (s.random_field_size & (s.random_field_size - 1))) {
         g_host_error = "gudni_create: unsupported spec (threads_per_tile must be 256, "
                        "max_shapes 1..127, random_field_size a power of two)";
@@ -399,7 +419,7 @@ void gudni_destroy(gudni_ctx *ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    DevBuf *bufs[] = {&ctx->geo, &ctx->subs, &ctx->pict, &ctx->uses, &ctx->random,
+    DevBuf *bufs[] = {&ctx->geo, &ctx->subs, &ctx->pict, &ctx->uses, &ctx->random, &ctx->grads,
                       &ctx->tiles, &ctx->tile_job, &ctx->shapes, &ctx->xrange, &ctx->work, &ctx->out,
                       &ctx->out_f32, &ctx->fast_thr, &ctx->fast_hdr, &ctx->fast_sidx,
                       &ctx->slow_thr, &ctx->slow_hdr, &ctx->slow_sidx, &ctx->control, &ctx->dump,
@@ -454,6 +474,19 @@ int gudni_upload_common(gudni_ctx *ctx, const void *geo, size_t geo_bytes, const
     CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->n_substances = n_substances;
     ctx->n_uses = n_pict_uses;
+    ctx->frame_ready = false;
+    return GUDNI_OK;
+}
+
+int gudni_upload_gradients(gudni_ctx *ctx, const void *gradients, int32_t n_gradients)
+{
+    if (!ctx) return GUDNI_ERR_ARG;
+    int rc = validate_gradients(ctx, gradients, n_gradients);
+    if (rc) return rc;
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    CHECK_HIP(ctx, upload(ctx, ctx->grads, (const uint8_t *)gradients, sizeof(gudni_gradient) * (size_t)n_gradients));
+    CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->n_grads = n_gradients;
     ctx->frame_ready = false;
     return GUDNI_OK;
 }
@@ -672,6 +705,8 @@ int gudni_scene_upload(gudni_ctx *ctx, const gudni_scene *sc)
                                  sc->pict_data, sc->pict_bytes, sc->pict_uses, sc->n_pict_uses,
                                  sc->random_field, sc->n_random);
     if (rc) return rc;
+    rc = gudni_upload_gradients(ctx, sc->gradients, sc->n_gradients);
+    if (rc) return rc;
     std::vector<float> bounds(4 * (size_t)sc->n_shapes + 4);
     rc = validate_job(ctx, ctx_view(ctx), (const ShapeRec *)sc->shapes, sc->n_shapes, nullptr, 0,
                       sc->width, sc->height, bounds);
@@ -839,7 +874,7 @@ struct gudni_batch {
     int32_t n_scenes = 0;
     int device = 0;
     DevBuf geo, subs, pict, uses, shapes, bounds, tiles, tile_job, tile_scene, work, scenes, done,
-        frames, random;
+        frames, random, grads;
     std::vector<size_t> frame_off;
     std::vector<int32_t> width, height;
     std::vector<int32_t> work_begin, work_count;
@@ -867,7 +902,8 @@ int gudni_batch_upload(gudni_ctx *ctx, int32_t n_scenes, const gudni_batch_scene
     std::vector<TileRec> tiles;
     std::vector<int32_t> tile_job, tile_scene, wbegin, wcount;
     std::vector<uint32_t> work;
-    std::vector<size_t> geo_off, subs_off, pict_off, uses_off, refs_off, frame_off;
+    std::vector<size_t> geo_off, subs_off, pict_off, uses_off, refs_off, frame_off, grads_off;
+    std::vector<gudni_gradient> grads;
     size_t frame_words = 0;
     for (int32_t s = 0; s < n_scenes; ++s) {
         const gudni_batch_scene &sc = scenes[s];
@@ -877,12 +913,14 @@ int gudni_batch_upload(gudni_ctx *ctx, int32_t n_scenes, const gudni_batch_scene
             return set_error(ctx, GUDNI_ERR_ARG, "gudni_batch_upload: bad scene " + std::to_string(s));
         int rc = validate_uses(ctx, sc.pict_uses, sc.n_pict_uses, sc.pict_bytes);
         if (rc) return rc;
+        rc = validate_gradients(ctx, sc.gradients, sc.n_gradients);
+        if (rc) return rc;
         gudni_jobs_view v;
         gudni_jobs_view_get(sc.jobs, &v);
         std::vector<float4> sub_host((size_t)sc.n_substances);
         if (sc.n_substances) std::memcpy(sub_host.data(), sc.substances, 16u * (size_t)sc.n_substances);
         const CommonView cv{(const uint8_t *)sc.geo, sc.geo_bytes, sub_host.data(), sc.n_substances,
-                            sc.n_pict_uses};
+                            sc.n_pict_uses, sc.n_gradients};
         std::vector<TileRec> t(( const TileRec *)v.tiles, (const TileRec *)v.tiles + v.n_tiles);
         std::vector<int32_t> tj((size_t)v.n_tiles);
         for (int j = 0; j < v.n_jobs; ++j)
@@ -899,6 +937,10 @@ int gudni_batch_upload(gudni_ctx *ctx, int32_t n_scenes, const gudni_batch_scene
         geo.resize(align_up(geo.size(), 16));
         subs_off.push_back(subs.size());
         subs.insert(subs.end(), sub_host.begin(), sub_host.end());
+        grads_off.push_back(grads.size());
+        if (sc.n_gradients)
+            grads.insert(grads.end(), (const gudni_gradient *)sc.gradients,
+                         (const gudni_gradient *)sc.gradients + sc.n_gradients);
         pict_off.push_back(pict.size());
         if (sc.pict_bytes)
             pict.insert(pict.end(), (const uint8_t *)sc.pict_data, (const uint8_t *)sc.pict_data + sc.pict_bytes);
@@ -957,6 +999,7 @@ int gudni_batch_upload(gudni_ctx *ctx, int32_t n_scenes, const gudni_batch_scene
     if (e == hipSuccess) e = up(b->tile_scene, tile_scene.data(), 4 * tile_scene.size());
     if (e == hipSuccess) e = up(b->work, work.data(), 4 * work.size());
     if (e == hipSuccess) e = up(b->random, random_field, 4 * (size_t)n_random);
+    if (e == hipSuccess) e = up(b->grads, grads.data(), sizeof(gudni_gradient) * grads.size());
     if (e == hipSuccess) e = b->frames.ensure(4 * frame_words);
     if (e == hipSuccess) e = hipMemset(b->frames.p, 0, 4 * frame_words);
     if (e == hipSuccess) e = b->done.ensure(8 * ((size_t)n_scenes + 1));
@@ -973,6 +1016,8 @@ int gudni_batch_upload(gudni_ctx *ctx, int32_t n_scenes, const gudni_batch_scene
         d.uses = (const PictUse *)b->uses.p + uses_off[s];
         d.out = (uint32_t *)b->frames.p + frame_off[s];
         d.out_f32 = nullptr;
+        d.grads = (const float4 *)((const gudni_gradient *)b->grads.p + grads_off[s]);
+        d.filter = ctx->spec.texture_filter;
         for (int i = 0; i < 4; ++i) d.bg[i] = scenes[s].background[i];
         d.width = scenes[s].width;
         d.height = scenes[s].height;
@@ -1103,7 +1148,7 @@ void gudni_batch_free(gudni_batch *b)
     if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
     DevBuf *bufs[] = {&b->geo, &b->subs, &b->pict, &b->uses, &b->shapes, &b->bounds, &b->tiles,
                       &b->tile_job, &b->tile_scene, &b->work, &b->scenes, &b->done, &b->frames,
-                      &b->random};
+                      &b->random, &b->grads};
     for (DevBuf *d : bufs) d->release();
     delete b;
 }

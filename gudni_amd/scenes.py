@@ -21,6 +21,8 @@ MASK64 = (1 << 64) - 1
 
 TAG_SOLID = 0x8000000000000000
 TAG_PICTURE = 0x4000000000000000
+# both type bits: a gradient substance (an extension; the reference has none)
+TAG_GRADIENT = 0xC000000000000000
 TAG_ADD = 0x2000000000000000
 TAG_SUBTRACT = 0x3000000000000000
 TAG_ID_MASK = 0x0FFFFFFFFFFFFFFF
@@ -72,6 +74,9 @@ class Scene:
     pict_uses: np.ndarray               # uint8 [n_uses, 24] PictureUse records
     background: tuple = (1.0, 1.0, 1.0, 1.0)
     name: str = ""
+    # gradient records, 28 float32 words (112 B) each: kind (i32: 0 linear,
+    # 1 radial), stop count (i32), x0 y0 x1 y1, 2 unused, 4 offsets, 4 RGBA
+    gradients: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros((0, 28), np.float32))
 
     @property
     def n_shapes(self) -> int:
@@ -102,7 +107,7 @@ def picture_substance(pict_id: int) -> np.ndarray:
 
 
 def build_scene(width, height, shapes, substances, pict_data=None, pict_uses=(),
-                background=(1.0, 1.0, 1.0, 1.0), name=""):
+                background=(1.0, 1.0, 1.0, 1.0), name="", gradients=None):
     """shapes: list of (tag, [outline, ...]) with outline an (k, 4) array of
     curve pairs (on.x, on.y, off.x, off.y)."""
     tags = np.array([t for t, _ in shapes], dtype=np.uint64)
@@ -118,9 +123,11 @@ def build_scene(width, height, shapes, substances, pict_data=None, pict_uses=(),
     pairs = np.concatenate(chunks) if chunks else np.zeros((0, 4), np.float32)
     subs = np.asarray(substances, dtype=np.float32).reshape(-1, 4)
     pd = np.zeros(4, np.uint8) if pict_data is None else np.asarray(pict_data, np.uint8).ravel()
+    grads = np.zeros((0, 28), np.float32) if gradients is None else \
+        np.ascontiguousarray(np.asarray(gradients, np.float32).reshape(-1, 28))
     return Scene(width, height, tags, np.array(so, np.int32), np.array(oo, np.int32),
                  np.ascontiguousarray(pairs), np.ascontiguousarray(subs), pd,
-                 pict_use_records(list(pict_uses)), tuple(background), name)
+                 pict_use_records(list(pict_uses)), tuple(background), name, grads)
 
 
 def straight_pairs(points) -> np.ndarray:
@@ -146,17 +153,67 @@ def picture_tag(sid: int) -> int:
     return TAG_PICTURE | TAG_ADD | (sid & TAG_ID_MASK)
 
 
+def gradient_tag(sid: int) -> int:
+    return TAG_GRADIENT | TAG_ADD | (sid & TAG_ID_MASK)
+
+
+def gradient_record(kind: int, p0, p1, offsets, colors) -> np.ndarray:
+    """One 112-byte gradient record: kind 0 runs linearly from p0 to p1, kind
+    1 radially around p0 with radius p1[0]; 1..4 stops (offset, RGBA)."""
+    r = np.zeros(28, np.float32)
+    iv = r.view(np.int32)
+    n = len(offsets)
+    if kind not in (0, 1) or not 1 <= n <= 4:
+        raise ValueError(f"gradient kind {kind} with {n} stops: kind 0 or 1, 1..4 stops")
+    iv[0], iv[1] = int(kind), n
+    r[2:4] = p0
+    r[4:6] = p1
+    r[8:8 + n] = offsets
+    r[12:12 + 4 * n] = np.asarray(colors, np.float32).ravel()
+    return r
+
+
+def gradient_substance(grad_id: int) -> np.ndarray:
+    s = np.zeros(4, np.float32)
+    s.view(np.uint32)[0] = grad_id
+    return s
+
+
+def random_gradients(rng: "SplitMix64", centre, half) -> np.ndarray:
+    """Linear or radial gradients with 2-4 stops over boxes (centre, half size)."""
+    n = len(centre)
+    out = np.zeros((n, 28), np.float32)
+    kind = (rng.next(n) & np.uint64(1)).astype(np.int64)
+    stops = rng.integers(2, 4, n)
+    ang = rng.uniform(n) * (2.0 * math.pi)
+    u = rng.uniform(4 * n).reshape(n, 4)
+    cols = (rng.next(16 * n) & np.uint64(0xFF)).astype(np.float32).reshape(n, 4, 4) / 255.0
+    for i in range(n):
+        cx, cy = float(centre[i][0]), float(centre[i][1])
+        h = max(float(half[i]), 0.5)
+        m = int(stops[i])
+        offs = np.sort(np.concatenate([[0.0, 1.0], u[i, :m - 2]]))[:m]
+        offs[-1] = 1.0
+        if kind[i] == 0:
+            dx, dy = math.cos(ang[i]) * h, math.sin(ang[i]) * h
+            out[i] = gradient_record(0, (cx - dx, cy - dy), (cx + dx, cy + dy), offs, cols[i, :m])
+        else:
+            out[i] = gradient_record(1, (cx, cy), (h * (1.0 + u[i, 3]), 0.0), offs, cols[i, :m])
+    return out
+
+
 def _assemble(width, height, tags, k, pairs, substances, pict_data, pict_uses, name,
-              background=(1.0, 1.0, 1.0, 1.0)):
+              background=(1.0, 1.0, 1.0, 1.0), gradients=None):
     """One outline per shape with k[i] pairs each."""
     n = len(k)
     oo = np.zeros(n + 1, np.int32)
     oo[1:] = np.cumsum(k)
     so = np.arange(n + 1, dtype=np.int32)
+    grads = np.zeros((0, 28), np.float32) if gradients is None else gradients
     return Scene(width, height, np.asarray(tags, np.uint64), so, oo,
                  np.ascontiguousarray(pairs, dtype=np.float32),
                  np.ascontiguousarray(substances, dtype=np.float32),
-                 pict_data, pict_uses, background, name)
+                 pict_data, pict_uses, background, name, grads)
 
 
 def random_colors(rng: SplitMix64, n: int) -> np.ndarray:
@@ -206,14 +263,16 @@ def _blob_pairs(rng, cx, cy, r, k):
 
 def config2(n_blobs: int = 180_000, n_glyphs: int = 20_000, size: int = 4096,
             seed: int = SEEDS[2], texture_side: int = 1024, radius_median: float = 8.0,
-            em_scale: float = 1.0, picture_share: int = 1, k_range=(4, 16)) -> Scene:
+            em_scale: float = 1.0, picture_share: int = 1, k_range=(4, 16),
+            gradient_share: int = 2) -> Scene:
     """4096x4096, 200k shapes, ~2.1M quadratics.  Blobs: centre uniform,
     radius 8*exp(1.2*N(0,1)) (median 8 px), 4-16 quadratics.  Glyph-like
-    outlines: 4-30 random quadratics inside a 12-48 px em box.  Substances:
-    90% solid random RGBA8, 10% a seeded 1024^2 RGBA8 noise texture sampled
-    nearest (the reference's only picture mode).  The gradient share of the
-    workload is drawn as solid (first stop): the reference has no gradient
-    substance."""
+    outlines: 4-30 random quadratics inside a 12-48 px em box.  Substances (in
+    tenths): picture_share a seeded 1024^2 RGBA8 noise texture (sampled as the
+    spec's texture filter says: nearest is the reference's only mode),
+    gradient_share linear/radial gradients of 2-4 stops over the shape's box
+    (an extension; gradient_share=0 is the reference-runnable copy, which
+    draws those shapes solid), the rest solid random RGBA8."""
     rng = SplitMix64(seed)
     cx = rng.uniform(n_blobs) * size
     cy = rng.uniform(n_blobs) * size
@@ -243,6 +302,7 @@ def config2(n_blobs: int = 180_000, n_glyphs: int = 20_000, size: int = 4096,
     # substances
     kind = rng.next(n) % np.uint64(10)
     picture = kind >= np.uint64(10 - picture_share)
+    gradient = (kind >= np.uint64(10 - picture_share - gradient_share)) & ~picture
     colors = random_colors(rng, n)
     tex = noise_texture(rng, texture_side)
     subs = colors.copy()
@@ -257,13 +317,24 @@ def config2(n_blobs: int = 180_000, n_glyphs: int = 20_000, size: int = 4096,
     sub_u32 = subs.view(np.uint32)
     sub_u32[picture, 0] = pid[picture].astype(np.uint32)
     sub_u32[picture, 1:] = 0
+    grads = None
+    if gradient.any():
+        gi = np.nonzero(gradient)[0]
+        pts = pairs.reshape(-1, 4)
+        lo = np.minimum.reduceat(np.minimum(pts[:, :2], pts[:, 2:]), first, axis=0)[gi]
+        hi = np.maximum.reduceat(np.maximum(pts[:, :2], pts[:, 2:]), first, axis=0)[gi]
+        grads = random_gradients(rng, (lo + hi) * 0.5, np.max(hi - lo, axis=1) * 0.5)
+        tags[gi] = np.uint64(TAG_GRADIENT | TAG_ADD) | ids[gi]
+        sub_u32[gi, 0] = np.arange(len(gi), dtype=np.uint32)
+        sub_u32[gi, 1:] = 0
     # each picture usage maps the texture's origin onto a random point of the canvas
     ox = rng.uniform(len(pic_idx)) * size
     oy = rng.uniform(len(pic_idx)) * size
     for j in range(len(pic_idx)):
         uses.append((-float(np.float32(ox[j])), -float(np.float32(oy[j])), texture_side,
                      texture_side, 0, 1.0))
-    return _assemble(size, size, tags, k, pairs, subs, tex, pict_use_records(uses), "cfg2")
+    return _assemble(size, size, tags, k, pairs, subs, tex, pict_use_records(uses), "cfg2",
+                     gradients=grads)
 
 
 def scaled_config2(size: int, n_shapes: int, seed: int, glyph_share: float = 0.1) -> Scene:
@@ -273,16 +344,18 @@ def scaled_config2(size: int, n_shapes: int, seed: int, glyph_share: float = 0.1
                    texture_side=256)
 
 
-def batch_scene(size: int, n_shapes: int, seed: int, k_range=(3, 12)) -> Scene:
+def batch_scene(size: int, n_shapes: int, seed: int, k_range=(3, 12), gradient_share: int = 2,
+                picture_share: int = 0, texture_side: int = 16) -> Scene:
     """One scene of the batched workload: config-2-style blobs and glyph
     outlines on a size x size canvas, radii scaled so that the expected
-    overdraw matches the 4096^2 scene's, solid substances only."""
+    overdraw matches the 4096^2 scene's; solid and (gradient_share tenths)
+    gradient substances."""
     radius = 8.0 * (size / 4096.0) * math.sqrt(200_000.0 / max(n_shapes, 1))
     radius = float(min(max(radius, 1.5), size / 8.0))
     n_g = max(1, n_shapes // 10)
     sc = config2(n_blobs=max(1, n_shapes - n_g), n_glyphs=n_g, size=size, seed=seed,
-                 texture_side=16, radius_median=radius, em_scale=max(0.25, radius / 8.0),
-                 picture_share=0, k_range=k_range)
+                 texture_side=texture_side, radius_median=radius, em_scale=max(0.25, radius / 8.0),
+                 picture_share=picture_share, k_range=k_range, gradient_share=gradient_share)
     sc.name = f"batch{size}x{n_shapes}"
     return sc
 
@@ -296,3 +369,44 @@ def config3(n_scenes: int = 256, seed: int = SEEDS[3]):
     seeds = rng.next(n_scenes)
     return [batch_scene(int(s), int(c), int(z & np.uint64(0x7FFFFFFF)))
             for s, c, z in zip(sides, counts, seeds)]
+
+
+def perturb(scene: Scene, seed: int, point_sigma: float = 4.0, color_sigma: float = 0.1) -> Scene:
+    """A copy whose outline points move by N(0, point_sigma) px and whose solid
+    colours move by N(0, color_sigma), clipped to [0, 1] (the learnable
+    parameters of the differentiable workloads)."""
+    rng = SplitMix64(seed)
+    pairs = scene.pairs.astype(np.float64) + point_sigma * rng.normal(scene.pairs.size).reshape(scene.pairs.shape)
+    subs = np.array(scene.substances, np.float32, copy=True)
+    solid = (np.asarray(scene.tags, np.uint64) & np.uint64(0xC000000000000000)) == np.uint64(TAG_SOLID)
+    sid = (np.asarray(scene.tags, np.uint64) & np.uint64(TAG_ID_MASK)).astype(np.int64)
+    ids = np.unique(sid[solid])
+    noise = color_sigma * rng.normal(4 * len(ids)).reshape(-1, 4)
+    subs[ids] = np.clip(subs[ids].astype(np.float64) + noise, 0.0, 1.0).astype(np.float32)
+    return dataclasses.replace(scene, pairs=pairs.astype(np.float32), substances=subs,
+                               name=scene.name + "-perturbed")
+
+
+def config4(seed: int = SEEDS[4]) -> Scene:
+    """Differentiable render: a 1024x1024 scene of 2,000 shapes (the config-2
+    generator scaled, solid substances); the target is its render and the
+    parameters a perturbed copy (``perturb``)."""
+    sc = batch_scene(1024, 2000, seed & 0x7FFFFFFF, gradient_share=0)
+    sc.name = "cfg4"
+    return sc
+
+
+def config5(n_scenes: int = 64, seed: int = SEEDS[5]):
+    """Batched differentiable render: 64 scenes of 512x512 with 500-5,000
+    shapes (log-uniform) of 3-8 quadratics, solid plus a 1024^2 RGBA8 noise
+    texture (sampled with the spec's texture filter)."""
+    rng = SplitMix64(seed)
+    counts = np.exp(np.log(500.0) + rng.uniform(n_scenes) * (np.log(5000.0) - np.log(500.0)))
+    seeds = rng.next(n_scenes)
+    out = []
+    for c, z in zip(counts, seeds):
+        sc = batch_scene(512, int(c), int(z & np.uint64(0x7FFFFFFF)), k_range=(3, 8), gradient_share=0,
+                         picture_share=1, texture_side=1024)
+        sc.name = f"cfg5-{int(c)}"
+        out.append(sc)
+    return out

@@ -49,7 +49,36 @@ typedef struct gudni_spec {
     float   stochastic_factor;    /* STOCHASTIC_FACTOR    = 0.0               */
     int32_t coverage_mode;        /* GUDNI_COVERAGE_ACCURATE (default) or
                                      GUDNI_COVERAGE_REFERENCE                 */
+    int32_t texture_filter;       /* GUDNI_FILTER_NEAREST (default: the
+                                     reference's sampling, Kernels.cl:856-860)
+                                     or GUDNI_FILTER_BILINEAR                 */
 } gudni_spec;
+
+/* Picture sampling at the pixel's integer position p (Kernels.cl:1426):
+ * s = p / scale - translate.  NEAREST takes the texel at rtz(s); BILINEAR
+ * blends the four texels around s - 0.5 (texels outside the picture are
+ * transparent). */
+#define GUDNI_FILTER_NEAREST  0
+#define GUDNI_FILTER_BILINEAR 1
+
+/* Gradient substances (no counterpart in the reference): a shape whose tag
+ * has both substance-type bits set (0xC000...) takes its colour from the
+ * gradient record its substance's first word indexes, evaluated at the
+ * pixel's integer position like a picture.  112 bytes:
+ *   kind 0: linear, t = ((p - p0) . (p1 - p0)) / |p1 - p0|^2
+ *   kind 1: radial, t = |p - p0| / p1.x
+ * t clamped to [0, 1]; the colour interpolates the straight RGBA stops
+ * linearly between offsets (offsets ascending, 1..4 stops). */
+typedef struct gudni_gradient {
+    int32_t kind;
+    int32_t n_stops;
+    float   x0, y0, x1, y1;
+    float   unused[2];
+    float   offset[4];
+    float   rgba[4][4];
+} gudni_gradient;
+#define GUDNI_GRADIENT_LINEAR 0
+#define GUDNI_GRADIENT_RADIAL 1
 
 /* Evaluation of section areas and colours.  Records, their order, the
  * section structure and the shapes each colour walk visits are identical in
@@ -93,6 +122,10 @@ int gudni_upload_common(gudni_ctx *ctx,
                         const void *pict_data, size_t pict_bytes,
                         const void *pict_uses, int32_t n_pict_uses,
                         const float *random_field, int32_t n_random);
+
+/* Gradient records (gudni_gradient) for the scene of gudni_upload_common;
+ * an extension with no counterpart in the reference. */
+int  gudni_upload_gradients(gudni_ctx *ctx, const void *gradients, int32_t n_gradients);
 
 /* Replaces generateCall (OpenCL/CallKernels.hs:88-179): runs the
  * generateThresholds / sortThresholds / renderThresholds work of one
@@ -195,6 +228,7 @@ typedef struct gudni_scene {
     const float *random_field;  int32_t n_random;
     int32_t width, height;
     float background[4];
+    const void *gradients;      int32_t n_gradients;  /* gudni_gradient records */
 } gudni_scene;
 /* Uploads and validates the scene (the common buffers of gudni_upload_common
  * plus the shape records and boxes). */
@@ -237,6 +271,7 @@ typedef struct gudni_adjoint_scene {
     const float *target;                /* width*height RGBA floats          */
     int32_t width, height;
     float background[4];
+    const void *gradients;  int32_t n_gradients;   /* gudni_gradient records */
 } gudni_adjoint_scene;
 typedef struct gudni_adjoint_batch gudni_adjoint_batch;
 /* Serialises, bins and uploads n_scenes scenes with their targets. */
@@ -288,6 +323,7 @@ typedef struct gudni_batch_scene {
     const gudni_jobs *jobs;
     int32_t width, height;
     float background[4];
+    const void *gradients;    int32_t n_gradients;    /* gudni_gradient records */
 } gudni_batch_scene;
 
 typedef struct gudni_batch gudni_batch;

@@ -64,6 +64,9 @@
 #define TAG_CONTINUE           0x1000000000000000ull
 #define TAG_ADD                0x2000000000000000ull
 #define TAG_SUBSTANCEID_MASK   0x0FFFFFFFFFFFFFFFull
+/* Extension with no counterpart in the reference: a third substance type
+ * (both type bits set) whose colour is a linear or radial gradient. */
+#define TAG_GRADIENT           0xC000000000000000ull
 
 static inline float cl_max(float x, float y) { return x < y ? y : x; }
 static inline float cl_min(float x, float y) { return y < x ? y : x; }
@@ -82,6 +85,7 @@ void go_default_spec(go_spec *s)
     s->random_field_size = 4096;
     s->stochastic_factor = 0.0f;
     s->coverage_mode = 0;
+    s->texture_filter = 0;
 }
 
 /* ======================================================================== */
@@ -857,6 +861,7 @@ typedef struct {
 typedef struct {
     const float *substances; const uint8_t *pict; const uint8_t *uses;
     const uint8_t *shapes; const float *random;
+    const float *gradients; int filter;
     float bg[4]; int absx, absy;
 } colorctx_t;
 
@@ -895,19 +900,83 @@ static int f2i_rtz(float v)
     return (int)v;
 }
 
-/* readColor (Kernels.cl:1411-1442) */
-static void read_color(const colorctx_t *cs, uint64_t sid, int solid, float *out)
+static int f2i_floor(float v)
+{   /* floor, saturating like f2i_rtz */
+    if (!(v == v)) return 0;
+    if (v >= 2147483647.0f) return 2147483647;
+    if (v <= -2147483648.0f) return (-2147483647 - 1);
+    return (int)floorf(v);
+}
+
+/* Gradient substance (extension): 28 floats per record: kind (0 linear from
+ * (x0,y0) to (x1,y1), 1 radial around (x0,y0) with radius x1), stop count
+ * (1..4), x0 y0 x1 y1, two unused, 4 stop offsets, 4 straight RGBA stops.
+ * Evaluated at the pixel's integer position, as pictures are. */
+static void eval_gradient(const float *g, int ax, int ay, float *out)
 {
+    int32_t kind, n;
+    memcpy(&kind, g, 4); memcpy(&n, g + 1, 4);
+    const float x0 = g[2], y0 = g[3], x1 = g[4], y1 = g[5];
+    const float *off = g + 8, *col = g + 12;
+    const float px = (float)ax, py = (float)ay;
+    float t;
+    if (kind == 1) {
+        const float ex = px - x0, ey = py - y0;
+        t = x1 > 0.0f ? sqrtf((ex * ex) + (ey * ey)) / x1 : 0.0f;
+    } else {
+        const float dx = x1 - x0, dy = y1 - y0;
+        const float den = (dx * dx) + (dy * dy);
+        t = den > 0.0f ? (((px - x0) * dx) + ((py - y0) * dy)) / den : 0.0f;
+    }
+    t = cl_min(cl_max(t, 0.0f), 1.0f);
+    if (n < 1) n = 1;
+    if (n > 4) n = 4;
+    if (!(t > off[0])) { memcpy(out, col, 16); return; }
+    int i = 1;
+    while (i < n && t > off[i]) i++;
+    if (i == n) { memcpy(out, col + 4 * (n - 1), 16); return; }
+    const float span = off[i] - off[i - 1];
+    const float f = span > 0.0f ? (t - off[i - 1]) / span : 0.0f;
+    for (int c = 0; c < 4; c++) out[c] = col[4 * (i - 1) + c] + ((col[4 * i + c] - col[4 * (i - 1) + c]) * f);
+}
+
+/* readColor (Kernels.cl:1411-1442): solid colour, picture texel (nearest as
+ * the reference samples, or bilinear), or gradient */
+static void read_color(const colorctx_t *cs, uint64_t tag, float *out)
+{
+    const uint64_t sid = tag & TAG_SUBSTANCEID_MASK;
+    const uint64_t type = tag & TAG_SUBSTANCETYPE_MASK;
     const float *sub = cs->substances + 4 * sid;
-    if (solid) { memcpy(out, sub, 16); return; }
-    uint32_t pict_id; memcpy(&pict_id, sub, 4);
-    const uint8_t *u = cs->uses + 24L * pict_id;
+    if (type == TAG_SOLID) { memcpy(out, sub, 16); return; }
+    uint32_t id; memcpy(&id, sub, 4);
+    if (type == TAG_GRADIENT) { eval_gradient(cs->gradients + 28L * id, cs->absx, cs->absy, out); return; }
+    const uint8_t *u = cs->uses + 24L * id;
     float tx, ty, scale; int32_t w, h, off;
     memcpy(&tx, u, 4); memcpy(&ty, u + 4, 4); memcpy(&w, u + 8, 4); memcpy(&h, u + 12, 4);
     memcpy(&off, u + 16, 4); memcpy(&scale, u + 20, 4);
     scale = scale < 0.0000001f ? 0.0000001f : scale;
-    int rx = f2i_rtz(((float)cs->absx / scale) - tx);
-    int ry = f2i_rtz(((float)cs->absy / scale) - ty);
+    const float sx = ((float)cs->absx / scale) - tx, sy = ((float)cs->absy / scale) - ty;
+    if (cs->filter == 1) {
+        const float fx = sx - 0.5f, fy = sy - 0.5f;
+        const int ix = f2i_floor(fx), iy = f2i_floor(fy);
+        const float wx = fx - (float)ix, wy = fy - (float)iy;
+        float t[4][4];
+        for (int j = 0; j < 4; j++) {
+            const long qx = (long)ix + (j & 1), qy = (long)iy + (j >> 1);
+            if (qx >= 0 && qy >= 0 && qx < w && qy < h) {
+                const uint8_t *px = cs->pict + off + 4L * (qy * w + qx);
+                for (int c = 0; c < 4; c++) t[j][c] = (float)px[c] / 255.0f;
+            } else t[j][0] = t[j][1] = t[j][2] = t[j][3] = 0.0f;
+        }
+        for (int c = 0; c < 4; c++) {
+            const float top = (t[0][c] * (1.0f - wx)) + (t[1][c] * wx);
+            const float bot = (t[2][c] * (1.0f - wx)) + (t[3][c] * wx);
+            out[c] = (top * (1.0f - wy)) + (bot * wy);
+        }
+        return;
+    }
+    int rx = f2i_rtz(sx);
+    int ry = f2i_rtz(sy);
     if (rx >= 0 && ry >= 0 && rx < w && ry < h) {
         const uint8_t *px = cs->pict + off + 4L * ((long)ry * w + rx);
         for (int i = 0; i < 4; i++) out[i] = (float)px[i] / 255.0f;
@@ -930,7 +999,7 @@ static void grouped_color(col_t *c, const colorctx_t *cs, float *base)
             if ((n >> 5) != g) continue;
             uint64_t tag; memcpy(&tag, cs->shapes + 16L * c->shape_indices[b], 8);
             float col[4];
-            read_color(cs, tag & TAG_SUBSTANCEID_MASK, (tag & TAG_SUBSTANCETYPE_MASK) == TAG_SOLID, col);
+            read_color(cs, tag, col);
             float pm[4] = { col[0] * col[3], col[1] * col[3], col[2] * col[3], col[3] };
             float t = 1.0f - ga;
             for (int i = 0; i < 3; i++) gp[i] = gp[i] + pm[i] * t;
@@ -987,7 +1056,7 @@ static void determine_color(col_t *c, const colorctx_t *cs, float *base, double 
                 should_composite = 0;
             }
             if (sid != last_id) {
-                read_color(cs, sid, (tag & TAG_SUBSTANCETYPE_MASK) == TAG_SOLID, next);
+                read_color(cs, tag, next);
                 should_composite = 1;
                 last_is_set = is_add || is_cont;
             }
@@ -1306,6 +1375,7 @@ static int run_column(const go_spec *sp, const go_job_in *in, int tile, int colu
     }
     colorctx_t cs;
     cs.substances = in->substances; cs.pict = in->pict_data; cs.uses = in->pict_uses;
+    cs.gradients = in->gradients; cs.filter = sp->texture_filter;
     cs.shapes = in->shapes; cs.random = in->random_field;
     memcpy(cs.bg, in->background, 16);
     render_column(&c, &cs, out, out_f32, out_f64);

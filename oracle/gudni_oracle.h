@@ -35,6 +35,8 @@ typedef struct go_spec {
     int32_t coverage_mode;        /* 0: section mid-x evaluated relative to the
                                      threshold start (accurate); 1: in the
                                      reference's order (Kernels.cl:916-926)   */
+    int32_t texture_filter;       /* 0: nearest texel (the reference's only
+                                     mode, Kernels.cl:856-860); 1: bilinear   */
 } go_spec;
 
 /* Strand serialisation ---------------------------------------------------- */
@@ -95,6 +97,8 @@ typedef struct go_job_in {
     const float   *random_field; /* random_field_size floats                */
     float          background[4];
     int32_t        width, height, frame, job_index;
+    const float   *gradients;    /* 28 floats (112 B) per gradient record    */
+    int32_t        n_gradients;
 } go_job_in;
 
 /* Optional dumps for parity tests.  Column slot = tile_in_job*tpt + column.

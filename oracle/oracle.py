@@ -20,7 +20,8 @@ class Spec(C.Structure):
                 ("compute_depth", C.c_int32), ("threads_per_tile", C.c_int32),
                 ("tiles_per_call", C.c_int32), ("max_tile_size", C.c_int32),
                 ("max_strands_per_tile", C.c_int32), ("random_field_size", C.c_int32),
-                ("stochastic_factor", C.c_float), ("coverage_mode", C.c_int32)]
+                ("stochastic_factor", C.c_float), ("coverage_mode", C.c_int32),
+                ("texture_filter", C.c_int32)]
 
 
 class Serialized(C.Structure):
@@ -43,7 +44,8 @@ class JobIn(C.Structure):
                 ("pict_data", C.c_void_p), ("pict_bytes", C.c_long), ("pict_uses", C.c_void_p),
                 ("n_pict_uses", C.c_int32), ("random_field", C.c_void_p),
                 ("background", C.c_float * 4), ("width", C.c_int32), ("height", C.c_int32),
-                ("frame", C.c_int32), ("job_index", C.c_int32)]
+                ("frame", C.c_int32), ("job_index", C.c_int32), ("gradients", C.c_void_p),
+                ("n_gradients", C.c_int32)]
 
 
 class Dump(C.Structure):
@@ -198,6 +200,10 @@ def _job_in(geo, shapes, tiles, scene, rnd, job_index):
     for i in range(4):
         ji.background[i] = float(scene.background[i])
     ji.width, ji.height, ji.frame, ji.job_index = scene.width, scene.height, 0, job_index
+    grads = getattr(scene, "gradients", None)
+    if grads is not None and len(grads):
+        ji.gradients = _ptr(np.ascontiguousarray(grads, np.float32))
+        ji.n_gradients = len(grads)
     return ji
 
 

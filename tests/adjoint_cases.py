@@ -28,7 +28,7 @@ def perturbed(scene, seed, point_sigma=4.0, color_sigma=0.1):
 
 def case(size, n_shapes, seed):
     """(perturbed scene, target (H, W, 4) float32)."""
-    base = S.batch_scene(size, n_shapes, seed)
+    base = S.batch_scene(size, n_shapes, seed, gradient_share=0)
     ref = H.oracle_render(base, want_f64=False)
     target = ref["f32"].reshape(base.height, base.width, 4).astype(np.float32)
     return perturbed(base, seed + 1000), target

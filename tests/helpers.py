@@ -19,7 +19,8 @@ def oracle_spec(spec: G.RasterSpec) -> O.Spec:
                           max_strands_per_tile=spec.max_strands_per_tile,
                           random_field_size=spec.random_field_size,
                           stochastic_factor=spec.stochastic_factor,
-                          coverage_mode=spec.coverage_mode)
+                          coverage_mode=spec.coverage_mode,
+                          texture_filter=spec.texture_filter)
 
 
 def oracle_render(scene, spec=None, want_f64=True, nthreads=8):

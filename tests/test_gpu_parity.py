@@ -212,3 +212,37 @@ def test_nonsquare_canvas(rasterizer):
     ref = H.oracle_render(scene)
     u2, f2 = H.gpu_render_level2(rasterizer, scene)
     check_frames(scene, u2, f2, ref)
+
+
+def gradient_scene():
+    """Overlapping translucent gradient, picture and solid shapes."""
+    rng = S.SplitMix64(23)
+    tex = (rng.next(24 * 24) & np.uint64(0xFFFFFFFF)).astype(np.uint32).view(np.uint8)
+    cols = [[1.0, 0.2, 0.1, 0.9], [0.1, 0.8, 0.3, 0.5], [0.2, 0.3, 1.0, 0.7], [0.9, 0.9, 0.1, 1.0]]
+    grads = [S.gradient_record(0, (5.0, 6.0), (70.0, 40.0), [0.0, 0.4, 0.7, 1.0], cols),
+             S.gradient_record(1, (40.0, 45.0), (30.0, 0.0), [0.0, 1.0], cols[:2])]
+    shapes = [(S.gradient_tag(0), [S.straight_pairs([(3.5, 4.2), (75.1, 12.7), (40.3, 70.9)])]),
+              (S.gradient_tag(1), [S.rectangle_pairs(20.25, 22.5, 66.75, 71.0)]),
+              (S.picture_tag(2), [S.straight_pairs([(10.0, 50.0), (50.5, 30.5), (44.0, 78.0)])]),
+              (S.solid_tag(3), [S.rectangle_pairs(30.5, 5.5, 60.5, 35.5)])]
+    subs = [S.gradient_substance(0), S.gradient_substance(1), S.picture_substance(0),
+            [0.3, 0.6, 0.9, 0.6]]
+    return S.build_scene(80, 80, shapes, subs, tex, [(-7.5, -13.25, 24, 24, 0, 0.8)],
+                         background=(0.95, 0.9, 0.8, 1.0), name="gradients", gradients=grads)
+
+
+@pytest.mark.parametrize("filt", [0, 1])
+def test_gradients_and_texture_filters(filt):
+    """Gradient substances and nearest / bilinear pictures: BGRA bit-identical
+    to the oracle, colours within the coverage tolerance."""
+    spec = G.default_spec(texture_filter=filt)
+    r = G.Rasterizer(spec)
+    try:
+        for scene in (gradient_scene(), S.scaled_config2(512, 4000, 17)):
+            say(f"filter {filt}: {scene.name}")
+            ref = H.oracle_render(scene, spec)
+            assert ref["rc"] == 0
+            u2, f2 = H.gpu_render_level2(r, scene, spec)
+            check_frames(scene, u2, f2, ref)
+    finally:
+        r.close()

@@ -43,7 +43,7 @@ def test_fp64_forward_matches_oracle_on_polygons():
 def test_fp64_forward_matches_oracle_on_blobs():
     """Blobs with knob splits: the FP64 forward splits at the exact tangent,
     the serialiser at its FP32 bisection (within 1e-4 in t)."""
-    sc = S.batch_scene(192, 200, 5)
+    sc = S.batch_scene(192, 200, 5, gradient_share=0)
     ref = H.oracle_render(sc, want_f64=True)
     d = np.abs(fp64_image(sc) - ref["f64"].reshape(sc.height, sc.width, 4)).max(axis=2)
     assert np.median(d) < 1e-7

@@ -239,3 +239,62 @@ def test_reference_mode_differs_only_in_section_width():
     da = a["out"].view(np.uint8).astype(int)
     db = b["out"].view(np.uint8).astype(int)
     assert np.abs(da - db).max() <= 1
+
+
+def test_bilinear_texture_sampling():
+    """Bilinear pictures: each pixel blends the four texels around
+    p/scale - translate - 0.5, texels outside the picture transparent."""
+    rng = S.SplitMix64(7)
+    tex = (rng.next(12 * 12) & np.uint64(0xFFFFFFFF)).astype(np.uint32).view(np.uint8)
+    scale, tx, ty = 0.75, -1.5, -2.25
+    scene = S.build_scene(20, 20, [(S.picture_tag(0), [S.rectangle_pairs(0, 0, 20, 20)])],
+                          [S.picture_substance(0)], tex, [(tx, ty, 12, 12, 0, scale)],
+                          background=(0, 0, 0, 1), name="bilinear")
+    spec = G.default_spec(texture_filter=1)
+    res = H.oracle_render(scene, spec)
+    t = tex.reshape(12, 12, 4).astype(np.float64) / 255.0
+
+    def at(i, j):
+        return t[j, i] if 0 <= i < 12 and 0 <= j < 12 else np.zeros(4)
+    for y in range(20):
+        for x in range(20):
+            sx = float(np.float32(np.float32(x) / np.float32(scale)) - np.float32(tx)) - 0.5
+            sy = float(np.float32(np.float32(y) / np.float32(scale)) - np.float32(ty)) - 0.5
+            ix, iy = int(np.floor(sx)), int(np.floor(sy))
+            wx, wy = sx - ix, sy - iy
+            c = (at(ix, iy) * (1 - wx) + at(ix + 1, iy) * wx) * (1 - wy) + \
+                (at(ix, iy + 1) * (1 - wx) + at(ix + 1, iy + 1) * wx) * wy
+            expect = c[:3] * c[3]            # over opaque black
+            assert np.abs(res["f64"][y, x, :3] - expect).max() < 1e-5, (x, y)
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_gradient_substances(kind):
+    """Linear and radial gradients over a covering rectangle: every pixel is
+    the piecewise-linear stop colour at t of its integer position."""
+    offs = np.array([0.0, 0.3, 1.0])
+    cols = np.array([[1.0, 0.0, 0.0, 1.0], [0.0, 1.0, 0.0, 0.5], [0.0, 0.0, 1.0, 1.0]])
+    if kind == 0:
+        g = S.gradient_record(0, (2.0, 3.0), (14.0, 9.0), offs, cols)
+    else:
+        g = S.gradient_record(1, (8.0, 7.0), (9.0, 0.0), offs, cols)
+    scene = S.build_scene(16, 16, [(S.gradient_tag(0), [S.rectangle_pairs(0, 0, 16, 16)])],
+                          [S.gradient_substance(0)], background=(1, 1, 1, 1), name="grad",
+                          gradients=[g])
+    res = H.oracle_render(scene)
+    for y in range(16):
+        for x in range(16):
+            if kind == 0:
+                d = np.array([12.0, 6.0])
+                t = ((x - 2.0) * d[0] + (y - 3.0) * d[1]) / (d @ d)
+            else:
+                t = np.hypot(x - 8.0, y - 7.0) / 9.0
+            t = min(max(t, 0.0), 1.0)
+            i = int(np.searchsorted(offs, t, side="left"))
+            if i == 0:
+                c = cols[0]
+            else:
+                f = (t - offs[i - 1]) / (offs[i] - offs[i - 1])
+                c = cols[i - 1] + (cols[i] - cols[i - 1]) * f
+            expect = c[:3] * c[3] + (1 - c[3]) * 1.0      # over white
+            assert np.abs(res["f64"][y, x, :3] - expect).max() < 1e-5, (x, y, t)
