@@ -304,10 +304,13 @@ class Rasterizer:
         self._check(self.L.gudni_last_timings(self.ctx, _ptr(ms), 2))
         return ms
 
-    def last_stage_cycles(self):
-        """(generate, sort, render, units) shader cycles summed over waves."""
-        c = np.zeros(4, np.uint64)
-        self._check(self.L.gudni_last_stage_cycles(self.ctx, _ptr(c), 4))
+    def last_stage_cycles(self, n: int = 4):
+        """Shader cycles summed over waves.  After a timed render: (generate,
+        sort, render, units).  After a timed adjoint run and its result: the
+        forward's (generate, sort, sweep), the backward's (generate, sort,
+        sweep), the point pass, units."""
+        c = np.zeros(n, np.uint64)
+        self._check(self.L.gudni_last_stage_cycles(self.ctx, _ptr(c), n))
         return c
 
 

@@ -404,19 +404,18 @@ __device__ void generate_o(const Frame &f, const AdjDesc &d, Queue &q, const Col
     }
 }
 
-// Tag of the shape labelled b.
+// Straight (non-premultiplied) colour at pixel (ax, ay) of the shape
+// labelled b, and its tag; a compact tile reads the substance record from
+// the tile's LDS copy.
 template <bool kCompact>
-__device__ __forceinline__ uint64_t label_tag(const SceneDesc &s, const int32_t *sidx, const TileTable *tt,
-                                              int b, int &shape_index)
+__device__ __forceinline__ float4 label_color(const SceneDesc &s, const int32_t *sidx, const TileTable *tt,
+                                              const float4 *straight, int b, int ax, int ay, uint64_t &tag)
 {
-    if (kCompact) { shape_index = -1; return tt->tag[b]; }
-    shape_index = sidx[b * kWave];
-    return s.shapes[shape_index].tag;
-}
-
-// Straight (non-premultiplied) colour of a shape at pixel (ax, ay).
-__device__ __forceinline__ float4 shape_color(const SceneDesc &s, uint64_t tag, int ax, int ay)
-{
+    if (kCompact) {
+        tag = tt->tag[b];
+        return read_color_sub(s, tag, straight[b], ax, ay);
+    }
+    tag = s.shapes[sidx[b * kWave]].tag;
     return read_color(s, tag, ax, ay);
 }
 
@@ -429,7 +428,8 @@ __device__ __forceinline__ float4 shape_color(const SceneDesc &s, uint64_t tag, 
 // the layers below it.
 template <bool kCompact>
 __device__ void color_grad(const SceneDesc &s, const Stack &st, const int32_t *sidx, const TileTable *tt,
-                           float4 *dcol, float *d_rgba, int ax, int ay, float4 S, float4 C, float4 gc)
+                           const float4 *straight, float4 *dcol, float *d_rgba, int ax, int ay, float4 S,
+                           float4 C, float4 gc)
 {
     if (!(S.w > 0.0f)) return;
     const float gPx = gc.x / S.w, gPy = gc.y / S.w, gPz = gc.z / S.w;
@@ -441,9 +441,8 @@ __device__ void color_grad(const SceneDesc &s, const Stack &st, const int32_t *s
         if (w1) { b = 127 - __clzll((long long)w1); w1 &= ~(1ull << (b - 64)); }
         else if (w0) { b = 63 - __clzll((long long)w0); w0 &= ~(1ull << b); }
         else break;
-        int shape_index;
-        const uint64_t tag = label_tag<kCompact>(s, sidx, tt, b, shape_index);
-        const float4 col = shape_color(s, tag, ax, ay);
+        uint64_t tag;
+        const float4 col = label_color<kCompact>(s, sidx, tt, straight, b, ax, ay, tag);
         const float T = 1.0f - F.w;
         const float4 pm = make_float4(col.x * col.w, col.y * col.w, col.z * col.w, col.w);
         float4 Fn;
@@ -465,8 +464,8 @@ __device__ void color_grad(const SceneDesc &s, const Stack &st, const int32_t *s
                     int bb;
                     if (u1) { bb = 127 - __clzll((long long)u1); u1 &= ~(1ull << (bb - 64)); }
                     else { bb = 63 - __clzll((long long)u0); u0 &= ~(1ull << bb); }
-                    int si2;
-                    const float4 c2 = shape_color(s, label_tag<kCompact>(s, sidx, tt, bb, si2), ax, ay);
+                    uint64_t tag2;
+                    const float4 c2 = label_color<kCompact>(s, sidx, tt, straight, bb, ax, ay, tag2);
                     const float t2 = 1.0f - U.w;
                     U.x = U.x + c2.x * c2.w * t2;
                     U.y = U.y + c2.y * c2.w * t2;
@@ -604,7 +603,8 @@ __device__ uint32_t sweep_forward(const Frame &f, const AdjDesc &d, Queue &q, co
 // Backward sweep over the same sections.
 template <bool kCompact>
 __device__ uint32_t sweep_backward(const Frame &f, const AdjDesc &d, Queue &q, const Column &c, Stack st,
-                                   const int32_t *sidx, const TileTable *tt, float4 *dcol, float4 *orig)
+                                   const int32_t *sidx, const TileTable *tt, const float4 *straight,
+                                   float4 *dcol, float4 *orig)
 {
     const SceneDesc &s = d.s;
     Sweep p;
@@ -694,7 +694,7 @@ __device__ uint32_t sweep_backward(const Frame &f, const AdjDesc &d, Queue &q, c
             const float4 gc = make_float4(G.x * area, G.y * area, G.z * area, G.w * area);
             if (gc.x != 0.0f || gc.y != 0.0f || gc.z != 0.0f || gc.w != 0.0f) {
                 const float4 S = make_float4(col.x * col.w, col.y * col.w, col.z * col.w, col.w);
-                color_grad<kCompact>(s, st, sidx, tt, dcol, d.d_rgba, c.dx, ay, S, col, gc);
+                color_grad<kCompact>(s, st, sidx, tt, straight, dcol, d.d_rgba, c.dx, ay, S, col, gc);
             }
             if (p.current < p.num_active) {
                 const uint32_t h = q.get_h(p.current);
@@ -725,7 +725,8 @@ __device__ __forceinline__ void wave_sync()
 
 template <bool kCompact>
 __device__ void adj_unit(const Frame &f, const AdjDesc &d, int lane, uint32_t unit, Queue &q, int32_t *sidx,
-                         TileTable *tt, float4 *dcol, float4 *orig, uint32_t &status, float &loss)
+                         TileTable *tt, float4 *straight, float4 *dcol, float4 *orig, uint32_t &status,
+                         float &loss)
 {
     const SceneDesc &s = d.s;
     const int tile = (int)(unit >> 2), quarter = (int)(unit & 3u);
@@ -738,11 +739,10 @@ __device__ void adj_unit(const Frame &f, const AdjDesc &d, int lane, uint32_t un
         for (int n = lane; n < c.num_shapes; n += kWave) {
             const uint64_t tag = s.shapes[c.shape_start + n].tag;
             tt->tag[n] = tag;
-            float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
-            if ((tag & kTagTypeMask) == kTagSolid) {
-                col = s.substances[tag & kTagIdMask];
+            float4 col = s.substances[tag & kTagIdMask];
+            straight[n] = col;
+            if ((tag & kTagTypeMask) == kTagSolid)
                 col = make_float4(col.x * col.w, col.y * col.w, col.z * col.w, col.w);
-            }
             tt->color[n] = col;
             dcol[n] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
@@ -753,23 +753,39 @@ __device__ void adj_unit(const Frame &f, const AdjDesc &d, int lane, uint32_t un
     const int x0 = t.box[0] + (hd >= 6 ? ((quarter * kWave) & (w - 1)) : 0);
     const int x1 = x0 + (hd >= 6 ? kWave : w);
     const float span0 = (float)x0 - 1.0f, span1 = (float)x1 + 1.0f;
+    // optional phase clocks (wave-uniform points): [0] A generate [1] A sort
+    // [2] A sweep [3] B generate [4] B sort [5] B sweep [6] C [7] units
+    unsigned long long *clk = f.stage_cycles;
+    long long tclk = clk ? clock64() : 0;
+    auto stamp = [&](int i) {
+        if (!clk) return;
+        const long long now = clock64();
+        if (lane == 0) atomicAdd(clk + i, (unsigned long long)(now - tclk));
+        tclk = now;
+    };
     // A: forward
     // every generate_o call is made by the whole wave (it ballots over shapes)
     uint32_t bits = 0;
     uint64_t s0 = 0, s1 = 0;
     generate_o<kCompact, kRecord>(f, d, q, c, active, lane, span0, span1, sidx, bits, s0, s1, orig);
+    stamp(0);
     bool ok = active && !q.spilled && !q.overflow;
     if (ok) sort_queue(q);
+    stamp(1);
     if (ok) status = sweep_forward<kCompact>(f, d, q, c, Stack{s0, s1, 0u}, sidx, tt, loss);
+    stamp(2);
     if (active && q.overflow) status |= kStatusOverflow;
     if (__any(q.spilled) || __any(status != 0)) return;
     // B: recompute the records, backward sweep
     q.head = 0; q.len = 0;
     bits = 0; s0 = 0; s1 = 0;
     generate_o<kCompact, kRecord>(f, d, q, c, active, lane, span0, span1, sidx, bits, s0, s1, orig);
+    stamp(3);
     ok = active && !q.spilled && !q.overflow;
     if (ok) sort_queue(q);
-    if (ok) status = sweep_backward<kCompact>(f, d, q, c, Stack{s0, s1, 0u}, sidx, tt, dcol, orig);
+    stamp(4);
+    if (ok) status = sweep_backward<kCompact>(f, d, q, c, Stack{s0, s1, 0u}, sidx, tt, straight, dcol, orig);
+    stamp(5);
     // C: line weights -> outline points
     {
         uint32_t bits2 = 0;
@@ -777,6 +793,8 @@ __device__ void adj_unit(const Frame &f, const AdjDesc &d, int lane, uint32_t un
         generate_o<kCompact, kApply>(f, d, q, c, ok && status == 0, lane, span0, span1, sidx, bits2, a0,
                                      a1, orig);
     }
+    stamp(6);
+    if (clk && lane == 0) atomicAdd(clk + 7, 1ull);
     // D: the tile's colour gradients -> substances
     if (kCompact) {
         wave_sync();
@@ -794,10 +812,11 @@ __device__ void adj_unit(const Frame &f, const AdjDesc &d, int lane, uint32_t un
     }
 }
 
-__global__ void __launch_bounds__(256) adjoint_pass(AdjFrame a, int from_retry)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) adjoint_pass(AdjFrame a, int from_retry)
 {
     __shared__ TileTable tables[4];
     __shared__ float4 dcols[4][128];
+    __shared__ float4 straights[4][128];   // compact tiles: the shapes' substance records
     const Frame &f = a.f;
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
@@ -809,6 +828,7 @@ __global__ void __launch_bounds__(256) adjoint_pass(AdjFrame a, int from_retry)
     int32_t *sidx = f.shape_idx + (size_t)slot * f.max_shapes * kWave + lane;
     TileTable *tt = &tables[wave];
     float4 *dcol = dcols[wave];
+    float4 *straight = straights[wave];
     const int n_work = from_retry ? (int)min((uint32_t)f.retry_capacity, f.retry[0]) : f.n_work;
     Claimer claim;
     claim.init(f.counter, f.n_parts, n_work, (int)blockIdx.x);
@@ -834,8 +854,8 @@ __global__ void __launch_bounds__(256) adjoint_pass(AdjFrame a, int from_retry)
         q.spilled = false; q.overflow = false;
         uint32_t status = 0;
         float loss = 0.0f;
-        if (count <= f.max_shapes) adj_unit<true>(f, d, lane, unit, q, sidx, tt, dcol, orig, status, loss);
-        else adj_unit<false>(f, d, lane, unit, q, sidx, tt, dcol, orig, status, loss);
+        if (count <= f.max_shapes) adj_unit<true>(f, d, lane, unit, q, sidx, tt, straight, dcol, orig, status, loss);
+        else adj_unit<false>(f, d, lane, unit, q, sidx, tt, straight, dcol, orig, status, loss);
         if (status) atomicOr(f.status, status);
         const bool spilled = __any(q.spilled);
         if (spilled) {
@@ -1108,6 +1128,13 @@ int launch(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t work_begin, int32_t n
     adj::AdjFrame af;
     af.f = base_frame(ctx);
     af.f.coverage_mode = GUDNI_COVERAGE_ACCURATE;
+    af.f.stage_cycles = nullptr;
+    if (ctx->timing) {
+        CHECK_HIP(ctx, ctx->cycles.ensure(8 * sizeof(unsigned long long)));
+        hipLaunchKernelGGL(adj::adj_reset, dim3(1), dim3(64), 0, st, (uint32_t *)ctx->cycles.p, 16);
+        af.f.stage_cycles = (unsigned long long *)ctx->cycles.p;
+        ctx->timed_adjoint = true;
+    }
     af.f.stochastic = 0.0f;
     af.f.scenes = nullptr;
     af.f.tiles = (const TileRec *)a->tiles.p;
@@ -1217,6 +1244,11 @@ int gudni_adjoint_result(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t scene, 
     if (!ctx || !a || scene < 0 || scene >= (int32_t)a->scenes.size()) return GUDNI_ERR_ARG;
     int rc = read_status(ctx);
     if (rc) return rc;
+    if (ctx->timed_adjoint) {
+        CHECK_HIP(ctx, hipMemcpy(ctx->last_cycles, ctx->cycles.p, sizeof(ctx->last_cycles),
+                                 hipMemcpyDeviceToHost));
+        ctx->timed_adjoint = false;
+    }
     const gudni_adjoint_batch::Scene &sc = a->scenes[scene];
     if (loss)
         CHECK_HIP(ctx, hipMemcpy(loss, (double *)a->loss.p + scene, 8, hipMemcpyDeviceToHost));

@@ -46,7 +46,7 @@ struct gudni_ctx {
     std::string error;
     int cus = 0;
     int fast_blocks = 0, slow_blocks = 0;
-    int blocks_per_cu = 0;        // fast-pass blocks per CU (0: as many as fit, at most 4)
+    int blocks_per_cu = 0;        // fast-pass blocks per CU (0: as many as fit, at most 8)
     // common buffers
     gudni::DevBuf geo, subs, pict, uses, random, grads;
     int32_t n_substances = 0, n_uses = 0, n_grads = 0;
@@ -75,10 +75,10 @@ struct gudni_ctx {
     hipEvent_t bin_ev[2] = {nullptr, nullptr};
     float last_bin_ms = 0;
     // timing
-    bool timing = false, timed_render = false;
+    bool timing = false, timed_render = false, timed_adjoint = false;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     float last_ms[2] = {0, 0};
-    unsigned long long last_cycles[4] = {0, 0, 0, 0};
+    unsigned long long last_cycles[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 #define CHECK_HIP(ctx, expr)                                                        \

@@ -676,11 +676,11 @@ __device__ float4 eval_gradient(const float4 *g, int ax, int ay)
 }
 
 // readColor (Kernels.cl:1411-1442): solid colour, picture texel (nearest, as
-// the reference samples, or bilinear) or gradient
-__device__ float4 read_color(const SceneDesc &s, uint64_t tag, int ax, int ay)
+// the reference samples, or bilinear) or gradient, from the shape's
+// substance record `sub`
+__device__ float4 read_color_sub(const SceneDesc &s, uint64_t tag, float4 sub, int ax, int ay)
 {
     const uint64_t type = tag & kTagTypeMask;
-    const float4 sub = s.substances[tag & kTagIdMask];
     if (type == kTagSolid) return sub;
     const uint32_t id = __float_as_uint(sub.x);
     if (type == kTagGradient) return eval_gradient(s.grads + 7 * (size_t)id, ax, ay);
@@ -713,8 +713,14 @@ __device__ float4 read_color(const SceneDesc &s, uint64_t tag, int ax, int ay)
     return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-// Per-wave LDS table of a compact tile's shapes: tag and (solid) colour by
-// tile-local index.
+__device__ __forceinline__ float4 read_color(const SceneDesc &s, uint64_t tag, int ax, int ay)
+{
+    return read_color_sub(s, tag, s.substances[tag & kTagIdMask], ax, ay);
+}
+
+// Per-wave LDS table of a compact tile's shapes by tile-local index: tag,
+// and the solid colour (in the coverage mode's form) or, for a picture or
+// gradient, the substance record itself.
 struct TileTable {
     uint64_t tag[128];
     float4 color[128];
@@ -725,7 +731,9 @@ struct TileTable {
     int grouped;           // the tile qualifies for grouped colours
 };
 
-// Composite of one group's set shapes, top-down, premultiplied.
+// Composite of one group's set shapes, top-down, premultiplied.  Once the
+// alpha is exactly 1 every later layer adds its colour times 0, so the walk
+// stops there with the same result.
 __device__ __forceinline__ float4 group_composite(const SceneDesc &s, const TileTable *tt,
                                                   uint32_t word, int g, int ax, int ay)
 {
@@ -739,7 +747,7 @@ __device__ __forceinline__ float4 group_composite(const SceneDesc &s, const Tile
         if ((tag & kTagTypeMask) == kTagSolid) {
             pm = tt->color[n];
         } else {
-            const float4 c = read_color(s, tag, ax, ay);
+            const float4 c = read_color_sub(s, tag, tt->color[n], ax, ay);
             pm = make_float4(c.x * c.w, c.y * c.w, c.z * c.w, c.w);
         }
         const float t = 1.0f - acc.w;
@@ -747,20 +755,25 @@ __device__ __forceinline__ float4 group_composite(const SceneDesc &s, const Tile
         acc.y = acc.y + pm.y * t;
         acc.z = acc.z + pm.z * t;
         acc.w = acc.w + pm.w * t;
+        if (acc.w == 1.0f) break;
     }
     return acc;
 }
 
 // Accurate-mode colour of a grouped tile: over(group 3 .. group 0,
-// background), recomputing only the dirty groups' cached composites.
+// background), recomputing only the dirty groups' cached composites.  Groups
+// under an alpha of exactly 1 add nothing (their colour times 0): they are
+// neither read nor recomputed, and stay dirty.
 __device__ float4 grouped_color(const SceneDesc &s, TileTable *tt, Stack &st, int lane, int ax,
                                 int ay)
 {
     const uint32_t words[4] = {(uint32_t)st.s0, (uint32_t)(st.s0 >> 32), (uint32_t)st.s1,
                                (uint32_t)(st.s1 >> 32)};
     float P0 = 0.f, P1 = 0.f, P2 = 0.f, A = 0.f;
+    uint32_t fresh = 0;
 #pragma unroll
     for (int g = 3; g >= 0; --g) {
+        if (A == 1.0f) break;
         float4 gcol;
         if (st.dirty & (1u << g)) {
             gcol = group_composite(s, tt, words[g], g, ax, ay);
@@ -768,6 +781,7 @@ __device__ float4 grouped_color(const SceneDesc &s, TileTable *tt, Stack &st, in
             tt->gc[g][1][lane] = gcol.y;
             tt->gc[g][2][lane] = gcol.z;
             tt->gc[g][3][lane] = gcol.w;
+            fresh |= 1u << g;
         } else {
             gcol = make_float4(tt->gc[g][0][lane], tt->gc[g][1][lane], tt->gc[g][2][lane],
                                tt->gc[g][3][lane]);
@@ -778,7 +792,7 @@ __device__ float4 grouped_color(const SceneDesc &s, TileTable *tt, Stack &st, in
         P2 = P2 + gcol.z * t;
         A = A + gcol.w * t;
     }
-    st.dirty = 0;
+    st.dirty &= ~fresh;
     {
         const float4 pm = make_float4(s.bg[0] * s.bg[3], s.bg[1] * s.bg[3], s.bg[2] * s.bg[3], s.bg[3]);
         const float t = 1.0f - A;
@@ -836,7 +850,8 @@ __device__ float4 determine_color(const Frame &f, const SceneDesc &s, const Stac
                 if (kCompact && solid) {
                     next = tt->color[top_bit];          // already in the mode's form
                 } else {
-                    next = read_color(s, tag, ax, ay);
+                    next = kCompact ? read_color_sub(s, tag, tt->color[top_bit], ax, ay)
+                                    : read_color(s, tag, ax, ay);
                     positional = positional || !solid;
                     if (!ref_mode)
                         next = make_float4(next.x * next.w, next.y * next.w, next.z * next.w, next.w);
@@ -1057,11 +1072,9 @@ __device__ void run_unit(const Frame &f, const SceneDesc &s, int lane, uint32_t 
         for (int n = lane; n < c.num_shapes; n += kWave) {
             const uint64_t tag = s.shapes[c.shape_start + n].tag;
             tt->tag[n] = tag;
-            float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
-            if ((tag & kTagTypeMask) == kTagSolid) {
-                col = s.substances[tag & kTagIdMask];
-                if (premultiply) col = make_float4(col.x * col.w, col.y * col.w, col.z * col.w, col.w);
-            }
+            float4 col = s.substances[tag & kTagIdMask];
+            if ((tag & kTagTypeMask) == kTagSolid && premultiply)
+                col = make_float4(col.x * col.w, col.y * col.w, col.z * col.w, col.w);
             tt->color[n] = col;
             all_add = all_add && (tag & kTagCompoundMask) == kTagAdd;
         }

@@ -296,7 +296,7 @@ int  gudni_adjoint(gudni_ctx *ctx, const gudni_adjoint_scene *scene, double *los
  * launch stream when enabled and read by gudni_frame_check: [0] counter
  * reset + fast pass, [1] retry pass. */
 /* Launch shape of the persistent fast pass: blocks (of four waves) per
- * compute unit, 1..4, or 0 for as many as fit (at most 4).  Fewer resident
+ * compute unit, 1..8, or 0 for as many as fit (at most 8).  Fewer resident
  * waves keep more of each wave's threshold rings in L2; more hide more
  * latency.  No counterpart in the reference (its launch shape is fixed by
  * the OpenCL runtime). */
@@ -305,7 +305,10 @@ int gudni_enable_timing(gudni_ctx *ctx, int enable);
 int gudni_last_timings(gudni_ctx *ctx, float *ms, int32_t n);
 /* Shader-clock cycles of the last timed render's fast pass, summed over
  * waves: [0] threshold generation, [1] sort, [2] coverage/compositing sweep,
- * [3] work units processed. */
+ * [3] work units processed.  After a timed gudni_adjoint_run and its
+ * gudni_adjoint_result (up to 8 entries): the forward's generation, sort and
+ * sweep, the backward's generation, sort and sweep, the pass that moves line
+ * weights onto outline points, and work units. */
 int gudni_last_stage_cycles(gudni_ctx *ctx, uint64_t *cycles, int32_t n);
 
 /* ---- Batches: many independent scenes in one persistent launch ----------- */
