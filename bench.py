@@ -129,22 +129,58 @@ def render_f32(r, scene, spec):
     return f32.cpu().numpy().reshape(scene.height, scene.width, 4)
 
 
-def adjoint_leg(spec, steps, log_fn):
-    """The differentiable workload: a 1024^2, 2,000-shape target and a
-    perturbed copy as parameters; one step = forward + backward (loss,
-    dL/d points, dL/d RGBA) in one launch.  Device time from events around the
-    launches; the full step adds re-serialising and re-binning the updated
-    parameters on the host, the upload and reading the gradients back."""
+def _adjoint_spec(spec, tile, radix):
+    import gudni_amd as G
+    return G.default_spec(texture_filter=spec.texture_filter, max_tile_size=tile, sort_radix_bits=radix)
+
+
+def _time_adjoint(spec, items_fn, runs=2):
+    """ms per adjoint step of the items items_fn(rasterizer, spec) builds."""
     import torch
     import gudni_amd as G
-    from gudni_amd import scenes as S
-    base = S.config4()
-    params = S.perturb(base, S.SEEDS[4] + 1)
     r = G.Rasterizer(spec)
     stream = torch.cuda.current_stream()
     r.set_stream(stream.cuda_stream)
-    target = render_f32(r, base, spec)
-    adj = G.Adjoint(r, [(params, target)])
+    adj = G.Adjoint(r, items_fn(r, spec))
+    try:
+        adj.run()
+        adj.result(0)
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for _ in range(runs):
+            adj.run()
+        stop.record(stream)
+        adj.result(0)
+        return start.elapsed_time(stop) / runs
+    finally:
+        adj.close()
+        r.close()
+
+
+def adjoint_leg(spec, steps, log_fn):
+    """The differentiable workload: a 1024^2, 2,000-shape target and a
+    perturbed copy as parameters; one step = forward + backward (loss,
+    dL/d points, dL/d RGBA) in one launch.  Tile size and sort radix come
+    from the autotuner (timed adjoint steps).  Device time from events around
+    the launches; the full step adds re-serialising and re-binning the updated
+    parameters on the host, the upload and reading the gradients back."""
+    import torch
+    import gudni_amd as G
+    from gudni_amd import autotune, scenes as S
+    base = S.config4()
+    params = S.perturb(base, S.SEEDS[4] + 1)
+
+    def items(r, sp):
+        return [(params, render_f32(r, base, sp))]
+
+    tile, radix, timings = autotune.search(lambda ts, rb: _time_adjoint(_adjoint_spec(spec, ts, rb), items))
+    log_fn(f"adjoint autotune {json.dumps(timings)} -> tile {tile}, radix {radix}")
+    sp = _adjoint_spec(spec, tile, radix)
+    r = G.Rasterizer(sp)
+    stream = torch.cuda.current_stream()
+    r.set_stream(stream.cuda_stream)
+    adj = G.Adjoint(r, items(r, sp))
     adj.run()
     loss0, dp, dc, _ = adj.result(0)
     start = torch.cuda.Event(enable_timing=True)
@@ -156,6 +192,7 @@ def adjoint_leg(spec, steps, log_fn):
     adj.result(0)
     dev_ms = start.elapsed_time(stop) / steps
     pairs = params.pairs.copy()
+    loss = loss0
     t = time.perf_counter()
     for _ in range(steps):
         pairs = pairs - 0.05 * dp.reshape(pairs.shape)
@@ -169,6 +206,7 @@ def adjoint_leg(spec, steps, log_fn):
            f"loss {loss0:.2f} -> {loss:.2f}")
     return {"workload": "1024x1024, 2000 shapes (17k quadratics), solid; params perturbed "
                         "(points + N(0,4px), colours + N(0,0.1))",
+            "tile_size": tile, "sort_radix_bits": radix, "autotune_ms": timings,
             "ms_per_step_device": dev_ms, "steps_per_s_device": 1e3 / dev_ms,
             "ms_per_step_with_host_update": full_ms, "steps_per_s_with_host_update": 1e3 / full_ms,
             "loss_first": loss0, "loss_after_steps": loss, "n_points": int(params.pairs.size // 2),
@@ -178,15 +216,25 @@ def adjoint_leg(spec, steps, log_fn):
 def batched_adjoint_leg(spec, steps, log_fn):
     """64 scenes of 512^2 (500-5,000 shapes, solid + a 1024^2 texture): forward
     and backward of all scenes in one persistent launch, against 64
-    per-scene adjoint launches."""
+    per-scene adjoint launches.  Tile size and radix are tuned on 8 of the
+    scenes."""
     import torch
     import gudni_amd as G
-    from gudni_amd import scenes as S
+    from gudni_amd import autotune, scenes as S
     bases = S.config5()
-    r = G.Rasterizer(spec)
+    params = [S.perturb(b, S.SEEDS[5] + i) for i, b in enumerate(bases)]
+
+    def items_of(k):
+        return lambda r, sp: [(params[i], render_f32(r, bases[i], sp)) for i in range(k)]
+
+    tile, radix, timings = autotune.search(
+        lambda ts, rb: _time_adjoint(_adjoint_spec(spec, ts, rb), items_of(8), runs=1))
+    log_fn(f"batched adjoint autotune {json.dumps(timings)} -> tile {tile}, radix {radix}")
+    sp = _adjoint_spec(spec, tile, radix)
+    r = G.Rasterizer(sp)
     stream = torch.cuda.current_stream()
     r.set_stream(stream.cuda_stream)
-    items = [(S.perturb(b, S.SEEDS[5] + i), render_f32(r, b, spec)) for i, b in enumerate(bases)]
+    items = items_of(len(bases))(r, sp)
     adj = G.Adjoint(r, items)
     adj.run()
     adj.result(0)
@@ -210,6 +258,7 @@ def batched_adjoint_leg(spec, steps, log_fn):
     log_fn(f"batched adjoint: {one_ms:.2f} ms per step in one launch, {seq_ms:.2f} ms as "
            f"{len(items)} launches")
     return {"workload": f"{len(items)} scenes of 512x512, 500-5000 shapes, solid + 1024^2 texture",
+            "tile_size": tile, "sort_radix_bits": radix, "autotune_ms": timings,
             "ms_per_step": one_ms, "steps_per_s": 1e3 / one_ms,
             "sequential_ms_per_step": seq_ms, "sequential_steps_per_s": 1e3 / seq_ms,
             "speedup_vs_sequential": seq_ms / one_ms,
@@ -218,7 +267,7 @@ def batched_adjoint_leg(spec, steps, log_fn):
 
 def batched_leg(spec, steps, log_fn):
     """The batched workload: 256 scenes of 256^2..2048^2 with 100..50k shapes,
-    tile size per scene-size group chosen by the autotuner, all scenes in one
+    tile size and sort radix per scene-size group chosen by the autotuner, all scenes in one
     persistent launch replayed from a captured hipGraph; compared with the same
     scenes rendered by one launch each."""
     import torch
@@ -233,7 +282,7 @@ def batched_leg(spec, steps, log_fn):
     t = time.perf_counter()
     table = autotune.tune(r, scenes, per_bucket=2, reps=2, log=log_fn)
     t_tune = time.perf_counter() - t
-    items, chosen, t_prep = autotune.prepare_batch(scenes, table)
+    items, chosen, radices, t_prep = autotune.prepare_batch(scenes, table)
     t = time.perf_counter()
     b = G.Batch(r, items)
     t_up = time.perf_counter() - t
@@ -270,7 +319,7 @@ def batched_leg(spec, steps, log_fn):
         "sequential_latency_p50_ms": float(np.percentile(seq_lat, 50)),
         "sequential_latency_p99_ms": float(np.percentile(seq_lat, 99)),
         "speedup_vs_sequential": float(seq.sum() / ms),
-        "tile_size_by_group": table, "graph": True,
+        "tuned_by_group": table, "graph": True,
         "host_s": {"generate": t_gen, "autotune": t_tune, "serialize_and_bin": t_prep, "upload": t_up},
     }
 
@@ -298,7 +347,7 @@ def main():
 
     import torch
     import gudni_amd as G
-    from gudni_amd import scenes as S
+    from gudni_amd import autotune, scenes as S
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
@@ -329,6 +378,28 @@ def main():
     t_upload = time.perf_counter() - t
     npix = scene.width * scene.height
     out = torch.zeros(npix, dtype=torch.int32, device="cuda")
+
+    # sort radix for this frame (tile size stays the pinned 256)
+    def frame_ms(radix):
+        j = G.RasterJobs(G.default_spec(texture_filter=args.texture_filter, sort_radix_bits=radix), geo,
+                         scene.width, scene.height)
+        r.frame_upload(j, scene)
+        r.frame_render(out.data_ptr())
+        r.frame_check()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(3):
+            r.frame_render(out.data_ptr())
+        e1.record(stream)
+        r.frame_check()
+        return e0.elapsed_time(e1) / 3
+
+    radix, radix_ms = autotune.tune_frame(frame_ms)
+    log(f"frame sort radix: {json.dumps(radix_ms)} -> {radix}")
+    spec = G.default_spec(texture_filter=args.texture_filter, sort_radix_bits=radix)
+    jobs = G.RasterJobs(spec, geo, scene.width, scene.height)
+    r.frame_upload(jobs, scene)
 
     for _ in range(args.warmup):
         r.frame_render(out.data_ptr())
@@ -406,7 +477,8 @@ def main():
                                + " sampling)",
                    "canvas": [scene.width, scene.height], "shapes": scene.n_shapes,
                    "curves": scene.n_curves, "tiles": jobs.n_tiles, "tile_shape_refs": jobs.n_refs,
-                   "coverage_mode": "accurate"},
+                   "coverage_mode": "accurate", "sort_radix_bits": radix,
+                   "sort_radix_autotune_ms": radix_ms},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_detail": traffic_info,

@@ -13,7 +13,8 @@ class Spec(C.Structure):
                 ("max_tiles_per_call", C.c_int32), ("max_thresholds", C.c_int32),
                 ("max_strands_per_tile", C.c_int32), ("max_shapes", C.c_int32),
                 ("random_field_size", C.c_int32), ("stochastic_factor", C.c_float),
-                ("coverage_mode", C.c_int32), ("texture_filter", C.c_int32)]
+                ("coverage_mode", C.c_int32), ("texture_filter", C.c_int32),
+                ("sort_radix_bits", C.c_int32)]
 
 
 class Outlines(C.Structure):

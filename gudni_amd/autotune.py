@@ -1,13 +1,19 @@
-"""Tile-size autotuner for batched rendering.
+"""Autotuner for tile size and sort radix.
 
 The reference's tile size is whatever the device reports as its maximum
 work-group size (determineRasterSpec, OpenCL/Setup.hs:71-87); any power of two
 up to 256 gives a well-defined tile tree, and every column still runs the
 same threshold / sort / coverage semantics.  Smaller tiles split crowded
-regions further (shorter columns, fewer shapes per tile) at the cost of more
-shape references.  This tuner groups scenes by canvas side and shape count,
-renders a few scenes of each group as a batch with each candidate tile size,
-and keeps the fastest per group.
+regions further (shorter columns, fewer shapes per tile, more waves) at the
+cost of more shape references.  The sort radix is the digit width of the
+radix pass ahead of each column's insertion sort: it pays where columns hold
+many records, and every width yields the same permutation.
+
+`tune` groups the scenes of a batch by canvas side and shape count, renders a
+few scenes of each group as a batch with each candidate tile size (at the
+default radix), then with each candidate radix at the chosen tile size, and
+keeps the fastest per group.  `tune_frame` and `tune_adjoint` do the same for
+one frame or one adjoint workload with a caller-supplied timing function.
 """
 from __future__ import annotations
 
@@ -19,6 +25,7 @@ import numpy as np
 from . import Batch, default_spec, prepare
 
 CANDIDATES = (64, 128, 256)
+RADIX_CANDIDATES = (0, 3, 5)
 
 
 def bucket(scene) -> str:
@@ -48,37 +55,62 @@ def _time_batch(rasterizer, items, reps: int) -> float:
         b.close()
 
 
+def search(time_fn, tiles=CANDIDATES, radices=RADIX_CANDIDATES, default_radix: int | None = None):
+    """Coordinate search: the fastest tile size at the default radix, then the
+    fastest radix at that tile size.  time_fn(tile, radix) -> ms.  Returns
+    (tile, radix, {"tile,radix": ms})."""
+    r0 = default_spec().sort_radix_bits if default_radix is None else default_radix
+    timings = {}
+    for ts in tiles:
+        timings[(ts, r0)] = time_fn(ts, r0)
+    best_tile = min(tiles, key=lambda ts: timings[(ts, r0)])
+    for rb in radices:
+        if (best_tile, rb) not in timings:
+            timings[(best_tile, rb)] = time_fn(best_tile, rb)
+    best = min(((best_tile, rb) for rb in set(radices) | {r0}), key=lambda k: timings[k])
+    return best[0], best[1], {f"{t},{r}": round(ms, 3) for (t, r), ms in timings.items()}
+
+
 def tune(rasterizer, scenes, per_bucket: int = 3, reps: int = 3, candidates=CANDIDATES,
-         log=None) -> dict:
-    """Returns {bucket: tile_size} measured on the rasterizer's device."""
+         radices=RADIX_CANDIDATES, log=None) -> dict:
+    """Returns {bucket: {"tile": size, "radix": bits}} measured on the
+    rasterizer's device."""
     groups: dict[str, list] = {}
     for sc in scenes:
         groups.setdefault(bucket(sc), []).append(sc)
     table = {}
     for key, members in sorted(groups.items()):
         sample = members[:per_bucket]
-        best, best_ms = None, None
-        timings = {}
-        for ts in candidates:
-            spec = default_spec(max_tile_size=ts)
+
+        def time_fn(ts, rb):
+            spec = default_spec(max_tile_size=ts, sort_radix_bits=rb)
             items = [(sc,) + prepare(sc, spec)[1:] for sc in sample]
-            ms = _time_batch(rasterizer, items, reps)
-            timings[ts] = ms
-            if best_ms is None or ms < best_ms:
-                best, best_ms = ts, ms
-        table[key] = best
+            return _time_batch(rasterizer, items, reps)
+
+        ts, rb, timings = search(time_fn, candidates, radices)
+        table[key] = {"tile": ts, "radix": rb}
         if log:
-            log(f"autotune {key}: {json.dumps(timings)} -> {best}")
+            log(f"autotune {key}: {json.dumps(timings)} -> tile {ts}, radix {rb}")
     return table
 
 
+def tune_frame(time_fn, radices=(0, 3, 4, 5)):
+    """Radix for one frame at a fixed tile size.  time_fn(radix) -> ms."""
+    timings = {rb: time_fn(rb) for rb in radices}
+    best = min(timings, key=timings.get)
+    return best, {str(k): round(v, 3) for k, v in timings.items()}
+
+
 def prepare_batch(scenes, table: dict | None = None, default_tile: int = 256):
-    """(scene, Geometry, RasterJobs) per scene with the tuned tile size."""
-    items, chosen = [], []
+    """(scene, Geometry, RasterJobs) per scene with the tuned tile size and
+    radix; returns (items, tile sizes, radices, host seconds)."""
+    items, tiles, radices = [], [], []
     t0 = time.perf_counter()
+    r0 = default_spec().sort_radix_bits
     for sc in scenes:
-        ts = (table or {}).get(bucket(sc), default_tile)
-        spec = default_spec(max_tile_size=ts)
+        choice = (table or {}).get(bucket(sc), {"tile": default_tile, "radix": r0})
+        spec = default_spec(max_tile_size=choice["tile"], sort_radix_bits=choice["radix"])
         items.append((sc,) + prepare(sc, spec)[1:])
-        chosen.append(ts)
-    return items, np.array(chosen), time.perf_counter() - t0
+        tiles.append(choice["tile"])
+        radices.append(choice["radix"])
+    return items, np.array(tiles), np.array(radices), time.perf_counter() - t0

@@ -756,6 +756,7 @@ __device__ void adj_unit(const Frame &f, const AdjDesc &d, int lane, uint32_t un
     // optional phase clocks (wave-uniform points): [0] A generate [1] A sort
     // [2] A sweep [3] B generate [4] B sort [5] B sweep [6] C [7] units
     unsigned long long *clk = f.stage_cycles;
+    uint16_t *hist = reinterpret_cast<uint16_t *>(&tt->gc[0][0][0]) + lane;   // sort histogram
     long long tclk = clk ? clock64() : 0;
     auto stamp = [&](int i) {
         if (!clk) return;
@@ -770,7 +771,7 @@ __device__ void adj_unit(const Frame &f, const AdjDesc &d, int lane, uint32_t un
     generate_o<kCompact, kRecord>(f, d, q, c, active, lane, span0, span1, sidx, bits, s0, s1, orig);
     stamp(0);
     bool ok = active && !q.spilled && !q.overflow;
-    if (ok) sort_queue(q);
+    if (ok) radix_sort_queue(q, hist, s.sort_bits, c.float_height);
     stamp(1);
     if (ok) status = sweep_forward<kCompact>(f, d, q, c, Stack{s0, s1, 0u}, sidx, tt, loss);
     stamp(2);
@@ -782,7 +783,7 @@ __device__ void adj_unit(const Frame &f, const AdjDesc &d, int lane, uint32_t un
     generate_o<kCompact, kRecord>(f, d, q, c, active, lane, span0, span1, sidx, bits, s0, s1, orig);
     stamp(3);
     ok = active && !q.spilled && !q.overflow;
-    if (ok) sort_queue(q);
+    if (ok) radix_sort_queue(q, hist, s.sort_bits, c.float_height);
     stamp(4);
     if (ok) status = sweep_backward<kCompact>(f, d, q, c, Stack{s0, s1, 0u}, sidx, tt, straight, dcol, orig);
     stamp(5);
@@ -1052,6 +1053,7 @@ int build(gudni_ctx *ctx, gudni_adjoint_batch *a)
         d.s.out_f32 = (float4 *)a->image.p + a->pix_off[s];
         d.s.grads = (const float4 *)((const gudni_gradient *)a->grads.p + grads_off[s]);
         d.s.filter = ctx->spec.texture_filter;
+        d.s.sort_bits = ctx->spec.sort_radix_bits;
         for (int i = 0; i < 4; ++i) d.s.bg[i] = sc.bg[i];
         d.s.width = sc.width;
         d.s.height = sc.height;

@@ -64,6 +64,7 @@ struct gudni_ctx {
     int32_t retry_capacity = 0;
     // uploaded frame (Level 2)
     int32_t frame_w = 0, frame_h = 0, frame_n_work = 0, frame_n_tiles = 0, frame_index = 0;
+    int32_t frame_sort_bits = 0;  // sort radix of the spec the uploaded jobs were built with
     float background[4] = {1, 1, 1, 1};
     bool frame_ready = false;
     // Level 2 scene: shapes in traversal order, binned on the device

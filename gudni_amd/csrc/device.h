@@ -136,7 +136,7 @@ struct alignas(16) SceneDesc {
     float bg[4];                  // background RGBA
     int32_t width, height;
     int32_t filter;               // picture sampling: 0 nearest, 1 bilinear
-    int32_t pad0;
+    int32_t sort_bits;            // radix digit width of the record sort (0: insertion only)
 };
 
 struct Frame {
@@ -479,6 +479,60 @@ __device__ void sort_queue(Queue &q)
     }
 }
 
+// Stage (b) with a radix pass first.  The records' top digit is a
+// monotone key prefix: floor(top * 2^bits / h2), h2 the column segment's
+// height rounded up to a power of two (an exact scaling), so bucket order
+// never contradicts thresholdIsBelow and equal tops share a bucket.  A
+// counting pass (per-lane histogram in LDS), an exclusive scan and a stable
+// scatter into the ring's free half order the records by digit; the
+// insertion sort then only meets inversions inside a bucket.  Both steps
+// are stable, so the permutation is the stable sort's, i.e. the reference's
+// bubble sort's.  The 64 lanes of a wave sort their 64 columns in lockstep;
+// `hist` is the lane's column of a [32][64] u16 table (the tile table's
+// colour cache, which is rebuilt after the sort).  Columns with fewer than
+// kRadixMin records, or more than half a ring, take the insertion sort
+// alone.
+constexpr int kRadixMin = 8;
+constexpr int kRadixMaxBits = 5;
+
+__device__ __forceinline__ int top_digit(float top, float scale, int nb)
+{
+    const int d = (int)(top * scale);
+    return d < 0 ? 0 : (d >= nb ? nb - 1 : d);
+}
+
+__device__ void radix_sort_queue(Queue &q, uint16_t *hist, int bits, float height)
+{
+    const int n = q.len;
+    if (bits > 0 && n >= kRadixMin && 2 * n <= q.mask + 1) {
+        const int nb = 1 << bits;
+        float h2 = 1.0f;
+        while (h2 < height) h2 *= 2.0f;
+        const float scale = (float)nb / h2;
+        for (int b = 0; b < nb; ++b) hist[b * kWave] = 0;
+        for (int i = 0; i < n; ++i) {
+            const int d = top_digit(q.thr[q.row(i)].x, scale, nb);
+            hist[d * kWave] = (uint16_t)(hist[d * kWave] + 1);
+        }
+        uint32_t sum = 0;
+        for (int b = 0; b < nb; ++b) {
+            const uint32_t c = hist[b * kWave];
+            hist[b * kWave] = (uint16_t)sum;
+            sum += c;
+        }
+        for (int i = 0; i < n; ++i) {
+            const Thr t = q.get(i);
+            const uint32_t h = q.get_h(i);
+            const int d = top_digit(t.top, scale, nb);
+            const int pos = hist[d * kWave];
+            hist[d * kWave] = (uint16_t)(pos + 1);
+            q.set(n + pos, h, t);
+        }
+        q.head = (q.head + n) & q.mask;
+    }
+    sort_queue(q);
+}
+
 // insertThreshold (Kernels.cl:1105-1124)
 __device__ void insert_threshold(Queue &q, uint32_t h, const Thr &t)
 {
@@ -730,6 +784,8 @@ struct TileTable {
     uint64_t pic0, pic1;   // tile-local shapes whose substance is a picture
     int grouped;           // the tile qualifies for grouped colours
 };
+static_assert(sizeof(TileTable::gc) >= sizeof(uint16_t) * kWave * (1 << kRadixMaxBits),
+              "the sort histogram lives in the colour cache");
 
 // Composite of one group's set shapes, top-down, premultiplied.  Once the
 // alpha is exactly 1 every later layer adds its colour times 0, so the walk
@@ -1120,7 +1176,8 @@ __device__ void run_unit(const Frame &f, const SceneDesc &s, int lane, uint32_t 
     const bool ok = active && !q.spilled && !q.overflow;
     const long long t1 = timed ? clock64() : 0;
     if (ok && f.dump_qlen) dump_queue<kCompact>(f, q, dslot, true, c0, c1);
-    if (ok) sort_queue(q);
+    if (ok) radix_sort_queue(q, reinterpret_cast<uint16_t *>(&tt->gc[0][0][0]) + lane, s.sort_bits,
+                             c.float_height);
     if (ok && f.dump_qlen) {
         dump_queue<kCompact>(f, q, dslot, false, c0, c1);
         f.dump_qlen[dslot] = q.len;

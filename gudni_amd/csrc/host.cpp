@@ -347,6 +347,7 @@ struct Binner {
 int build_jobs(const gudni_spec &spec, const Geometry &g, int32_t cw, int32_t ch, Jobs &out)
 {
     Binner bn(g, spec);
+    out.sort_radix_bits = spec.sort_radix_bits;
     const int canvas_depth = ceil_log2(std::max(cw, ch));
     const int side = 1 << canvas_depth;
     const int root[4] = {0, 0, side, side};
@@ -387,7 +388,6 @@ int build_jobs(const gudni_spec &spec, const Geometry &g, int32_t cw, int32_t ch
 using namespace gudni;
 
 struct gudni_geometry { Geometry g; };
-struct gudni_jobs { Jobs j; };
 
 extern "C" {
 
@@ -403,6 +403,7 @@ void gudni_default_spec(gudni_spec *s)
     s->stochastic_factor = 0.0f;
     s->coverage_mode = GUDNI_COVERAGE_ACCURATE;
     s->texture_filter = GUDNI_FILTER_NEAREST;
+    s->sort_radix_bits = 4;
 }
 
 int gudni_geometry_build(const gudni_outlines *scene, int32_t cw, int32_t ch, gudni_geometry **out)
@@ -446,7 +447,8 @@ int gudni_jobs_build(const gudni_spec *spec, const gudni_geometry *g, int32_t cw
         return GUDNI_ERR_ARG;
     }
     if (spec->max_tile_size <= 0 || (spec->max_tile_size & (spec->max_tile_size - 1)) ||
-        spec->threads_per_tile <= 0 || spec->max_tiles_per_call <= 0) {
+        spec->threads_per_tile <= 0 || spec->max_tiles_per_call <= 0 ||
+        spec->sort_radix_bits < 0 || spec->sort_radix_bits > 5) {
         g_host_error = "gudni_jobs_build: bad spec";
         return GUDNI_ERR_ARG;
     }

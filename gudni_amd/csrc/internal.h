@@ -65,6 +65,7 @@ struct Geometry {
 };
 
 struct Jobs {
+    int32_t sort_radix_bits = 0;   // of the spec the jobs were built with
     std::vector<TileRec> tiles;
     std::vector<ShapeRec> refs;
     std::vector<int32_t> job_tile_offset;
@@ -88,3 +89,6 @@ int build_geometry(const gudni_outlines *sc, int32_t cw, int32_t ch, Geometry &g
 int build_jobs(const gudni_spec &spec, const Geometry &g, int32_t cw, int32_t ch, Jobs &out);
 
 } // namespace gudni
+
+// The opaque handle of gudni_jobs_build: the jobs with the spec's sort radix.
+struct gudni_jobs { gudni::Jobs j; };

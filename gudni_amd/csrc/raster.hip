@@ -221,6 +221,7 @@ static dev::SceneDesc single_desc(gudni_ctx *ctx, const float bg[4], int32_t w, 
     sd.width = w;
     sd.height = h;
     sd.filter = ctx->spec.texture_filter;
+    sd.sort_bits = ctx->spec.sort_radix_bits;
     return sd;
 }
 
@@ -397,9 +398,11 @@ int gudni_create(const gudni_spec *spec, int device, gudni_ctx **out)
         s.max_thresholds < 1 || s.random_field_size < 1 ||
         (s.coverage_mode != GUDNI_COVERAGE_ACCURATE && s.coverage_mode != GUDNI_COVERAGE_REFERENCE) ||
         (s.texture_filter != GUDNI_FILTER_NEAREST && s.texture_filter != GUDNI_FILTER_BILINEAR) ||
+        s.sort_radix_bits < 0 || s.sort_radix_bits > dev::kRadixMaxBits ||
         (s.random_field_size & (s.random_field_size - 1))) {
         g_host_error = "gudni_create: unsupported spec (threads_per_tile must be 256, "
-                       "max_shapes 1..127, random_field_size a power of two)";
+                       "max_shapes 1..127, random_field_size a power of two, "
+                       "sort_radix_bits 0..5)";
         delete ctx;
         return GUDNI_ERR_ARG;
     }
@@ -645,6 +648,7 @@ int gudni_frame_upload(gudni_ctx *ctx, const gudni_jobs *jobs, int32_t width, in
     ctx->frame_n_work = (int32_t)work.size();
     ctx->frame_n_tiles = v.n_tiles;
     ctx->frame_index = frame;
+    ctx->frame_sort_bits = jobs->j.sort_radix_bits;
     std::memcpy(ctx->background, background, sizeof(ctx->background));
     ctx->frame_ready = true;
     return GUDNI_OK;
@@ -659,8 +663,9 @@ int gudni_frame_render(gudni_ctx *ctx, uint32_t *out_dev, float *out_f32_dev)
     f.tile_job = (const int32_t *)ctx->tile_job.p;
     f.work = (const uint32_t *)ctx->work.p;
     f.n_work = ctx->frame_n_work;
-    int rc0 = set_single_scene(ctx, single_desc(ctx, ctx->background, ctx->frame_w, ctx->frame_h,
-                                                out_dev, out_f32_dev), f);
+    dev::SceneDesc sd = single_desc(ctx, ctx->background, ctx->frame_w, ctx->frame_h, out_dev, out_f32_dev);
+    sd.sort_bits = ctx->frame_sort_bits;
+    int rc0 = set_single_scene(ctx, sd, f);
     if (rc0) return rc0;
     if (ctx->timing) {
         for (hipEvent_t &e : ctx->ev)
@@ -1024,6 +1029,7 @@ int gudni_batch_upload(gudni_ctx *ctx, int32_t n_scenes, const gudni_batch_scene
         d.out_f32 = nullptr;
         d.grads = (const float4 *)((const gudni_gradient *)b->grads.p + grads_off[s]);
         d.filter = ctx->spec.texture_filter;
+        d.sort_bits = scenes[s].jobs->j.sort_radix_bits;
         for (int i = 0; i < 4; ++i) d.bg[i] = scenes[s].background[i];
         d.width = scenes[s].width;
         d.height = scenes[s].height;

@@ -52,6 +52,13 @@ typedef struct gudni_spec {
     int32_t texture_filter;       /* GUDNI_FILTER_NEAREST (default: the
                                      reference's sampling, Kernels.cl:856-860)
                                      or GUDNI_FILTER_BILINEAR                 */
+    int32_t sort_radix_bits;      /* 0..5: digit width of the radix pass that
+                                     precedes the per-column insertion sort
+                                     (sortThresholds, Kernels.cl:2084-2115);
+                                     0 = insertion sort only.  The permutation
+                                     is the same for every value.  An
+                                     extension: the reference has no radix
+                                     pass.                                    */
 } gudni_spec;
 
 /* Picture sampling at the pixel's integer position p (Kernels.cl:1426):
@@ -197,7 +204,8 @@ void gudni_jobs_free(gudni_jobs *j);
 /* ---- Level 2: whole frame with every buffer resident in HBM --------------- */
 /* Replaces queueRasterJobs (CallKernels.hs:218-242) for a whole frame: the
  * jobs' tiles and shape references are uploaded once, then every frame render
- * runs all tiles of all jobs from device memory in one persistent launch. */
+ * runs all tiles of all jobs from device memory in one persistent launch,
+ * sorting with the sort_radix_bits of the spec the jobs were built with. */
 int gudni_frame_upload(gudni_ctx *ctx, const gudni_jobs *jobs,
                        int32_t width, int32_t height,
                        const float background[4], int32_t frame);
@@ -332,6 +340,9 @@ typedef struct gudni_batch_scene {
 typedef struct gudni_batch gudni_batch;
 int  gudni_batch_upload(gudni_ctx *ctx, int32_t n_scenes, const gudni_batch_scene *scenes,
                         const float *random_field, int32_t n_random, gudni_batch **out);
+/* Each scene's records are sorted with the sort_radix_bits of the spec its
+ * jobs were built with (so a tuner may choose tile size and radix per
+ * scene). */
 /* Renders every scene (stream-ordered).  use_graph: on the first call the
  * launch sequence is captured into a hipGraph on a stream the batch owns,
  * then that graph is replayed on the context's stream (the null stream

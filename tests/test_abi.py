@@ -33,7 +33,7 @@ def test_python_signatures_cover_the_header():
 
 
 def test_struct_sizes_match_the_header():
-    assert C.sizeof(_lib.Spec) == 40
+    assert C.sizeof(_lib.Spec) == 44
     assert C.sizeof(_lib.BatchScene) == 112
     assert C.sizeof(_lib.SceneDesc) == 144
     assert C.sizeof(_lib.AdjointScene) == 136
@@ -46,6 +46,7 @@ def test_default_spec_is_the_pinned_spec():
     assert (s.max_tile_size, s.threads_per_tile, s.max_tiles_per_call, s.max_thresholds,
             s.max_strands_per_tile, s.max_shapes, s.random_field_size) == (256, 256, 256, 4096, 4094, 127, 4096)
     assert s.stochastic_factor == 0.0 and s.coverage_mode == 0 and s.texture_filter == 0
+    assert s.sort_radix_bits == 4
 
 
 def test_host_entry_points_reject_bad_arguments():
