@@ -334,6 +334,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable workloads")
+    ap.add_argument("--sort-radix", type=int, default=-1,
+                    help="sort radix bits of the frame (default: chosen by the autotuner)")
     ap.add_argument("--texture-filter", type=int, default=1,
                     help="picture sampling of the benchmark scenes: 1 bilinear, 0 nearest")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -395,8 +397,11 @@ def main():
         r.frame_check()
         return e0.elapsed_time(e1) / 3
 
-    radix, radix_ms = autotune.tune_frame(frame_ms)
-    log(f"frame sort radix: {json.dumps(radix_ms)} -> {radix}")
+    if args.sort_radix >= 0:
+        radix, radix_ms = args.sort_radix, None
+    else:
+        radix, radix_ms = autotune.tune_frame(frame_ms)
+        log(f"frame sort radix: {json.dumps(radix_ms)} -> {radix}")
     spec = G.default_spec(texture_filter=args.texture_filter, sort_radix_bits=radix)
     jobs = G.RasterJobs(spec, geo, scene.width, scene.height)
     r.frame_upload(jobs, scene)
