@@ -31,7 +31,8 @@ def log(msg):
 def algorithmic_bytes(scene, geo, jobs):
     """Bytes the path must move once per frame: every tile reads its shape
     records and their strand bytes, the tile records, the substance table,
-    and writes the BGRA frame (texel reads are not counted)."""
+    the gradient records, the picture bytes once (every texel a picture use
+    can reach, read once), and writes the BGRA frame."""
     shapes = geo.shapes.view(np.uint32).reshape(-1, 4)
     starts = shapes[:, 2].astype(np.int64)
     order = np.argsort(starts)
@@ -42,7 +43,10 @@ def algorithmic_bytes(scene, geo, jobs):
     refs = jobs.shape_refs.view(np.uint32).reshape(-1, 4)
     g = np.array([shape_bytes[int(s)] for s in refs[:, 2]], np.int64) if len(refs) else np.zeros(0, np.int64)
     per_ref = int((16 + g).sum())
-    return per_ref + 32 * jobs.n_tiles + 16 * scene.substances.shape[0] + 4 * scene.width * scene.height
+    n_grads = np.asarray(scene.gradients, np.float32).reshape(-1, 28).shape[0]
+    pict = int(np.asarray(scene.pict_data).size) if len(scene.pict_uses) else 0
+    return (per_ref + 32 * jobs.n_tiles + 16 * scene.substances.shape[0] + 112 * n_grads + pict
+            + 4 * scene.width * scene.height)
 
 
 def cpu_baseline(scene, seconds_target=15.0, texture_filter=1):
