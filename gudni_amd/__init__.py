@@ -261,6 +261,45 @@ class Rasterizer:
         d, arrs = self._scene_desc(geometry, scene, rnd)
         self._check(self.L.gudni_scene_upload(self.ctx, C.byref(d)))
 
+    def scene_upload_outlines(self, scene, rnd: np.ndarray | None = None):
+        """Level 2 from the onShape sequence: outlines go up and the device
+        serialises them (strand heap, shape records, boxes), ready for
+        scene_bin."""
+        rnd = random_field(self.spec.random_field_size) if rnd is None else rnd
+        arrs = [np.ascontiguousarray(scene.tags, np.uint64),
+                np.ascontiguousarray(scene.shape_outline_offsets, np.int32),
+                np.ascontiguousarray(scene.outline_pair_offsets, np.int32),
+                np.ascontiguousarray(scene.pairs, np.float32),
+                np.ascontiguousarray(scene.substances, np.float32),
+                np.ascontiguousarray(scene.pict_data, np.uint8),
+                np.ascontiguousarray(scene.pict_uses, np.uint8),
+                np.ascontiguousarray(rnd, np.float32), _gradients(scene)]
+        tags, soff, ooff, pairs, subs, pict, uses, rnd, grads = arrs
+        o = _lib.Outlines(scene.n_shapes, tags.ctypes.data, soff.ctypes.data, ooff.ctypes.data,
+                          pairs.ctypes.data if pairs.size else None)
+        d = _lib.SceneDesc(None, 0, None, 0, None, _ptr(subs) if subs.size else None, subs.shape[0],
+                           _ptr(pict) if pict.size else None, pict.size,
+                           _ptr(uses) if uses.size else None, uses.shape[0], _ptr(rnd), rnd.size,
+                           scene.width, scene.height,
+                           (C.c_float * 4)(*[float(x) for x in scene.background]),
+                           _ptr(grads) if len(grads) else None, len(grads))
+        self._check(self.L.gudni_scene_upload_outlines(self.ctx, C.byref(o), C.byref(d)))
+
+    def scene_geometry(self):
+        """The uploaded scene's strand heap, 16-byte shape records, boxes and
+        strand extents as built on the device."""
+        sz = np.zeros(2, np.int64)
+        self._check(self.L.gudni_scene_geometry_sizes(self.ctx, _ptr(sz)))
+        n, nb = int(sz[0]), int(sz[1])
+        out = {"geo": np.zeros(nb, np.uint8), "shapes": np.zeros(n * 16, np.uint8),
+               "bbox": np.zeros((n, 4), np.float32), "bounds": np.zeros((n, 4), np.float32)}
+        self._check(self.L.gudni_scene_geometry_read(self.ctx, _ptr(out["geo"]), _ptr(out["shapes"]),
+                                                     _ptr(out["bbox"]), _ptr(out["bounds"])))
+        return out
+
+    def last_serialize_ms(self) -> float:
+        return float(self.L.gudni_last_serialize_ms(self.ctx))
+
     def scene_bin(self):
         """Stage (a) on the device; the frame is then ready for frame_render."""
         self._check(self.L.gudni_scene_bin(self.ctx))

@@ -15,7 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libgudni_hip.so")
 SOURCES = [os.path.join(HERE, "csrc", "host.cpp"), os.path.join(HERE, "csrc", "raster.hip"),
-           os.path.join(HERE, "csrc", "bin.hip"), os.path.join(HERE, "csrc", "adjoint.hip")]
+           os.path.join(HERE, "csrc", "bin.hip"), os.path.join(HERE, "csrc", "adjoint.hip"),
+           os.path.join(HERE, "csrc", "serialize.hip")]
 ARCH = os.environ.get("GUDNI_OFFLOAD_ARCH", "gfx950")
 
 

@@ -25,6 +25,7 @@
 // over counts it reads from device memory, so the whole build is enqueued
 // without a host round trip; capacities are checked on the device.
 #include "bin.h"
+#include "scan.h"
 
 namespace gudni {
 namespace binning {
@@ -47,36 +48,6 @@ __device__ __forceinline__ uint64_t route(const float4 bb, const int4 box, bool 
     const bool fa = vertical ? bb.y < fc : bb.x < fc;
     const bool fb = vertical ? bb.w > fc : bb.z > fc;
     return (uint64_t)(fa ? 1u : 0u) | ((uint64_t)(fb ? 1u : 0u) << 32);
-}
-
-// ---------------------------------------------------------------------------
-// block scans (blockDim.x a multiple of 64, at most 512)
-
-__device__ __forceinline__ uint64_t wave_incl_scan(uint64_t v, int lane)
-{
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint64_t t = __shfl_up(v, (unsigned)o, 64);
-        if (lane >= o) v += t;
-    }
-    return v;
-}
-
-__device__ uint64_t block_excl_scan(uint64_t v, uint64_t &total)
-{
-    __shared__ uint64_t wave_sum[8];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int waves = blockDim.x >> 6;
-    const uint64_t incl = wave_incl_scan(v, lane);
-    if (lane == 63) wave_sum[wave] = incl;
-    __syncthreads();
-    uint64_t pre = 0, tot = 0;
-    for (int k = 0; k < waves; ++k) {
-        if (k < wave) pre += wave_sum[k];
-        tot += wave_sum[k];
-    }
-    __syncthreads();
-    total = tot;
-    return pre + incl - v;
 }
 
 // Scan inputs: a count read from device memory and a value per index.
@@ -139,57 +110,6 @@ struct WorkFlags {
         return k;
     }
 };
-
-template <class F>
-__global__ void __launch_bounds__(kThreads) scan_reduce(F f, uint64_t *partials)
-{
-    f.load();
-    const int n = f.count();
-    const int chunk = (n + kBlocks - 1) / kBlocks;
-    const int b0 = min(n, (int)blockIdx.x * chunk), b1 = min(n, b0 + chunk);
-    uint64_t s = 0;
-    for (int i = b0 + (int)threadIdx.x; i < b1; i += kThreads) s += f(i);
-    uint64_t total;
-    (void)block_excl_scan(s, total);
-    if (threadIdx.x == 0) partials[blockIdx.x] = total;
-}
-
-__global__ void __launch_bounds__(kBlocks) scan_top(uint64_t *partials)
-{
-    const uint64_t v = partials[threadIdx.x];
-    uint64_t total;
-    const uint64_t ex = block_excl_scan(v, total);
-    partials[threadIdx.x] = ex;
-    if (threadIdx.x == 0) partials[kBlocks] = total;
-}
-
-template <class F>
-__global__ void __launch_bounds__(kThreads) scan_down(F f, const uint64_t *partials, uint64_t *out)
-{
-    f.load();
-    const int n = f.count();
-    const int chunk = (n + kBlocks - 1) / kBlocks;
-    const int b0 = min(n, (int)blockIdx.x * chunk), b1 = min(n, b0 + chunk);
-    uint64_t carry = partials[blockIdx.x];
-    for (int base = b0; base < b1; base += kThreads) {
-        const int i = base + (int)threadIdx.x;
-        const uint64_t v = i < b1 ? f(i) : 0;
-        uint64_t total;
-        const uint64_t ex = block_excl_scan(v, total);
-        if (i < b1) out[i] = carry + ex;
-        carry += total;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = partials[kBlocks];
-}
-
-template <class F>
-void scan(const F &f, uint64_t *partials, uint64_t *out, hipStream_t stream)
-{
-    hipLaunchKernelGGL(scan_reduce<F>, dim3(kBlocks), dim3(kThreads), 0, stream, f, partials);
-    hipLaunchKernelGGL(scan_top, dim3(1), dim3(kBlocks), 0, stream, partials);
-    hipLaunchKernelGGL(scan_down<F>, dim3(kBlocks), dim3(kThreads), 0, stream, f,
-                       (const uint64_t *)partials, out);
-}
 
 // ---------------------------------------------------------------------------
 // tree levels
@@ -531,9 +451,9 @@ hipError_t enqueue(const Params &p, const Workspace &w, const ShapeRec *shapes, 
     for (int l = 0; l < p.max_levels; ++l) {
         hipLaunchKernelGGL(bin_decide, g, b, 0, stream, d, shapes, l);
         SplitFlags sf{w.levels, w.node_meta, l, 0, 0};
-        scan(sf, w.partials, d.node_scan, stream);
+        devscan::excl_scan(sf, w.partials, d.node_scan, stream);
         PairFlags pf{w.levels, w.node_meta, w.node_box, w.pair_shape, w.pair_node, bbox, l, 0, 0};
-        scan(pf, w.partials, w.scan, stream);
+        devscan::excl_scan(pf, w.partials, w.scan, stream);
         hipLaunchKernelGGL(bin_advance, dim3(1), dim3(1), 0, stream, d, l);
         hipLaunchKernelGGL(bin_children, g, b, 0, stream, d, l);
         hipLaunchKernelGGL(bin_scatter, g, b, 0, stream, d, bbox, l);
@@ -548,7 +468,7 @@ hipError_t enqueue(const Params &p, const Workspace &w, const ShapeRec *shapes, 
                        out.ref_bounds);
     WorkFlags wf{out.tiles, w.status, ceil_log2(p.threads_per_tile), p.threads_per_tile / 64,
                  p.width, p.height, 0};
-    scan(wf, w.partials, w.scan, stream);
+    devscan::excl_scan(wf, w.partials, w.scan, stream);
     hipLaunchKernelGGL(bin_emit_work, g, b, 0, stream, d, wf, out.work);
     return hipGetLastError();
 }

@@ -75,6 +75,12 @@ struct gudni_ctx {
     bool scene_ready = false;
     hipEvent_t bin_ev[2] = {nullptr, nullptr};
     float last_bin_ms = 0;
+    // device serialisation (gudni_scene_upload_outlines)
+    gudni::DevBuf ser_tags, ser_soff, ser_ooff, ser_pairs, ser_work;
+    uint64_t *ser_partials = nullptr;   // inside ser_work
+    size_t scene_geo_bytes = 0;
+    hipEvent_t ser_ev[2] = {nullptr, nullptr};
+    float last_ser_ms = 0;
     // timing
     bool timing = false, timed_render = false, timed_adjoint = false;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};

@@ -45,6 +45,8 @@ struct TreeOrders {
 };
 static const TreeOrders g_orders;
 
+const int *strand_tree_order(int m) { return g_orders.order[m]; }
+
 struct Pt { float x, y; };
 struct Quad { Pt p0, c, p1; };
 

@@ -72,6 +72,10 @@ struct Jobs {
     std::vector<int32_t> job_shape_offset;
 };
 
+// Slot -> interior pair index of a strand of m curves (m - 1 tree slots):
+// the Eytzinger order of ReorderTable.hs:44-111.
+const int *strand_tree_order(int m);
+
 inline int ceil_log2(int x)
 {   // adjustedLog (TileTree.hs:74-75): ceiling (log2 x), 0 for x < 1
     int k = 0;

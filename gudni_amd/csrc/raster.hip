@@ -431,11 +431,14 @@ void gudni_destroy(gudni_ctx *ctx)
                       &ctx->out_f32, &ctx->fast_thr, &ctx->fast_hdr, &ctx->fast_sidx,
                       &ctx->slow_thr, &ctx->slow_hdr, &ctx->slow_sidx, &ctx->control, &ctx->dump,
                       &ctx->cycles, &ctx->scene_table, &ctx->scene_shapes, &ctx->scene_bbox,
-                      &ctx->scene_bounds, &ctx->bin_ws};
+                      &ctx->scene_bounds, &ctx->bin_ws, &ctx->ser_tags, &ctx->ser_soff,
+                      &ctx->ser_ooff, &ctx->ser_pairs, &ctx->ser_work};
     for (DevBuf *b : bufs) b->release();
     for (hipEvent_t &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t &e : ctx->bin_ev)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t &e : ctx->ser_ev)
         if (e) (void)hipEventDestroy(e);
     delete ctx;
 }
@@ -725,6 +728,7 @@ int gudni_scene_upload(gudni_ctx *ctx, const gudni_scene *sc)
     CHECK_HIP(ctx, upload(ctx, ctx->scene_bounds, bounds.data(), 16u * (size_t)sc->n_shapes));
     CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->scene_n_shapes = sc->n_shapes;
+    ctx->scene_geo_bytes = sc->geo_bytes;
     ctx->scene_w = sc->width;
     ctx->scene_h = sc->height;
     std::memcpy(ctx->scene_bg, sc->background, sizeof(ctx->scene_bg));

@@ -254,6 +254,27 @@ int   gudni_scene_read(gudni_ctx *ctx, void *tiles, int32_t *tile_job, void *ref
                        float *ref_bounds, uint32_t *work);
 /* Milliseconds of the last gudni_scene_bin on the device (timing enabled). */
 float gudni_last_bin_ms(const gudni_ctx *ctx);
+
+/* Level 2 from outlines: the serialiser runs on the device too.  Replaces
+ * the onShape serialisation of the geometry pile (Raster/Serialize.hs:76-177:
+ * splitShape -> strands, Raster/Strand.hs:77-206, Deknob.hs:61-108,
+ * ReorderTable.hs:44-111; enclose/excludeBox; makeShapeEntry) in addition to
+ * what gudni_scene_upload takes over.  `outlines` is the onShape sequence;
+ * `scene` supplies the substances, pictures, gradients, random field, canvas
+ * and background, and its geo / shapes / bbox fields must be NULL / 0.  The
+ * strand heap, Shape records and boxes built on the device are byte-for-byte
+ * those gudni_geometry_build builds on the host.  Then gudni_scene_bin and
+ * gudni_frame_render as after gudni_scene_upload.  (The heap stays on the
+ * device; host-built jobs cannot be validated against it.) */
+int   gudni_scene_upload_outlines(gudni_ctx *ctx, const gudni_outlines *outlines,
+                                  const gudni_scene *scene);
+/* sizes[0] kept shapes, sizes[1] strand heap bytes of the uploaded scene. */
+int   gudni_scene_geometry_sizes(gudni_ctx *ctx, int64_t sizes[2]);
+/* Reads back the scene's strand heap, Shape records (16 B), boxes and strand
+ * extents (4 floats each); any pointer may be NULL. */
+int   gudni_scene_geometry_read(gudni_ctx *ctx, void *geo, void *shapes, float *bbox, float *bounds);
+/* Milliseconds of the last device serialisation (timing enabled). */
+float gudni_last_serialize_ms(const gudni_ctx *ctx);
 /* Upload + bin + render + read back, synchronously: the scene's W*H BGRA
  * words into host_out. */
 int   gudni_raster_scene(gudni_ctx *ctx, const gudni_scene *scene, uint32_t *host_out);
