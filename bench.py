@@ -119,6 +119,48 @@ def device_binning_leg(r, scene, geo, stream, steps):
             "tiles": nt, "refs": nr, "work_units": nw}
 
 
+def device_pipeline_leg(r, scene, spec, stream, steps):
+    """The whole host side of a frame on the device too: outlines in host
+    memory -> upload -> device serialiser -> device binning -> render, against
+    the host serialiser + host tile tree + upload + render.  Wall clock from
+    outlines in host memory to the frame in HBM."""
+    import torch
+    import gudni_amd as G
+    out = torch.zeros(scene.width * scene.height, dtype=torch.int32, device="cuda")
+    r.enable_timing(True)
+    ser_ms, bin_ms = [], []
+    for _ in range(3):
+        r.scene_upload_outlines(scene)
+        ser_ms.append(r.last_serialize_ms())
+        r.scene_bin()
+        bin_ms.append(r.last_bin_ms())
+    r.enable_timing(False)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        r.scene_upload_outlines(scene)
+        r.scene_bin()
+        r.frame_render(out.data_ptr())
+    r.frame_check()
+    torch.cuda.synchronize()
+    dev = (time.perf_counter() - t) / steps * 1e3
+    t = time.perf_counter()
+    for _ in range(steps):
+        geo = G.Geometry(scene)
+        jobs = G.RasterJobs(spec, geo, scene.width, scene.height)
+        r.upload_common(geo.geo, scene)
+        r.frame_upload(jobs, scene)
+        r.frame_render(out.data_ptr())
+    r.frame_check()
+    torch.cuda.synchronize()
+    host = (time.perf_counter() - t) / steps * 1e3
+    return {"serialize_ms_device": float(np.median(ser_ms)), "bin_ms_device": float(np.median(bin_ms)),
+            "outlines_to_frame_ms": dev, "outlines_to_frame_ms_host_stages": host,
+            "speedup": host / dev,
+            "outline_bytes": int(scene.pairs.nbytes + scene.tags.nbytes + scene.shape_outline_offsets.nbytes
+                                 + scene.outline_pair_offsets.nbytes)}
+
+
 def render_f32(r, scene, spec):
     """The scene's pre-quantisation RGBA image from the raster path (H, W, 4)."""
     import torch
@@ -408,6 +450,10 @@ def main():
         log(f"frame sort radix: {json.dumps(radix_ms)} -> {radix}")
     spec = G.default_spec(texture_filter=args.texture_filter, sort_radix_bits=radix)
     jobs = G.RasterJobs(spec, geo, scene.width, scene.height)
+    r.close()
+    r = G.Rasterizer(spec)
+    r.set_stream(stream.cuda_stream)
+    r.upload_common(geo.geo, scene)
     r.frame_upload(jobs, scene)
 
     for _ in range(args.warmup):
@@ -457,6 +503,9 @@ def main():
     log("device binning (stage a) of the same frame")
     binning = device_binning_leg(r, scene, geo, stream, max(3, min(args.steps, 10)))
     traffic, traffic_info = measured_traffic(fast_ms)
+    log("outlines to frame on the device (serialise + bin + render)")
+    pipeline = device_pipeline_leg(r, scene, spec, stream, max(2, min(args.steps, 5)))
+    log(f"device pipeline: {json.dumps({k: round(v, 3) for k, v in pipeline.items()})}")
 
     adjoint = batched_adjoint = None
     if not args.no_adjoint:
@@ -495,6 +544,7 @@ def main():
                      "algorithmic_bytes": b_alg, "retry_pass_ms": float(np.mean(retry)),
                      "stage_cycle_share": stage_share},
         "device_binning": binning,
+        "device_pipeline": pipeline,
         "batched": batched,
         "adjoint": adjoint,
         "batched_adjoint": batched_adjoint,

@@ -799,6 +799,7 @@ int gudni_scene_bin(gudni_ctx *ctx)
     ctx->scene_n_refs = (int32_t)status[2];
     ctx->frame_n_work = (int32_t)status[3];
     ctx->frame_index = 0;
+    ctx->frame_sort_bits = ctx->spec.sort_radix_bits;
     std::memcpy(ctx->background, ctx->scene_bg, sizeof(ctx->background));
     ctx->frame_ready = true;
     return GUDNI_OK;
