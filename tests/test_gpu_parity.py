@@ -78,6 +78,28 @@ def test_small_scenes_level2(rasterizer, name):
     check_frames(scene, u32, f32, ref)
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_stochastic_field(mode):
+    """The reference's stochastic anti-aliasing (initRandomField / getRandom,
+    Kernels.cl:1597-1613, 1738-1741), inert at the default factor 0: with a
+    non-zero factor every section's area is scaled by 1 + r * factor, r walking
+    the random field from a per-column start.  Frames equal the oracle's in
+    both coverage modes (and differ from the factor-0 frames)."""
+    spec = G.default_spec(stochastic_factor=0.35, coverage_mode=mode)
+    r = G.Rasterizer(spec)
+    try:
+        for scene in (S.config1(200, 128, 3), textured_scene(), gradient_scene()):
+            say(f"stochastic mode {mode}: {scene.name}")
+            ref = H.oracle_render(scene, spec)
+            assert ref["rc"] == 0
+            plain = H.oracle_render(scene, G.default_spec(coverage_mode=mode), want_f64=False)
+            assert (plain["out"] != ref["out"]).any()
+            u32, f32 = H.gpu_render_level2(r, scene, spec)
+            check_frames(scene, u32, f32, ref)
+    finally:
+        r.close()
+
+
 def test_tiny_square_values(rasterizer):
     """Closed form: pixel (0,0) is 0.81 red over white, (1,1) pure red."""
     scene = H.tiny_square()
