@@ -80,13 +80,13 @@ def measured_traffic(kernel_ms):
     """HBM bytes per fast-pass launch from the committed rocprofv3 counter
     summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the
     counter reads for a plain copy of known size)."""
-    path = os.path.join(ROOT, "profiles", "raster_pass_cfg2_4096_traffic.json")
+    path = os.path.join(ROOT, "profiles", "raster_pass_blobs_4096_traffic.json")
     if not os.path.exists(path):
         return None, None
     with open(path) as fh:
         t = json.load(fh)
     return float(t["traffic_bytes"]), {
-        "source": "profiles/raster_pass_cfg2_4096_counters.md",
+        "source": "profiles/raster_pass_blobs_4096_counters.md",
         "fetch_bytes": t["fetch_kib"] * 1024 * t["fetch_factor"],
         "write_bytes": t["write_kib"] * 1024 * t["write_factor"],
         "profiled_kernel_ms": t["fast_pass_avg_ms"], "this_run_kernel_ms": kernel_ms,
@@ -214,8 +214,8 @@ def adjoint_leg(spec, steps, log_fn):
     import torch
     import gudni_amd as G
     from gudni_amd import autotune, scenes as S
-    base = S.config4()
-    params = S.perturb(base, S.SEEDS[4] + 1)
+    base = S.adjoint_1024()
+    params = S.perturb(base, S.SEEDS["adjoint_1024"] + 1)
 
     def items(r, sp):
         return [(params, render_f32(r, base, sp))]
@@ -267,8 +267,8 @@ def batched_adjoint_leg(spec, steps, log_fn):
     import torch
     import gudni_amd as G
     from gudni_amd import autotune, scenes as S
-    bases = S.config5()
-    params = [S.perturb(b, S.SEEDS[5] + i) for i, b in enumerate(bases)]
+    bases = S.adjoint_batch_512()
+    params = [S.perturb(b, S.SEEDS["adjoint_batch_512"] + i) for i, b in enumerate(bases)]
 
     def items_of(k):
         return lambda r, sp: [(params[i], render_f32(r, bases[i], sp)) for i in range(k)]
@@ -320,7 +320,7 @@ def batched_leg(spec, steps, log_fn):
     import gudni_amd as G
     from gudni_amd import autotune, scenes as S
     t = time.perf_counter()
-    scenes = S.config3()
+    scenes = S.batch_256()
     t_gen = time.perf_counter() - t
     r = G.Rasterizer(spec)
     stream = torch.cuda.current_stream()
@@ -409,7 +409,7 @@ def main():
         print(json.dumps({"copy_bytes": n * 4, "copies": max(1, args.steps)}), flush=True)
         return
     t = time.perf_counter()
-    scene = S.config2()
+    scene = S.blobs_4096()
     t_gen = time.perf_counter() - t
     t = time.perf_counter()
     spec, geo, jobs = G.prepare(scene, G.default_spec(texture_filter=args.texture_filter))

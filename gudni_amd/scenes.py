@@ -27,7 +27,9 @@ TAG_ADD = 0x2000000000000000
 TAG_SUBTRACT = 0x3000000000000000
 TAG_ID_MASK = 0x0FFFFFFFFFFFFFFF
 
-SEEDS = {1: 0x67756431, 2: 0x67756432, 3: 0x67756433, 4: 0x67756434, 5: 0x67756435}
+# SplitMix64 seeds of the benchmark scenes ("gud1".."gud5" as integers)
+SEEDS = {"uniform_512": 0x67756431, "blobs_4096": 0x67756432, "batch_256": 0x67756433,
+         "adjoint_1024": 0x67756434, "adjoint_batch_512": 0x67756435}
 
 
 class SplitMix64:
@@ -222,7 +224,7 @@ def random_colors(rng: SplitMix64, n: int) -> np.ndarray:
     return (b / np.float32(255.0)).astype(np.float32)
 
 
-def config1(n_shapes: int = 1000, size: int = 512, seed: int = SEEDS[1]) -> Scene:
+def uniform_512(n_shapes: int = 1000, size: int = 512, seed: int = SEEDS["uniform_512"]) -> Scene:
     """512x512, 1000 closed shapes of 3-12 quadratics whose on- and off-curve
     points are uniform over the canvas; solid random RGBA8 colours.  The
     reference fills even-odd only (Kernels.cl:298-304), so every shape here is
@@ -234,7 +236,7 @@ def config1(n_shapes: int = 1000, size: int = 512, seed: int = SEEDS[1]) -> Scen
     colors = random_colors(rng, n_shapes)
     tags = [solid_tag(i) for i in range(n_shapes)]
     return _assemble(size, size, tags, k, pts, colors, np.zeros(4, np.uint8),
-                     np.zeros((0, 24), np.uint8), "cfg1")
+                     np.zeros((0, 24), np.uint8), "uniform_512")
 
 
 def noise_texture(rng: SplitMix64, side: int = 1024) -> np.ndarray:
@@ -261,10 +263,10 @@ def _blob_pairs(rng, cx, cy, r, k):
     return np.stack([x, y, xc, yc], axis=1).astype(np.float32), owner
 
 
-def config2(n_blobs: int = 180_000, n_glyphs: int = 20_000, size: int = 4096,
-            seed: int = SEEDS[2], texture_side: int = 1024, radius_median: float = 8.0,
-            em_scale: float = 1.0, picture_share: int = 1, k_range=(4, 16),
-            gradient_share: int = 2) -> Scene:
+def blobs_4096(n_blobs: int = 180_000, n_glyphs: int = 20_000, size: int = 4096,
+               seed: int = SEEDS["blobs_4096"], texture_side: int = 1024, radius_median: float = 8.0,
+               em_scale: float = 1.0, picture_share: int = 1, k_range=(4, 16),
+               gradient_share: int = 2) -> Scene:
     """4096x4096, 200k shapes, ~2.1M quadratics.  Blobs: centre uniform,
     radius 8*exp(1.2*N(0,1)) (median 8 px), 4-16 quadratics.  Glyph-like
     outlines: 4-30 random quadratics inside a 12-48 px em box.  Substances (in
@@ -333,34 +335,34 @@ def config2(n_blobs: int = 180_000, n_glyphs: int = 20_000, size: int = 4096,
     for j in range(len(pic_idx)):
         uses.append((-float(np.float32(ox[j])), -float(np.float32(oy[j])), texture_side,
                      texture_side, 0, 1.0))
-    return _assemble(size, size, tags, k, pairs, subs, tex, pict_use_records(uses), "cfg2",
+    return _assemble(size, size, tags, k, pairs, subs, tex, pict_use_records(uses), "blobs",
                      gradients=grads)
 
 
-def scaled_config2(size: int, n_shapes: int, seed: int, glyph_share: float = 0.1) -> Scene:
-    """The config-2 generator at another canvas size and shape count."""
+def blobs_scaled(size: int, n_shapes: int, seed: int, glyph_share: float = 0.1) -> Scene:
+    """The 4096^2 blob generator at another canvas size and shape count."""
     n_g = int(round(n_shapes * glyph_share))
-    return config2(n_blobs=n_shapes - n_g, n_glyphs=max(n_g, 1), size=size, seed=seed,
-                   texture_side=256)
+    return blobs_4096(n_blobs=n_shapes - n_g, n_glyphs=max(n_g, 1), size=size, seed=seed,
+                      texture_side=256)
 
 
 def batch_scene(size: int, n_shapes: int, seed: int, k_range=(3, 12), gradient_share: int = 2,
                 picture_share: int = 0, texture_side: int = 16) -> Scene:
-    """One scene of the batched workload: config-2-style blobs and glyph
+    """One scene of the batched workload: blobs and glyph
     outlines on a size x size canvas, radii scaled so that the expected
     overdraw matches the 4096^2 scene's; solid and (gradient_share tenths)
     gradient substances."""
     radius = 8.0 * (size / 4096.0) * math.sqrt(200_000.0 / max(n_shapes, 1))
     radius = float(min(max(radius, 1.5), size / 8.0))
     n_g = max(1, n_shapes // 10)
-    sc = config2(n_blobs=max(1, n_shapes - n_g), n_glyphs=n_g, size=size, seed=seed,
-                 texture_side=texture_side, radius_median=radius, em_scale=max(0.25, radius / 8.0),
-                 picture_share=picture_share, k_range=k_range, gradient_share=gradient_share)
+    sc = blobs_4096(n_blobs=max(1, n_shapes - n_g), n_glyphs=n_g, size=size, seed=seed,
+                    texture_side=texture_side, radius_median=radius, em_scale=max(0.25, radius / 8.0),
+                    picture_share=picture_share, k_range=k_range, gradient_share=gradient_share)
     sc.name = f"batch{size}x{n_shapes}"
     return sc
 
 
-def config3(n_scenes: int = 256, seed: int = SEEDS[3]):
+def batch_256(n_scenes: int = 256, seed: int = SEEDS["batch_256"]):
     """256 independent scenes: side uniform over 256..2048, shape count
     log-uniform over [100, 50000], 3-12 quadratics per blob."""
     rng = SplitMix64(seed)
@@ -387,16 +389,16 @@ def perturb(scene: Scene, seed: int, point_sigma: float = 4.0, color_sigma: floa
                                name=scene.name + "-perturbed")
 
 
-def config4(seed: int = SEEDS[4]) -> Scene:
-    """Differentiable render: a 1024x1024 scene of 2,000 shapes (the config-2
+def adjoint_1024(seed: int = SEEDS["adjoint_1024"]) -> Scene:
+    """Differentiable render: a 1024x1024 scene of 2,000 shapes (the blob
     generator scaled, solid substances); the target is its render and the
     parameters a perturbed copy (``perturb``)."""
     sc = batch_scene(1024, 2000, seed & 0x7FFFFFFF, gradient_share=0)
-    sc.name = "cfg4"
+    sc.name = "adjoint_1024"
     return sc
 
 
-def config5(n_scenes: int = 64, seed: int = SEEDS[5]):
+def adjoint_batch_512(n_scenes: int = 64, seed: int = SEEDS["adjoint_batch_512"]):
     """Batched differentiable render: 64 scenes of 512x512 with 500-5,000
     shapes (log-uniform) of 3-8 quadratics, solid plus a 1024^2 RGBA8 noise
     texture (sampled with the spec's texture filter)."""
@@ -407,6 +409,6 @@ def config5(n_scenes: int = 64, seed: int = SEEDS[5]):
     for c, z in zip(counts, seeds):
         sc = batch_scene(512, int(c), int(z & np.uint64(0x7FFFFFFF)), k_range=(3, 8), gradient_share=0,
                          picture_share=1, texture_side=1024)
-        sc.name = f"cfg5-{int(c)}"
+        sc.name = f"adjoint-512-{int(c)}"
         out.append(sc)
     return out

@@ -53,7 +53,7 @@ def test_host_entry_points_reject_bad_arguments():
     L = _lib.load()
     h = C.c_void_p()
     assert L.gudni_geometry_build(None, 10, 10, C.byref(h)) == -1
-    scene = S.config1(5, 32, 1)
+    scene = S.uniform_512(5, 32, 1)
     geo = G.Geometry(scene)
     spec = G.default_spec()
     assert L.gudni_jobs_build(C.byref(spec), geo.handle, 0, 10, C.byref(h)) == -1

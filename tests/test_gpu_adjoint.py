@@ -25,7 +25,7 @@ def say(msg):
 
 
 @pytest.fixture(scope="module")
-def cfg4_small():
+def adjoint_small():
     sc, target = A.case(256, 150, 41)
     return sc, target
 
@@ -39,8 +39,8 @@ def gpu_adjoint(r, sc, target):
         adj.close()
 
 
-def test_adjoint_forward_matches_oracle(rasterizer, cfg4_small):
-    sc, target = cfg4_small
+def test_adjoint_forward_matches_oracle(rasterizer, adjoint_small):
+    sc, target = adjoint_small
     say("adjoint forward vs oracle")
     loss, dp, dc, img = gpu_adjoint(rasterizer, sc, target)
     ref = H.oracle_render(sc, want_f64=True)
@@ -54,8 +54,8 @@ def test_adjoint_forward_matches_oracle(rasterizer, cfg4_small):
     assert np.all(np.isfinite(dp)) and np.all(np.isfinite(dc))
 
 
-def test_adjoint_gradients_match_finite_differences(rasterizer, cfg4_small):
-    sc, target = cfg4_small
+def test_adjoint_gradients_match_finite_differences(rasterizer, adjoint_small):
+    sc, target = adjoint_small
     say("adjoint gradients")
     loss, dp, dc, _ = gpu_adjoint(rasterizer, sc, target)
     say("finite differences (64 samples, FP64)")

@@ -21,7 +21,7 @@ def say(msg):
 
 def batch_scenes():
     return [S.batch_scene(256, 300, 1), S.batch_scene(384, 2000, 2), textured_scene(),
-            compound_scene(), comb_scene(), S.config1(150, 200, 9), S.batch_scene(640, 5000, 3),
+            compound_scene(), comb_scene(), S.uniform_512(150, 200, 9), S.batch_scene(640, 5000, 3),
             H.tiny_square()]
 
 
@@ -92,7 +92,7 @@ def test_batch_other_tile_sizes(tile):
     r = G.Rasterizer(spec)
     try:
         r.set_stream(torch.cuda.current_stream().cuda_stream)
-        scenes = [S.batch_scene(320, 800, 5), S.config1(120, 160, 6), textured_scene()]
+        scenes = [S.batch_scene(320, 800, 5), S.uniform_512(120, 160, 6), textured_scene()]
         items = [(sc,) + G.prepare(sc, spec)[1:] for sc in scenes]
         say(f"tile {tile}: upload")
         b = G.Batch(r, items)

@@ -30,9 +30,9 @@ SCENES = {
     "textured": textured_scene,
     "comb": comb_scene,
     "empty": empty_scene,
-    "cfg1": lambda: S.config1(),
+    "uniform_512": lambda: S.uniform_512(),
     "batch_uneven": lambda: S.batch_scene(640, 5000, 3),
-    "cfg2_1024": lambda: S.scaled_config2(1024, 12500, 7),
+    "blobs_1024": lambda: S.blobs_scaled(1024, 12500, 7),
 }
 
 
@@ -66,22 +66,22 @@ def test_device_binning_other_tile_sizes(tile):
     spec = G.default_spec(max_tile_size=tile)
     r = G.Rasterizer(spec)
     try:
-        for scene in (S.batch_scene(320, 800, 5), S.config1(120, 160, 6)):
+        for scene in (S.batch_scene(320, 800, 5), S.uniform_512(120, 160, 6)):
             say(f"tile {tile}: {scene.name}")
             compare_binning(r, scene, spec)
     finally:
         r.close()
 
 
-def test_device_binning_full_cfg2(rasterizer):
-    scene = S.config2()
-    say("cfg2 4096^2 device binning")
+def test_device_binning_full_blobs_4096(rasterizer):
+    scene = S.blobs_4096()
+    say("blobs 4096^2 device binning")
     compare_binning(rasterizer, scene, rasterizer.spec)
 
 
 def test_raster_scene_matches_oracle(rasterizer):
     """gudni_raster_scene (upload + device binning + render + read back)."""
-    for scene in (textured_scene(), S.config1(200, 128, 3), S.batch_scene(384, 2000, 2)):
+    for scene in (textured_scene(), S.uniform_512(200, 128, 3), S.batch_scene(384, 2000, 2)):
         say(f"raster_scene {scene.name}")
         _, geo, _ = G.prepare(scene)
         out = rasterizer.raster_scene(geo, scene).reshape(scene.height, scene.width)

@@ -67,10 +67,10 @@ def check_frames(scene, gpu_u32, gpu_f32, ref):
     return err
 
 
-@pytest.mark.parametrize("name", ["tinySquare", "compound", "textured", "cfg1_small"])
+@pytest.mark.parametrize("name", ["tinySquare", "compound", "textured", "uniform_small"])
 def test_small_scenes_level2(rasterizer, name):
     scene = {"tinySquare": H.tiny_square, "compound": compound_scene, "textured": textured_scene,
-             "cfg1_small": lambda: S.config1(200, 128, 3)}[name]()
+             "uniform_small": lambda: S.uniform_512(200, 128, 3)}[name]()
     say(f"level2 {name} {scene.width}x{scene.height}")
     ref = H.oracle_render(scene)
     assert ref["rc"] == 0
@@ -88,7 +88,7 @@ def test_stochastic_field(mode):
     spec = G.default_spec(stochastic_factor=0.35, coverage_mode=mode)
     r = G.Rasterizer(spec)
     try:
-        for scene in (S.config1(200, 128, 3), textured_scene(), gradient_scene()):
+        for scene in (S.uniform_512(200, 128, 3), textured_scene(), gradient_scene()):
             say(f"stochastic mode {mode}: {scene.name}")
             ref = H.oracle_render(scene, spec)
             assert ref["rc"] == 0
@@ -134,18 +134,18 @@ def dump_parity(rasterizer, scene, job_ids, cap=64, spec=None):
                                       np.where(mask, od["shape_indices"], 0))
 
 
-def test_cfg1_dump_parity(rasterizer):
+def test_uniform_512_dump_parity(rasterizer):
     """Per-column records after generate and after sort, shape bits, initial
-    stack and shape indices: bit-identical to the oracle on two jobs.  cfg1's
+    stack and shape indices: bit-identical to the oracle on two jobs.  the 512^2 uniform scene's
     tiles sit at the 8-pixel floor with ~750 shapes each (general path)."""
-    dump_parity(rasterizer, S.config1(), lambda n: (0, n - 1))
+    dump_parity(rasterizer, S.uniform_512(), lambda n: (0, n - 1))
 
 
-def test_cfg2_dump_parity(rasterizer):
-    """The same on a 1024^2 slice of cfg2, whose tiles hold <= 126 shapes
+def test_blobs_1024_dump_parity(rasterizer):
+    """The same on a 1024^2 version of the 4096^2 blob scene, whose tiles hold <= 126 shapes
     (compact path: records labelled by tile-local shape index internally and
     relabelled to the reference's shape bits here)."""
-    dump_parity(rasterizer, S.scaled_config2(1024, 12500, 7), lambda n: range(n), cap=96)
+    dump_parity(rasterizer, S.blobs_scaled(1024, 12500, 7), lambda n: range(n), cap=96)
 
 
 @pytest.mark.parametrize("bits", [0, 2, 5])
@@ -153,52 +153,52 @@ def test_sort_radix_dump_parity(bits):
     """The radix pass ahead of the per-column insertion sort leaves the sorted
     records bit-identical to the oracle's for every digit width: 256-row
     columns of a sparse scene (tens of records each, compact path) and the
-    8-pixel-floor tiles of cfg1 (general path)."""
+    8-pixel-floor tiles of the 512^2 uniform scene (general path)."""
     spec = G.default_spec(sort_radix_bits=bits)
     r = G.Rasterizer(spec)
     try:
         dump_parity(r, S.batch_scene(512, 700, 21, gradient_share=0), lambda n: range(n), cap=256,
                     spec=spec)
-        dump_parity(r, S.config1(), lambda n: (n - 1,), spec=spec)
+        dump_parity(r, S.uniform_512(), lambda n: (n - 1,), spec=spec)
     finally:
         r.close()
 
 
-def test_cfg1_level1_and_level2(rasterizer):
-    scene = S.config1()
-    say("cfg1 oracle")
+def test_uniform_512_level1_and_level2(rasterizer):
+    scene = S.uniform_512()
+    say("uniform 512 oracle")
     ref = H.oracle_render(scene)
     assert ref["rc"] == 0
-    say("cfg1 level1 (per job, host buffers)")
+    say("uniform 512 level1 (per job, host buffers)")
     u1 = H.gpu_render_level1(rasterizer, scene)
     assert H.first_mismatch(u1, ref["out"]) is None
-    say("cfg1 level2 (whole frame, device buffers)")
+    say("uniform 512 level2 (whole frame, device buffers)")
     u2, f2 = H.gpu_render_level2(rasterizer, scene)
     err = check_frames(scene, u2, f2, ref)
-    say(f"cfg1 coverage max |f32 - f64| = {err:.3g}")
+    say(f"uniform 512 coverage max |f32 - f64| = {err:.3g}")
 
 
-def test_cfg2_scaled(rasterizer):
-    scene = S.scaled_config2(1024, 12500, 7)
-    say("cfg2 1024^2 scaled")
+def test_blobs_1024(rasterizer):
+    scene = S.blobs_scaled(1024, 12500, 7)
+    say("blobs 1024^2")
     ref = H.oracle_render(scene, nthreads=16)
     assert ref["rc"] == 0
     u2, f2 = H.gpu_render_level2(rasterizer, scene)
     err = check_frames(scene, u2, f2, ref)
-    say(f"cfg2-scaled coverage max err {err:.3g}")
+    say(f"blobs-1024 coverage max err {err:.3g}")
 
 
-def test_cfg2_full_frame(rasterizer):
+def test_blobs_4096_full_frame(rasterizer):
     """The full 4096^2, 200k-shape workload: every BGRA word equals the
     oracle's."""
-    scene = S.config2()
-    say("cfg2 full oracle (16 threads)")
+    scene = S.blobs_4096()
+    say("blobs 4096 oracle (16 threads)")
     ref = H.oracle_render(scene, want_f64=False, nthreads=16)
     assert ref["rc"] == 0
-    say("cfg2 full gpu")
+    say("blobs 4096 gpu")
     u2, _ = H.gpu_render_level2(rasterizer, scene)
     bad = H.first_mismatch(u2, ref["out"])
-    assert bad is None, f"cfg2 first mismatch {bad}"
+    assert bad is None, f"blobs 4096 first mismatch {bad}"
 
 
 def test_retry_pass_comb(rasterizer):
@@ -244,7 +244,7 @@ def test_empty_and_offcanvas(rasterizer):
 
 
 def test_nonsquare_canvas(rasterizer):
-    scene = S.config1(300, 200, 13)
+    scene = S.uniform_512(300, 200, 13)
     scene.height = 120
     say("non-square 200x120")
     ref = H.oracle_render(scene)
@@ -276,7 +276,7 @@ def test_gradients_and_texture_filters(filt):
     spec = G.default_spec(texture_filter=filt)
     r = G.Rasterizer(spec)
     try:
-        for scene in (gradient_scene(), S.scaled_config2(512, 4000, 17)):
+        for scene in (gradient_scene(), S.blobs_scaled(512, 4000, 17)):
             say(f"filter {filt}: {scene.name}")
             ref = H.oracle_render(scene, spec)
             assert ref["rc"] == 0

@@ -72,10 +72,10 @@ SCENES = {
     "empty": empty_scene,
     "awkward": awkward_scene,
     "awkward_finite": lambda: awkward_scene(with_nan=False),
-    "cfg1": lambda: S.config1(),
+    "uniform_512": lambda: S.uniform_512(),
     "batch_gradients": lambda: S.batch_scene(640, 5000, 3),
-    "cfg2_1024": lambda: S.scaled_config2(1024, 12500, 7),
-    "cfg4": lambda: S.config4(),
+    "blobs_1024": lambda: S.blobs_scaled(1024, 12500, 7),
+    "adjoint_1024": lambda: S.adjoint_1024(),
 }
 
 
@@ -107,13 +107,13 @@ def test_device_serialiser_equals_host(rasterizer, name):
     compare_geometry(rasterizer, scene)
 
 
-def test_device_serialiser_full_cfg2(rasterizer):
-    scene = S.config2()
-    say("cfg2 4096^2 geometry")
+def test_device_serialiser_full_blobs_4096(rasterizer):
+    scene = S.blobs_4096()
+    say("blobs 4096^2 geometry")
     compare_geometry(rasterizer, scene)
 
 
-@pytest.mark.parametrize("name", ["awkward_finite", "textured", "cfg2_1024"])
+@pytest.mark.parametrize("name", ["awkward_finite", "textured", "blobs_1024"])
 def test_frame_from_outlines(rasterizer, name):
     """Outlines -> device serialiser -> device binning -> render gives the
     host path's frame and the oracle's."""

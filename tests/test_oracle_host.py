@@ -102,7 +102,7 @@ def decode_strands(geo, start16, n):
 
 
 def test_strands_round_trip():
-    scene = S.config1(60, 200, 17)
+    scene = S.uniform_512(60, 200, 17)
     geo = G.Geometry(scene)
     og = O.Geometry(scene)
     assert np.array_equal(geo.geo, og.geo) and np.array_equal(geo.shapes, og.shapes)
@@ -155,7 +155,7 @@ def leaf_members(bbox, box, side):
                                   (1024, 5000, 300, 4), (100, 80, 4094, 5), (2048, 20000, 1000, 6)])
 def test_tile_tree_formulations_agree(case):
     size, n, max_strands, seed = case
-    scene = S.scaled_config2(size, n, seed) if seed % 2 == 0 else S.config1(n, size, seed)
+    scene = S.blobs_scaled(size, n, seed) if seed % 2 == 0 else S.uniform_512(n, size, seed)
     if seed == 2:
         scene.height = 400
     spec = G.default_spec(max_strands_per_tile=max_strands)

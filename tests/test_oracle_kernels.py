@@ -154,7 +154,7 @@ def test_porter_duff_stack_closed_form():
 
 def test_sorted_queue_is_stable_lexsort():
     """The sort stage equals numpy's stable sort by (top, topX, invSlope)."""
-    scene = S.config1(300, 256, 21)
+    scene = S.uniform_512(300, 256, 21)
     spec = O.default_spec()
     g = O.Geometry(scene)
     j = O.JobList(spec, g, scene.width, scene.height)
@@ -217,15 +217,15 @@ def test_shape_cap_keeps_bottom_most():
 
 
 def test_queue_overflow_reported():
-    scene = S.config1(100, 64, 3)
+    scene = S.uniform_512(100, 64, 3)
     res = H.oracle_render(scene, G.default_spec(max_thresholds=2))
     assert res["rc"] == -1
 
 
 def test_accurate_coverage_within_tolerance():
     """Accurate mode stays within 1e-5 of the FP64 integration on the
-    reference's workloads (cfg1 and a 1024^2 slice of cfg2)."""
-    for scene in (S.config1(300, 256, 4), S.scaled_config2(512, 3000, 8)):
+    reference's workloads (the 512^2 uniform scene and a 1024^2 version of the 4096^2 blob scene)."""
+    for scene in (S.uniform_512(300, 256, 4), S.blobs_scaled(512, 3000, 8)):
         res = H.oracle_render(scene)
         assert np.abs(res["f32"] - res["f64"]).max() < 1e-5
 
@@ -233,7 +233,7 @@ def test_accurate_coverage_within_tolerance():
 def test_reference_mode_differs_only_in_section_width():
     """Both coverage modes produce the same queue dumps; frames differ by at
     most one 8-bit step."""
-    scene = S.scaled_config2(512, 3000, 8)
+    scene = S.blobs_scaled(512, 3000, 8)
     a = H.oracle_render(scene, G.default_spec(coverage_mode=0))
     b = H.oracle_render(scene, G.default_spec(coverage_mode=1))
     da = a["out"].view(np.uint8).astype(int)

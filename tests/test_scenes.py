@@ -28,7 +28,7 @@ def _check_gradients(sc):
 @pytest.mark.parametrize("make", [
     lambda: S.batch_scene(256, 300, 5),
     lambda: S.batch_scene(512, 800, 11, gradient_share=5),
-    lambda: S.config2(n_blobs=3000, n_glyphs=300, size=1024),
+    lambda: S.blobs_4096(n_blobs=3000, n_glyphs=300, size=1024),
 ])
 def test_gradient_tables_are_valid(make):
     sc = make()
@@ -44,7 +44,7 @@ def test_gradient_record_rejects_bad_stops():
 
 
 def test_generators_are_reproducible():
-    a, b = S.config4(), S.config4()
+    a, b = S.adjoint_1024(), S.adjoint_1024()
     assert np.array_equal(a.pairs, b.pairs) and np.array_equal(a.substances, b.substances)
     p, q = S.perturb(a, 3), S.perturb(a, 3)
     assert np.array_equal(p.pairs, q.pairs)
