@@ -119,6 +119,27 @@ def device_binning_leg(r, scene, geo, stream, steps):
             "tiles": nt, "refs": nr, "work_units": nw}
 
 
+def copy_bandwidth(mib=2048, reps=10):
+    """Achievable HBM bandwidth on this device: a plain device-to-device copy
+    of `mib` MiB timed with events; bytes read + written per second."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return 2 * n * 4 / (ms * 1e-3) / 1e9
+
+
 def device_pipeline_leg(r, scene, spec, stream, steps):
     """The whole host side of a frame on the device too: outlines in host
     memory -> upload -> device serialiser -> device binning -> render, against
@@ -175,6 +196,19 @@ def render_f32(r, scene, spec):
     return f32.cpu().numpy().reshape(scene.height, scene.width, 4)
 
 
+def adjoint_bytes(scene, spec):
+    """Algorithmic bytes of one adjoint step: three passes over the tiles'
+    shape records and strands (forward, backward, point pass: recomputation,
+    nothing stored), the target read, the RGBA image written, the per-pixel
+    dL/d(colour) written and read back, and the gradients written (16 B per
+    outline pair, 16 B per substance)."""
+    import gudni_amd as G
+    _, geo, jobs = G.prepare(scene, spec)
+    px = scene.width * scene.height
+    return (3 * (algorithmic_bytes(scene, geo, jobs) - 4 * px) + 16 * px + 16 * px + 32 * px
+            + 16 * (scene.pairs.size // 4) + 16 * len(scene.substances))
+
+
 def _adjoint_spec(spec, tile, radix):
     import gudni_amd as G
     return G.default_spec(texture_filter=spec.texture_filter, max_tile_size=tile, sort_radix_bits=radix)
@@ -204,7 +238,7 @@ def _time_adjoint(spec, items_fn, runs=2):
         r.close()
 
 
-def adjoint_leg(spec, steps, log_fn):
+def adjoint_leg(spec, steps, log_fn, fixed=None):
     """The differentiable workload: a 1024^2, 2,000-shape target and a
     perturbed copy as parameters; one step = forward + backward (loss,
     dL/d points, dL/d RGBA) in one launch.  Tile size and sort radix come
@@ -220,8 +254,11 @@ def adjoint_leg(spec, steps, log_fn):
     def items(r, sp):
         return [(params, render_f32(r, base, sp))]
 
-    tile, radix, timings = autotune.search(lambda ts, rb: _time_adjoint(_adjoint_spec(spec, ts, rb), items))
-    log_fn(f"adjoint autotune {json.dumps(timings)} -> tile {tile}, radix {radix}")
+    if fixed:
+        (tile, radix), timings = fixed, None
+    else:
+        tile, radix, timings = autotune.search(lambda ts, rb: _time_adjoint(_adjoint_spec(spec, ts, rb), items))
+        log_fn(f"adjoint autotune {json.dumps(timings)} -> tile {tile}, radix {radix}")
     sp = _adjoint_spec(spec, tile, radix)
     r = G.Rasterizer(sp)
     stream = torch.cuda.current_stream()
@@ -248,22 +285,25 @@ def adjoint_leg(spec, steps, log_fn):
     full_ms = (time.perf_counter() - t) / steps * 1e3
     adj.close()
     r.close()
+    b_adj = adjoint_bytes(params, sp)
     log_fn(f"adjoint: {dev_ms:.2f} ms per step on the device, {full_ms:.2f} ms with host updates; "
            f"loss {loss0:.2f} -> {loss:.2f}")
     return {"workload": "1024x1024, 2000 shapes (17k quadratics), solid; params perturbed "
                         "(points + N(0,4px), colours + N(0,0.1))",
             "tile_size": tile, "sort_radix_bits": radix, "autotune_ms": timings,
             "ms_per_step_device": dev_ms, "steps_per_s_device": 1e3 / dev_ms,
+            "algorithmic_bytes": b_adj, "achieved_gbs": b_adj / (dev_ms * 1e-3) / 1e9,
             "ms_per_step_with_host_update": full_ms, "steps_per_s_with_host_update": 1e3 / full_ms,
             "loss_first": loss0, "loss_after_steps": loss, "n_points": int(params.pairs.size // 2),
             "n_substances": int(len(params.substances))}
 
 
-def batched_adjoint_leg(spec, steps, log_fn):
+def batched_adjoint_leg(spec, steps, log_fn, fixed=None):
     """64 scenes of 512^2 (500-5,000 shapes, solid + a 1024^2 texture): forward
     and backward of all scenes in one persistent launch, against 64
-    per-scene adjoint launches.  Tile size and radix are tuned on 8 of the
-    scenes."""
+    per-scene adjoint launches.  Tile size and radix are tuned on the whole
+    batch (a sample of a few scenes picked a tile size that was slower for
+    all 64)."""
     import torch
     import gudni_amd as G
     from gudni_amd import autotune, scenes as S
@@ -273,9 +313,12 @@ def batched_adjoint_leg(spec, steps, log_fn):
     def items_of(k):
         return lambda r, sp: [(params[i], render_f32(r, bases[i], sp)) for i in range(k)]
 
-    tile, radix, timings = autotune.search(
-        lambda ts, rb: _time_adjoint(_adjoint_spec(spec, ts, rb), items_of(8), runs=1))
-    log_fn(f"batched adjoint autotune {json.dumps(timings)} -> tile {tile}, radix {radix}")
+    if fixed:
+        (tile, radix), timings = fixed, None
+    else:
+        tile, radix, timings = autotune.search(
+            lambda ts, rb: _time_adjoint(_adjoint_spec(spec, ts, rb), items_of(len(bases)), runs=1))
+        log_fn(f"batched adjoint autotune {json.dumps(timings)} -> tile {tile}, radix {radix}")
     sp = _adjoint_spec(spec, tile, radix)
     r = G.Rasterizer(sp)
     stream = torch.cuda.current_stream()
@@ -301,11 +344,13 @@ def batched_adjoint_leg(spec, steps, log_fn):
     seq_ms = start.elapsed_time(stop) / steps
     adj.close()
     r.close()
+    b_adj = sum(adjoint_bytes(sc, sp) for sc, _ in items)
     log_fn(f"batched adjoint: {one_ms:.2f} ms per step in one launch, {seq_ms:.2f} ms as "
            f"{len(items)} launches")
     return {"workload": f"{len(items)} scenes of 512x512, 500-5000 shapes, solid + 1024^2 texture",
             "tile_size": tile, "sort_radix_bits": radix, "autotune_ms": timings,
             "ms_per_step": one_ms, "steps_per_s": 1e3 / one_ms,
+            "algorithmic_bytes": b_adj, "achieved_gbs": b_adj / (one_ms * 1e-3) / 1e9,
             "sequential_ms_per_step": seq_ms, "sequential_steps_per_s": 1e3 / seq_ms,
             "speedup_vs_sequential": seq_ms / one_ms,
             "shapes": int(sum(s.n_shapes for s, _ in items))}
@@ -329,6 +374,7 @@ def batched_leg(spec, steps, log_fn):
     table = autotune.tune(r, scenes, per_bucket=2, reps=2, log=log_fn)
     t_tune = time.perf_counter() - t
     items, chosen, radices, t_prep = autotune.prepare_batch(scenes, table)
+    b_alg = sum(algorithmic_bytes(sc, geo, jobs) for sc, geo, jobs in items)
     t = time.perf_counter()
     b = G.Batch(r, items)
     t_up = time.perf_counter() - t
@@ -359,6 +405,7 @@ def batched_leg(spec, steps, log_fn):
         "scenes": len(scenes), "pixels": pixels, "curves": curves,
         "ms_per_batch": ms, "gpixels_per_s": pixels / (ms * 1e-3) / 1e9,
         "mcurves_per_s": curves / (ms * 1e-3) / 1e6,
+        "algorithmic_bytes": b_alg, "achieved_gbs": b_alg / (ms * 1e-3) / 1e9,
         "latency_p50_ms": float(np.percentile(lat, 50)), "latency_p99_ms": float(np.percentile(lat, 99)),
         "sequential_ms": float(seq.sum()),
         "sequential_gpixels_per_s": pixels / (seq.sum() * 1e-3) / 1e9,
@@ -380,6 +427,11 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable workloads")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the two differentiable workloads and print their results")
+    ap.add_argument("--adjoint-tile", type=int, default=0,
+                    help="tile size of the differentiable workloads (default: autotuned)")
+    ap.add_argument("--adjoint-radix", type=int, default=4)
     ap.add_argument("--sort-radix", type=int, default=-1,
                     help="sort radix bits of the frame (default: chosen by the autotuner)")
     ap.add_argument("--texture-filter", type=int, default=1,
@@ -407,6 +459,13 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": n * 4, "copies": max(1, args.steps)}), flush=True)
+        return
+    if args.adjoint_only:
+        fixed = (args.adjoint_tile, args.adjoint_radix) if args.adjoint_tile else None
+        spec = G.default_spec(texture_filter=args.texture_filter)
+        res = {"adjoint": adjoint_leg(spec, max(3, args.batch_steps), log, fixed),
+               "batched_adjoint": batched_adjoint_leg(spec, max(2, args.batch_steps), log, fixed)}
+        print(json.dumps(res), flush=True)
         return
     t = time.perf_counter()
     scene = S.blobs_4096()
@@ -503,6 +562,8 @@ def main():
     log("device binning (stage a) of the same frame")
     binning = device_binning_leg(r, scene, geo, stream, max(3, min(args.steps, 10)))
     traffic, traffic_info = measured_traffic(fast_ms)
+    copy_gbs = copy_bandwidth()
+    log(f"device copy: {copy_gbs:.0f} GB/s achievable")
     log("outlines to frame on the device (serialise + bin + render)")
     pipeline = device_pipeline_leg(r, scene, spec, stream, max(2, min(args.steps, 5)))
     log(f"device pipeline: {json.dumps({k: round(v, 3) for k, v in pipeline.items()})}")
@@ -542,6 +603,7 @@ def main():
                      "traffic_detail": traffic_info,
                      "kernel": "raster_pass (fast pass)", "kernel_ms": fast_ms,
                      "algorithmic_bytes": b_alg, "retry_pass_ms": float(np.mean(retry)),
+                     "peak_measured_copy_gbs": copy_gbs, "frac_of_measured_copy": achieved / copy_gbs,
                      "stage_cycle_share": stage_share},
         "device_binning": binning,
         "device_pipeline": pipeline,
