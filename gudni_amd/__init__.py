@@ -467,6 +467,22 @@ class Adjoint:
         c = None if rgba is None else np.ascontiguousarray(rgba, np.float32)
         self.r._check(self.L.gudni_adjoint_update(self.r.ctx, self.handle, i, _ptr(p), _ptr(c)))
 
+    def update_device(self, i, pairs_ptr=None, rgba_ptr=None):
+        """New parameters for scene i from device memory (e.g. a torch
+        tensor's data_ptr()); re-serialised and re-binned on the device."""
+        self.r._check(self.L.gudni_adjoint_update_device(self.r.ctx, self.handle, i,
+                                                         C.c_void_p(pairs_ptr) if pairs_ptr else None,
+                                                         C.c_void_p(rgba_ptr) if rgba_ptr else None))
+
+    def result_device(self, i, loss_ptr=None, d_pairs_ptr=None, d_rgba_ptr=None, image_ptr=None):
+        """Stream-ordered copies of scene i's loss (f64), gradients and image
+        into device memory; errors surface at the next result() or check()."""
+        p = [C.c_void_p(x) if x else None for x in (loss_ptr, d_pairs_ptr, d_rgba_ptr, image_ptr)]
+        self.r._check(self.L.gudni_adjoint_result_device(self.r.ctx, self.handle, i, *p))
+
+    def check(self):
+        self.r._check(self.L.gudni_adjoint_check(self.r.ctx, self.handle))
+
     def run(self):
         self.r._check(self.L.gudni_adjoint_run(self.r.ctx, self.handle))
 

@@ -114,6 +114,10 @@ SIGNATURES = {
     "gudni_scene_read": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_void_p]),
     "gudni_last_bin_ms": (C.c_float, [C.c_void_p]),
+    "gudni_adjoint_update_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    "gudni_adjoint_result_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                                              C.c_void_p, C.c_void_p]),
+    "gudni_adjoint_check": (C.c_int, [C.c_void_p, C.c_void_p]),
     "gudni_scene_upload_outlines": (C.c_int, [C.c_void_p, C.POINTER(Outlines), C.POINTER(SceneDesc)]),
     "gudni_scene_geometry_sizes": (C.c_int, [C.c_void_p, C.c_void_p]),
     "gudni_scene_geometry_read": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

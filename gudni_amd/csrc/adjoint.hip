@@ -49,6 +49,7 @@
 
 #include "context.h"
 #include "device.h"
+#include "serialize.h"
 
 namespace gudni {
 namespace adj {
@@ -878,6 +879,19 @@ __global__ void adj_clear(float *p, size_t n)
         p[i] = 0.0f;
 }
 
+// Scene-local work units (tile << 2 | quarter) moved to the batch's tile
+// numbering, and the batch's per-tile scene index.
+__global__ void adj_rebase_work(const uint32_t *src, uint32_t *dst, int n, uint32_t tile_base)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        dst[i] = src[i] + (tile_base << 2);
+}
+
+__global__ void adj_fill_i32(int32_t *dst, int n, int32_t v)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = v;
+}
+
 __global__ void adj_reset(uint32_t *counters, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -903,15 +917,38 @@ struct gudni_adjoint_batch {
         std::vector<uint8_t> pict, uses;
         std::vector<gudni_gradient> grads;
     };
+    // A scene rebuilt on the device after a parameter update: its outline
+    // structure (uploaded once), serialised geometry with curve provenance,
+    // and its tile tree (scene-local work units).
+    struct DevScene {
+        DevBuf tags, soff, ooff;
+        DevBuf geo, shapes, bbox, sbounds, curves;
+        DevBuf tiles, tile_job, refs, ref_bounds, work;
+        int32_t n_tiles = 0, n_work = 0;
+        void release()
+        {
+            DevBuf *bufs[] = {&tags, &soff, &ooff, &geo, &shapes, &bbox, &sbounds, &curves,
+                              &tiles, &tile_job, &refs, &ref_bounds, &work};
+            for (DevBuf *b : bufs) b->release();
+        }
+    };
+    void drop_device_scenes()
+    {
+        for (DevScene &v : dev) v.release();
+        dev.clear();
+    }
     int device = 0;
     std::vector<Scene> scenes;
     DevBuf geo, subs, pict, uses, shapes, bounds, curves, points, target, image, gpix, d_points, d_rgba,
         loss, tiles, tile_job, tile_scene, work, descs, orig_fast, orig_slow, grads;
+    std::vector<DevScene> dev;              // empty until the first device rebuild
+    std::vector<size_t> geo_off, refs_off, curves_off, pict_off, uses_off, grads_off;
     std::vector<size_t> points_off, subs_off, pix_off;
     std::vector<int32_t> work_begin, work_count;
     int32_t n_work = 0;
     size_t n_points = 0, n_subs = 0, n_pix = 0;
     int blocks = 0;
+    bool broken = false;     // a device update failed part-way
 };
 
 namespace {
@@ -934,6 +971,8 @@ int check_scene(gudni_ctx *ctx, const gudni_adjoint_batch::Scene &sc, int s)
     return GUDNI_OK;
 }
 
+int fill_descs(gudni_ctx *ctx, gudni_adjoint_batch *a);
+
 // Serialises, bins and validates every scene, then (re)uploads all buffers
 // but the targets.
 int build(gudni_ctx *ctx, gudni_adjoint_batch *a)
@@ -946,8 +985,11 @@ int build(gudni_ctx *ctx, gudni_adjoint_batch *a)
     std::vector<TileRec> tiles;
     std::vector<int32_t> tile_job, tile_scene;
     std::vector<uint32_t> work;
-    std::vector<size_t> geo_off, pict_off, uses_off, refs_off, curves_off, grads_off;
+    std::vector<size_t> &geo_off = a->geo_off, &pict_off = a->pict_off, &uses_off = a->uses_off,
+                        &refs_off = a->refs_off, &curves_off = a->curves_off, &grads_off = a->grads_off;
+    geo_off.clear(); pict_off.clear(); uses_off.clear(); refs_off.clear(); curves_off.clear(); grads_off.clear();
     std::vector<gudni_gradient> grads;
+    a->drop_device_scenes();
     a->points_off.clear(); a->subs_off.clear(); a->pix_off.clear();
     a->work_begin.clear(); a->work_count.clear();
     size_t pix = 0;
@@ -1038,35 +1080,8 @@ int build(gudni_ctx *ctx, gudni_adjoint_batch *a)
     CHECK_HIP(ctx, a->d_points.ensure(8 * a->n_points));
     CHECK_HIP(ctx, a->d_rgba.ensure(16 * a->n_subs));
     CHECK_HIP(ctx, a->loss.ensure(8 * (size_t)n));
-    std::vector<adj::AdjDesc> descs((size_t)n);
-    for (int s = 0; s < n; ++s) {
-        const gudni_adjoint_batch::Scene &sc = a->scenes[s];
-        adj::AdjDesc &d = descs[s];
-        std::memset(&d, 0, sizeof(d));
-        d.s.geo = (const uint8_t *)a->geo.p + geo_off[s];
-        d.s.shapes = (const ShapeRec *)a->shapes.p + refs_off[s];
-        d.s.bounds = (const float4 *)a->bounds.p + refs_off[s];
-        d.s.substances = (const float4 *)a->subs.p + a->subs_off[s];
-        d.s.pict = (const uint8_t *)a->pict.p + pict_off[s];
-        d.s.uses = (const PictUse *)a->uses.p + uses_off[s];
-        d.s.out = nullptr;
-        d.s.out_f32 = (float4 *)a->image.p + a->pix_off[s];
-        d.s.grads = (const float4 *)((const gudni_gradient *)a->grads.p + grads_off[s]);
-        d.s.filter = ctx->spec.texture_filter;
-        d.s.sort_bits = ctx->spec.sort_radix_bits;
-        for (int i = 0; i < 4; ++i) d.s.bg[i] = sc.bg[i];
-        d.s.width = sc.width;
-        d.s.height = sc.height;
-        d.curves = (const CurveSrc *)a->curves.p + curves_off[s];
-        d.points = (const float2 *)a->points.p + a->points_off[s];
-        d.target = (const float4 *)a->target.p + a->pix_off[s];
-        d.gpix = (float4 *)a->gpix.p + a->pix_off[s];
-        d.d_points = (float *)a->d_points.p + 2 * a->points_off[s];
-        d.d_rgba = (float *)a->d_rgba.p + 4 * a->subs_off[s];
-        d.loss = (double *)a->loss.p + s;
-        d.n_points = (int32_t)(sc.pairs.size() / 2);
-    }
-    CHECK_HIP(ctx, upload(ctx, a->descs, descs.data(), sizeof(adj::AdjDesc) * descs.size()));
+    rc = fill_descs(ctx, a);
+    if (rc) return rc;
     // scratch: the raster passes' rings plus a ring of line origins
     if (!a->blocks) {
         int per_cu = 0;
@@ -1078,6 +1093,157 @@ int build(gudni_ctx *ctx, gudni_adjoint_batch *a)
     while ((int)scap < ctx->spec.max_thresholds) scap <<= 1;
     CHECK_HIP(ctx, a->orig_fast.ensure(fast_slots * ((size_t)1 << dev::kFastCapLog2) * 2 * 64 * 16));
     CHECK_HIP(ctx, a->orig_slow.ensure(slow_slots * scap * 2 * 64 * 16));
+    CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return GUDNI_OK;
+}
+
+// Scene descriptors: the geometry, tile references, their extents and the
+// curve provenance of each scene come from its device rebuild once there is
+// one, else from the host build's concatenated buffers.
+int fill_descs(gudni_ctx *ctx, gudni_adjoint_batch *a)
+{
+    const int n = (int)a->scenes.size();
+    std::vector<adj::AdjDesc> descs((size_t)n);
+    for (int s = 0; s < n; ++s) {
+        const gudni_adjoint_batch::Scene &sc = a->scenes[s];
+        adj::AdjDesc &d = descs[s];
+        std::memset(&d, 0, sizeof(d));
+        if (!a->dev.empty()) {
+            const gudni_adjoint_batch::DevScene &v = a->dev[s];
+            d.s.geo = (const uint8_t *)v.geo.p;
+            d.s.shapes = (const ShapeRec *)v.refs.p;
+            d.s.bounds = (const float4 *)v.ref_bounds.p;
+            d.curves = (const CurveSrc *)v.curves.p;
+        } else {
+            d.s.geo = (const uint8_t *)a->geo.p + a->geo_off[s];
+            d.s.shapes = (const ShapeRec *)a->shapes.p + a->refs_off[s];
+            d.s.bounds = (const float4 *)a->bounds.p + a->refs_off[s];
+            d.curves = (const CurveSrc *)a->curves.p + a->curves_off[s];
+        }
+        d.s.substances = (const float4 *)a->subs.p + a->subs_off[s];
+        d.s.pict = (const uint8_t *)a->pict.p + a->pict_off[s];
+        d.s.uses = (const PictUse *)a->uses.p + a->uses_off[s];
+        d.s.out = nullptr;
+        d.s.out_f32 = (float4 *)a->image.p + a->pix_off[s];
+        d.s.grads = (const float4 *)((const gudni_gradient *)a->grads.p + a->grads_off[s]);
+        d.s.filter = ctx->spec.texture_filter;
+        d.s.sort_bits = ctx->spec.sort_radix_bits;
+        for (int i = 0; i < 4; ++i) d.s.bg[i] = sc.bg[i];
+        d.s.width = sc.width;
+        d.s.height = sc.height;
+        d.points = (const float2 *)a->points.p + a->points_off[s];
+        d.target = (const float4 *)a->target.p + a->pix_off[s];
+        d.gpix = (float4 *)a->gpix.p + a->pix_off[s];
+        d.d_points = (float *)a->d_points.p + 2 * a->points_off[s];
+        d.d_rgba = (float *)a->d_rgba.p + 4 * a->subs_off[s];
+        d.loss = (double *)a->loss.p + s;
+        d.n_points = (int32_t)(sc.pairs.size() / 2);
+    }
+    CHECK_HIP(ctx, upload(ctx, a->descs, descs.data(), sizeof(adj::AdjDesc) * descs.size()));
+    return GUDNI_OK;
+}
+
+// Serialises (with curve provenance) and bins scene s on the device from the
+// parameters in a->points, into a->dev[s].
+int device_rebuild_scene(gudni_ctx *ctx, gudni_adjoint_batch *a, int s)
+{
+    const gudni_adjoint_batch::Scene &sc = a->scenes[s];
+    gudni_adjoint_batch::DevScene &v = a->dev[s];
+    const int32_t n = (int32_t)sc.tags.size();
+    if (!v.tags.p) {
+        CHECK_HIP(ctx, upload(ctx, v.tags, sc.tags.data(), 8u * (size_t)n));
+        CHECK_HIP(ctx, upload(ctx, v.soff, sc.shape_off.data(), 4u * sc.shape_off.size()));
+        CHECK_HIP(ctx, upload(ctx, v.ooff, sc.outline_off.data(), 4u * sc.outline_off.size()));
+    }
+    SerIn in;
+    in.tags = (const uint64_t *)v.tags.p;
+    in.soff = (const int32_t *)v.soff.p;
+    in.ooff = (const int32_t *)v.ooff.p;
+    in.pairs = (const float *)a->points.p + 2 * a->points_off[s];
+    in.n_shapes = n;
+    in.width = sc.width;
+    in.height = sc.height;
+    in.subs = (const float4 *)a->subs.p + a->subs_off[s];
+    in.n_subs = (int32_t)sc.subs.size();
+    in.n_uses = (int32_t)(sc.uses.size() / 24);
+    in.n_grads = (int32_t)sc.grads.size();
+    const SerOut out{&v.geo, &v.shapes, &v.bbox, &v.sbounds, &v.curves};
+    SerCounts cnt;
+    int rc = serialize_outlines(ctx, in, out, cnt);
+    if (rc) return rc;
+    const BinIn bin{(const ShapeRec *)v.shapes.p, (const float4 *)v.bbox.p, (const float4 *)v.sbounds.p,
+                    (int32_t)cnt.n_kept, sc.width, sc.height};
+    const BinOut bo{&v.tiles, &v.tile_job, &v.refs, &v.ref_bounds, &v.work};
+    BinCounts bc;
+    rc = bin_scene(ctx, bin, bo, bc);
+    if (rc) return rc;
+    v.n_tiles = bc.n_tiles;
+    v.n_work = bc.n_work;
+    return GUDNI_OK;
+}
+
+// The batch's tile, job, scene and work tables from every scene's device
+// rebuild, and the descriptors pointing at them.
+int assemble(gudni_ctx *ctx, gudni_adjoint_batch *a)
+{
+    const hipStream_t st = ctx->stream;
+    const int n = (int)a->scenes.size();
+    size_t T = 0, W = 0;
+    for (const auto &v : a->dev) { T += (size_t)v.n_tiles; W += (size_t)v.n_work; }
+    if (T >= (1u << 30)) return set_error(ctx, GUDNI_ERR_ARG, "adjoint: too many tiles");
+    CHECK_HIP(ctx, a->tiles.ensure(32 * T));
+    CHECK_HIP(ctx, a->tile_job.ensure(4 * T));
+    CHECK_HIP(ctx, a->tile_scene.ensure(4 * T));
+    CHECK_HIP(ctx, a->work.ensure(4 * W));
+    a->work_begin.assign((size_t)n, 0);
+    a->work_count.assign((size_t)n, 0);
+    size_t tb = 0, wb = 0;
+    for (int s = 0; s < n; ++s) {
+        const gudni_adjoint_batch::DevScene &v = a->dev[s];
+        if (v.n_tiles) {
+            CHECK_HIP(ctx, hipMemcpyAsync((uint8_t *)a->tiles.p + 32 * tb, v.tiles.p, 32 * (size_t)v.n_tiles,
+                                          hipMemcpyDeviceToDevice, st));
+            CHECK_HIP(ctx, hipMemcpyAsync((int32_t *)a->tile_job.p + tb, v.tile_job.p, 4 * (size_t)v.n_tiles,
+                                          hipMemcpyDeviceToDevice, st));
+            hipLaunchKernelGGL(adj::adj_fill_i32, dim3(std::min(1024, (v.n_tiles + 255) / 256)), dim3(256), 0, st,
+                               (int32_t *)a->tile_scene.p + tb, v.n_tiles, (int32_t)s);
+        }
+        if (v.n_work)
+            hipLaunchKernelGGL(adj::adj_rebase_work, dim3(std::min(1024, (v.n_work + 255) / 256)), dim3(256), 0, st,
+                               (const uint32_t *)v.work.p, (uint32_t *)a->work.p + wb, v.n_work, (uint32_t)tb);
+        a->work_begin[s] = (int32_t)wb;
+        a->work_count[s] = v.n_work;
+        tb += (size_t)v.n_tiles;
+        wb += (size_t)v.n_work;
+    }
+    CHECK_HIP(ctx, hipGetLastError());
+    a->n_work = (int32_t)wb;
+    int rc = ensure_control(ctx, a->n_work);
+    if (rc) return rc;
+    return fill_descs(ctx, a);
+}
+
+// After new outline points for `scene`: the first time every scene is
+// rebuilt on the device (so all tables come from one place), afterwards
+// only that scene.
+int device_update(gudni_ctx *ctx, gudni_adjoint_batch *a, int scene)
+{
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    int rc = GUDNI_OK;
+    if (a->dev.empty()) {
+        a->dev.resize(a->scenes.size());
+        for (int s = 0; s < (int)a->scenes.size() && rc == GUDNI_OK; ++s) rc = device_rebuild_scene(ctx, a, s);
+    } else {
+        rc = device_rebuild_scene(ctx, a, scene);
+    }
+    if (rc) {
+        // the parameters on the device no longer match any complete set of
+        // tables: the batch refuses to run until it is uploaded again
+        a->broken = true;
+        return rc;
+    }
+    rc = assemble(ctx, a);
+    if (rc) return rc;
     CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return GUDNI_OK;
 }
@@ -1094,6 +1260,11 @@ int copy_scene(gudni_ctx *ctx, const gudni_adjoint_scene &in, gudni_adjoint_batc
     if (rc) return rc;
     rc = validate_gradients(ctx, in.gradients, in.n_gradients);
     if (rc) return rc;
+    {
+        int32_t n_outlines = 0, n_pairs = 0;   // the device serialiser indexes with these
+        rc = check_outline_offsets(ctx, &o, n_outlines, n_pairs);
+        if (rc) return set_error(ctx, rc, "adjoint scene " + std::to_string(s) + ": " + ctx->error);
+    }
     sc.width = in.width;
     sc.height = in.height;
     std::memcpy(sc.bg, in.background, sizeof(sc.bg));
@@ -1218,15 +1389,44 @@ int gudni_adjoint_update(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t scene, 
 {
     if (!ctx || !a || scene < 0 || scene >= (int32_t)a->scenes.size()) return GUDNI_ERR_ARG;
     gudni_adjoint_batch::Scene &sc = a->scenes[scene];
-    if (pairs) std::memcpy(sc.pairs.data(), pairs, 4 * sc.pairs.size());
-    if (rgba) std::memcpy(sc.subs.data(), rgba, 16 * sc.subs.size());
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
     CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return build(ctx, a);
+    if (pairs) {
+        std::memcpy(sc.pairs.data(), pairs, 4 * sc.pairs.size());
+        CHECK_HIP(ctx, hipMemcpyAsync((float *)a->points.p + 2 * a->points_off[scene], sc.pairs.data(),
+                                      4 * sc.pairs.size(), hipMemcpyHostToDevice, ctx->stream));
+    }
+    if (rgba) {
+        std::memcpy(sc.subs.data(), rgba, 16 * sc.subs.size());
+        CHECK_HIP(ctx, hipMemcpyAsync((float4 *)a->subs.p + a->subs_off[scene], sc.subs.data(),
+                                      16 * sc.subs.size(), hipMemcpyHostToDevice, ctx->stream));
+    }
+    if (pairs) return device_update(ctx, a, scene);
+    CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return GUDNI_OK;
+}
+
+int gudni_adjoint_update_device(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t scene, const float *pairs,
+                                const float *rgba)
+{
+    if (!ctx || !a || scene < 0 || scene >= (int32_t)a->scenes.size()) return GUDNI_ERR_ARG;
+    const gudni_adjoint_batch::Scene &sc = a->scenes[scene];
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    if (pairs && !sc.pairs.empty())
+        CHECK_HIP(ctx, hipMemcpyAsync((float *)a->points.p + 2 * a->points_off[scene], pairs,
+                                      4 * sc.pairs.size(), hipMemcpyDeviceToDevice, ctx->stream));
+    if (rgba && !sc.subs.empty())
+        CHECK_HIP(ctx, hipMemcpyAsync((float4 *)a->subs.p + a->subs_off[scene], rgba, 16 * sc.subs.size(),
+                                      hipMemcpyDeviceToDevice, ctx->stream));
+    if (pairs) return device_update(ctx, a, scene);
+    CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return GUDNI_OK;
 }
 
 int gudni_adjoint_run(gudni_ctx *ctx, gudni_adjoint_batch *a)
 {
     if (!ctx || !a) return GUDNI_ERR_ARG;
+    if (a->broken) return set_error(ctx, GUDNI_ERR_ARG, "adjoint batch: an update failed; upload it again");
     CHECK_HIP(ctx, hipSetDevice(ctx->device));
     return launch(ctx, a, 0, a->n_work, 0, a->n_points, 0, a->n_subs, 0, (int)a->scenes.size());
 }
@@ -1234,6 +1434,7 @@ int gudni_adjoint_run(gudni_ctx *ctx, gudni_adjoint_batch *a)
 int gudni_adjoint_run_scene(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t scene)
 {
     if (!ctx || !a || scene < 0 || scene >= (int32_t)a->scenes.size()) return GUDNI_ERR_ARG;
+    if (a->broken) return set_error(ctx, GUDNI_ERR_ARG, "adjoint batch: an update failed; upload it again");
     CHECK_HIP(ctx, hipSetDevice(ctx->device));
     const size_t p0 = a->points_off[scene], p1 = p0 + a->scenes[scene].pairs.size() / 2;
     const size_t s0 = a->subs_off[scene], s1 = s0 + a->scenes[scene].subs.size();
@@ -1266,6 +1467,33 @@ int gudni_adjoint_result(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t scene, 
     return GUDNI_OK;
 }
 
+int gudni_adjoint_result_device(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t scene, double *loss,
+                                float *d_pairs, float *d_rgba, float *image)
+{
+    if (!ctx || !a || scene < 0 || scene >= (int32_t)a->scenes.size()) return GUDNI_ERR_ARG;
+    const gudni_adjoint_batch::Scene &sc = a->scenes[scene];
+    const hipStream_t st = ctx->stream;
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    if (loss)
+        CHECK_HIP(ctx, hipMemcpyAsync(loss, (double *)a->loss.p + scene, 8, hipMemcpyDeviceToDevice, st));
+    if (d_pairs && !sc.pairs.empty())
+        CHECK_HIP(ctx, hipMemcpyAsync(d_pairs, (float *)a->d_points.p + 2 * a->points_off[scene],
+                                      4 * sc.pairs.size(), hipMemcpyDeviceToDevice, st));
+    if (d_rgba && !sc.subs.empty())
+        CHECK_HIP(ctx, hipMemcpyAsync(d_rgba, (float *)a->d_rgba.p + 4 * a->subs_off[scene],
+                                      16 * sc.subs.size(), hipMemcpyDeviceToDevice, st));
+    if (image)
+        CHECK_HIP(ctx, hipMemcpyAsync(image, (float4 *)a->image.p + a->pix_off[scene],
+                                      16 * (size_t)sc.width * sc.height, hipMemcpyDeviceToDevice, st));
+    return GUDNI_OK;
+}
+
+int gudni_adjoint_check(gudni_ctx *ctx, gudni_adjoint_batch *a)
+{
+    if (!ctx || !a) return GUDNI_ERR_ARG;
+    return read_status(ctx);
+}
+
 void gudni_adjoint_free(gudni_adjoint_batch *a)
 {
     if (!a) return;
@@ -1275,6 +1503,7 @@ void gudni_adjoint_free(gudni_adjoint_batch *a)
                       &a->tiles, &a->tile_job, &a->tile_scene, &a->work, &a->descs, &a->orig_fast,
                       &a->orig_slow, &a->grads};
     for (DevBuf *b : bufs) b->release();
+    a->drop_device_scenes();
     delete a;
 }
 

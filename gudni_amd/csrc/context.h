@@ -77,7 +77,6 @@ struct gudni_ctx {
     float last_bin_ms = 0;
     // device serialisation (gudni_scene_upload_outlines)
     gudni::DevBuf ser_tags, ser_soff, ser_ooff, ser_pairs, ser_work;
-    uint64_t *ser_partials = nullptr;   // inside ser_work
     size_t scene_geo_bytes = 0;
     hipEvent_t ser_ev[2] = {nullptr, nullptr};
     float last_ser_ms = 0;
@@ -138,6 +137,23 @@ dev::Frame base_frame(gudni_ctx *ctx);
 // Waits for the stream and turns the status word into an error code.
 int read_status(gudni_ctx *ctx);
 int validate_uses(gudni_ctx *ctx, const void *pict_uses, int32_t n_pict_uses, size_t pict_bytes);
+// Stage (a) on the device for one scene whose shape records, boxes and
+// strand extents are in device memory: tiles, reversed job index per tile,
+// shape references, their extents and the work list into the caller's
+// buffers (grown as needed), with the context's spec.  One
+// synchronisation (the counts); the capacities retry when exceeded.
+struct BinIn {
+    const ShapeRec *shapes;
+    const float4 *bbox, *bounds;
+    int32_t n_shapes, width, height;
+};
+struct BinOut {
+    DevBuf *tiles, *tile_job, *refs, *ref_bounds, *work;
+};
+struct BinCounts {
+    int32_t n_tiles = 0, n_refs = 0, n_work = 0;
+};
+int bin_scene(gudni_ctx *ctx, const BinIn &in, const BinOut &out, BinCounts &cnt);
 // Bounds-checks shape records (and tiles) against the common buffers and
 // fills each shape's strand extent (xmin, ymin, xmax, ymax).
 int validate_job(gudni_ctx *ctx, const CommonView &cv, const ShapeRec *shapes, int32_t n_shapes,

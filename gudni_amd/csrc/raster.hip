@@ -377,6 +377,59 @@ int validate_job(gudni_ctx *ctx, const CommonView &cv, const ShapeRec *shapes,
     return GUDNI_OK;
 }
 
+int bin_scene(gudni_ctx *ctx, const BinIn &in, const BinOut &out, BinCounts &cnt)
+{
+    binning::Params bp;
+    bp.n_shapes = in.n_shapes;
+    bp.canvas_depth = ceil_log2(std::max(in.width, in.height));
+    bp.tile_depth = ceil_log2(ctx->spec.max_tile_size);
+    bp.cap_shapes = ctx->spec.max_shapes - 1;
+    bp.cap_strands = (uint32_t)ctx->spec.max_strands_per_tile;
+    bp.per_job = ctx->spec.threads_per_tile;
+    bp.col_step = ctx->spec.max_tiles_per_call;
+    bp.threads_per_tile = ctx->spec.threads_per_tile;
+    bp.width = in.width;
+    bp.height = in.height;
+    bp.max_levels = 2 * bp.canvas_depth + 4;
+    if (!ctx->bin_cap_pairs) {
+        ctx->bin_cap_pairs = std::max(1 << 16, 8 * bp.n_shapes);
+        ctx->bin_cap_nodes = std::max(1 << 15, bp.n_shapes / 4);
+    }
+    if (ctx->timing)
+        for (hipEvent_t &e : ctx->bin_ev)
+            if (!e) CHECK_HIP(ctx, hipEventCreate(&e));
+    uint32_t status[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int attempt = 0;; ++attempt) {
+        const int32_t cp = std::max(ctx->bin_cap_pairs, bp.n_shapes), cn = ctx->bin_cap_nodes;
+        CHECK_HIP(ctx, ctx->bin_ws.ensure(binning::workspace_bytes(cp, cn, bp.max_levels)));
+        CHECK_HIP(ctx, out.tiles->ensure(32u * (size_t)cn));
+        CHECK_HIP(ctx, out.tile_job->ensure(4u * (size_t)cn));
+        CHECK_HIP(ctx, out.refs->ensure(16u * (size_t)cp));
+        CHECK_HIP(ctx, out.ref_bounds->ensure(16u * (size_t)cp));
+        CHECK_HIP(ctx, out.work->ensure(16u * (size_t)cn));
+        binning::Workspace ws;
+        binning::carve(ws, ctx->bin_ws.p, cp, cn, bp.max_levels);
+        binning::Output o{(TileRec *)out.tiles->p, (int32_t *)out.tile_job->p, (ShapeRec *)out.refs->p,
+                          (float4 *)out.ref_bounds->p, (uint32_t *)out.work->p};
+        if (ctx->timing) CHECK_HIP(ctx, hipEventRecord(ctx->bin_ev[0], ctx->stream));
+        CHECK_HIP(ctx, binning::enqueue(bp, ws, in.shapes, in.bbox, in.bounds, o, ctx->stream));
+        if (ctx->timing) CHECK_HIP(ctx, hipEventRecord(ctx->bin_ev[1], ctx->stream));
+        CHECK_HIP(ctx, hipMemcpyAsync(status, ws.status, sizeof(status), hipMemcpyDeviceToHost,
+                                      ctx->stream));
+        CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (status[0] == 0) break;
+        if (status[0] != 1 || attempt >= 8)
+            return set_error(ctx, GUDNI_ERR_ARG, "scene binning: tile tree deeper than the canvas allows");
+        ctx->bin_cap_pairs = 4 * cp;
+        ctx->bin_cap_nodes = 4 * cn;
+    }
+    if (ctx->timing) CHECK_HIP(ctx, hipEventElapsedTime(&ctx->last_bin_ms, ctx->bin_ev[0], ctx->bin_ev[1]));
+    cnt.n_tiles = (int32_t)status[1];
+    cnt.n_refs = (int32_t)status[2];
+    cnt.n_work = (int32_t)status[3];
+    return GUDNI_OK;
+}
+
 } // namespace gudni
 
 extern "C" {
@@ -743,54 +796,13 @@ int gudni_scene_bin(gudni_ctx *ctx)
     CHECK_HIP(ctx, hipSetDevice(ctx->device));
     int rc = ensure_scratch(ctx);
     if (rc) return rc;
-    binning::Params bp;
-    bp.n_shapes = ctx->scene_n_shapes;
-    bp.canvas_depth = ceil_log2(std::max(ctx->scene_w, ctx->scene_h));
-    bp.tile_depth = ceil_log2(ctx->spec.max_tile_size);
-    bp.cap_shapes = ctx->spec.max_shapes - 1;
-    bp.cap_strands = (uint32_t)ctx->spec.max_strands_per_tile;
-    bp.per_job = ctx->spec.threads_per_tile;
-    bp.col_step = ctx->spec.max_tiles_per_call;
-    bp.threads_per_tile = ctx->spec.threads_per_tile;
-    bp.width = ctx->scene_w;
-    bp.height = ctx->scene_h;
-    bp.max_levels = 2 * bp.canvas_depth + 4;
-    if (!ctx->bin_cap_pairs) {
-        ctx->bin_cap_pairs = std::max(1 << 16, 8 * bp.n_shapes);
-        ctx->bin_cap_nodes = std::max(1 << 15, bp.n_shapes / 4);
-    }
-    if (ctx->timing)
-        for (hipEvent_t &e : ctx->bin_ev)
-            if (!e) CHECK_HIP(ctx, hipEventCreate(&e));
-    uint32_t status[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int attempt = 0;; ++attempt) {
-        const int32_t cp = std::max(ctx->bin_cap_pairs, bp.n_shapes), cn = ctx->bin_cap_nodes;
-        CHECK_HIP(ctx, ctx->bin_ws.ensure(binning::workspace_bytes(cp, cn, bp.max_levels)));
-        CHECK_HIP(ctx, ctx->tiles.ensure(32u * (size_t)cn));
-        CHECK_HIP(ctx, ctx->tile_job.ensure(4u * (size_t)cn));
-        CHECK_HIP(ctx, ctx->shapes.ensure(16u * (size_t)cp));
-        CHECK_HIP(ctx, ctx->xrange.ensure(16u * (size_t)cp));
-        CHECK_HIP(ctx, ctx->work.ensure(16u * (size_t)cn));
-        binning::Workspace ws;
-        binning::carve(ws, ctx->bin_ws.p, cp, cn, bp.max_levels);
-        binning::Output out{(TileRec *)ctx->tiles.p, (int32_t *)ctx->tile_job.p,
-                            (ShapeRec *)ctx->shapes.p, (float4 *)ctx->xrange.p,
-                            (uint32_t *)ctx->work.p};
-        if (ctx->timing) CHECK_HIP(ctx, hipEventRecord(ctx->bin_ev[0], ctx->stream));
-        CHECK_HIP(ctx, binning::enqueue(bp, ws, (const ShapeRec *)ctx->scene_shapes.p,
-                                        (const float4 *)ctx->scene_bbox.p,
-                                        (const float4 *)ctx->scene_bounds.p, out, ctx->stream));
-        if (ctx->timing) CHECK_HIP(ctx, hipEventRecord(ctx->bin_ev[1], ctx->stream));
-        CHECK_HIP(ctx, hipMemcpyAsync(status, ws.status, sizeof(status), hipMemcpyDeviceToHost,
-                                      ctx->stream));
-        CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        if (status[0] == 0) break;
-        if (status[0] != 1 || attempt >= 8)
-            return set_error(ctx, GUDNI_ERR_ARG, "gudni_scene_bin: tile tree deeper than the canvas allows");
-        ctx->bin_cap_pairs = 4 * cp;
-        ctx->bin_cap_nodes = 4 * cn;
-    }
-    if (ctx->timing) CHECK_HIP(ctx, hipEventElapsedTime(&ctx->last_bin_ms, ctx->bin_ev[0], ctx->bin_ev[1]));
+    const BinIn in{(const ShapeRec *)ctx->scene_shapes.p, (const float4 *)ctx->scene_bbox.p,
+                   (const float4 *)ctx->scene_bounds.p, ctx->scene_n_shapes, ctx->scene_w, ctx->scene_h};
+    const BinOut out{&ctx->tiles, &ctx->tile_job, &ctx->shapes, &ctx->xrange, &ctx->work};
+    BinCounts cnt;
+    rc = bin_scene(ctx, in, out, cnt);
+    if (rc) return rc;
+    const uint32_t status[4] = {0u, (uint32_t)cnt.n_tiles, (uint32_t)cnt.n_refs, (uint32_t)cnt.n_work};
     rc = ensure_control(ctx, (int)status[3]);
     if (rc) return rc;
     ctx->frame_w = ctx->scene_w;

@@ -18,6 +18,10 @@
 //             shape's record and box
 //   extents   per kept shape the strands' extent (the generation prefilter's
 //             input), read back from the heap as the host computes it
+// With curve tracking (the adjoint's serialisation) each strand's header
+// dword 1 holds the index of its first curve and every strand curve gets a
+// CurveSrc record (its outline points and knob-split half), numbered in heap
+// order, as the host's tracked build numbers them.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,6 +33,7 @@
 #include "context.h"
 #include "device.h"
 #include "scan.h"
+#include "serialize.h"
 
 namespace gudni {
 namespace ser {
@@ -55,6 +60,8 @@ struct Args {
     uint8_t *heap;
     ShapeRec *shapes;
     float4 *bbox, *bounds;
+    CurveSrc *curves;           // null: no curve tracking
+    uint64_t *cstart;           // per shape: its first curve (scan)
 };
 
 // between t v0 v1 = v0 ^* (1-t) ^+^ v1 ^* t   (Deknob.hs:40-41)
@@ -67,10 +74,11 @@ __device__ __forceinline__ Pt lerp_t(float t, Pt a, Pt b)
 // A control point strictly outside the end points' x range is a knob: split
 // where dx/dt changes sign, found by bisection on FP32 parameters to
 // iota = 1e-4 (Deknob.hs:61-79, Figure/Space.hs:101-102).
-__device__ int split_knob(const Quad &q, Quad *out)
+__device__ int split_knob(const Quad &q, Quad *out, float &split_t)
 {
     const bool bulge_left = q.c.x < q.p0.x && q.c.x < q.p1.x;
     const bool bulge_right = q.c.x > q.p0.x && q.c.x > q.p1.x;
+    split_t = 0.0f;
     if (!bulge_left && !bulge_right) {
         out[0] = q;
         return 1;
@@ -98,6 +106,7 @@ __device__ int split_knob(const Quad &q, Quad *out)
     }
     out[0] = Quad{q.p0, m0, on};
     out[1] = Quad{on, m1, q.p1};
+    split_t = t;
     return 2;
 }
 
@@ -124,19 +133,21 @@ struct Runs {
     uint32_t units, strands;      // of the whole outline
     uint32_t last_start;          // curve index where the last run starts
     uint32_t last_units;          // its units (written first)
+    uint32_t last_len;            // and its curves
 };
 
 // Maximal runs of knob-free curves sharing a non-zero x direction (a
 // vertical curve is a run of its own).
 __device__ Runs count_runs(const float *p, int k)
 {
-    Runs r{0u, 0u, 0u, 0u};
+    Runs r{0u, 0u, 0u, 0u, 0u};
     if (k < 2) return r;
     int dir = 0;
     uint32_t len = 0, start = 0, idx = 0;
     for (int i = 0; i < k; ++i) {
         Quad two[2];
-        const int n = split_knob(curve_at(p, i, k), two);
+        float t;
+        const int n = split_knob(curve_at(p, i, k), two, t);
         for (int h = 0; h < n; ++h, ++idx) {
             const int d = x_dir(two[h]);
             if (idx == 0) {
@@ -157,6 +168,7 @@ __device__ Runs count_runs(const float *p, int k)
     r.strands += (len + kMaxStrandCurves - 1) / kMaxStrandCurves;
     r.last_start = start;
     r.last_units = run_units(len);
+    r.last_len = len;
     return r;
 }
 
@@ -164,7 +176,7 @@ __device__ Runs count_runs(const float *p, int k)
 // units, u16 0, u32 0} and the right end point, the left end point and its
 // control, then the interior (point, control) pairs in tree order.  A chunk
 // that runs right to left is reversed first (reverseIfBackwards).
-__device__ void write_strand(const Quad *buf, int m, uint8_t *dst, const Orders &ord)
+__device__ void write_strand(const Quad *buf, int m, uint8_t *dst, const Orders &ord, uint32_t curve_base)
 {
     const bool rev = buf[0].p0.x > buf[m - 1].p1.x;
     auto at = [&](int q) -> Quad {
@@ -175,7 +187,7 @@ __device__ void write_strand(const Quad *buf, int m, uint8_t *dst, const Orders 
     const Quad first = at(0), last = at(m - 1);
     uint4 h;
     h.x = (uint32_t)(2 * (m + 1));
-    h.y = 0u;
+    h.y = curve_base;   // 0 without tracking
     h.z = __float_as_uint(last.p1.x);
     h.w = __float_as_uint(last.p1.y);
     reinterpret_cast<uint4 *>(dst)[0] = h;
@@ -186,26 +198,43 @@ __device__ void write_strand(const Quad *buf, int m, uint8_t *dst, const Orders 
     }
 }
 
-// Writes one outline's strands from heap unit `base` on.
+// Writes one outline's strands from heap unit `base` on; with tracking,
+// their curves' records from curve `cbase` on (pair_base: the outline's
+// first pair in the scene).
 __device__ void emit_outline(const float *p, int k, const Runs &r, uint8_t *heap, uint64_t base,
-                             const Orders &ord)
+                             const Orders &ord, CurveSrc *curves, uint64_t cbase, int32_t pair_base)
 {
     if (k < 2) return;
     uint64_t pos_last = base, pos_rest = base + r.last_units;
+    uint64_t cpos_last = cbase, cpos_rest = cbase + r.last_len;
     Quad buf[kMaxStrandCurves];
+    CurveSrc src[kMaxStrandCurves];
     int nb = 0, dir = 0;
     bool in_last = false;
     uint32_t idx = 0;
     auto flush = [&]() {
         if (!nb) return;
         uint64_t &pos = in_last ? pos_last : pos_rest;
-        write_strand(buf, nb, heap + 16 * pos, ord);
+        uint64_t &cpos = in_last ? cpos_last : cpos_rest;
+        write_strand(buf, nb, heap + 16 * pos, ord, curves ? (uint32_t)cpos : 0u);
+        if (curves) {
+            const bool rev = buf[0].p0.x > buf[nb - 1].p1.x;
+            for (int q = 0; q < nb; ++q) {
+                CurveSrc c = src[rev ? nb - 1 - q : q];
+                if (rev) c.flags |= kCurveReversed;
+                curves[cpos + q] = c;
+            }
+        }
         pos += (uint64_t)nb + 1;
+        cpos += (uint64_t)nb;
         nb = 0;
     };
     for (int i = 0; i < k; ++i) {
         Quad two[2];
-        const int n = split_knob(curve_at(p, i, k), two);
+        float t;
+        const int n = split_knob(curve_at(p, i, k), two, t);
+        const int j = (i + 1 == k) ? 0 : i + 1;
+        const int32_t a = 2 * (pair_base + i), b = 2 * (pair_base + j);
         for (int h = 0; h < n; ++h, ++idx) {
             const int d = x_dir(two[h]);
             if (idx == 0) {
@@ -218,7 +247,10 @@ __device__ void emit_outline(const float *p, int k, const Runs &r, uint8_t *heap
             } else if (nb == kMaxStrandCurves) {
                 flush();
             }
-            buf[nb++] = two[h];
+            buf[nb] = two[h];
+            if (curves)
+                src[nb] = CurveSrc{a, a + 1, b, t, n == 2 ? (h == 0 ? kCurveLowerHalf : kCurveUpperHalf) : 0u};
+            ++nb;
         }
     }
     flush();
@@ -292,13 +324,15 @@ __global__ void __launch_bounds__(kThreads) ser_emit(Args a, Orders ord)
         if (!a.keep[s]) continue;
         const uint64_t idx = a.kidx[s];
         uint64_t base = a.ustart[s];
+        uint64_t cbase = a.curves ? a.cstart[s] : 0;
         const int o0 = a.soff[s], o1 = a.soff[s + 1];
         for (int o = o0; o < o1; ++o) {
             const int e0 = a.ooff[o], e1 = a.ooff[o + 1];
             const float *p = a.pairs + 4 * (size_t)e0;
             const Runs rr = count_runs(p, e1 - e0);
-            emit_outline(p, e1 - e0, rr, a.heap, base, ord);
+            emit_outline(p, e1 - e0, rr, a.heap, base, ord, a.curves, cbase, e0);
             base += rr.units;
+            cbase += rr.units - rr.strands;
         }
         ShapeRec rec;
         rec.tag = a.tags[s];
@@ -355,6 +389,15 @@ struct Units {
     __device__ int count() const { return n; }
     __device__ uint64_t operator()(int i) const { return units[i]; }
 };
+// curves of a shape: each strand of m curves takes m + 1 units
+struct Curves {
+    const uint64_t *units;
+    const uint32_t *strands;
+    int n;
+    __device__ void load() {}
+    __device__ int count() const { return n; }
+    __device__ uint64_t operator()(int i) const { return units[i] - strands[i]; }
+};
 
 Orders make_orders()
 {
@@ -376,6 +419,103 @@ int grid_for(int n)
 }  // namespace ser
 }  // namespace gudni
 
+namespace gudni {
+
+int check_outline_offsets(gudni_ctx *ctx, const gudni_outlines *o, int32_t &n_outlines, int32_t &n_pairs)
+{
+    const int32_t n = o->n_shapes;
+    n_outlines = n ? o->shape_outline_offsets[n] : 0;
+    if (n && o->shape_outline_offsets[0] < 0)
+        return set_error(ctx, GUDNI_ERR_ARG, "shape outline offsets must be non-decreasing from 0");
+    for (int32_t i = 0; i < n; ++i)
+        if (o->shape_outline_offsets[i] < 0 || o->shape_outline_offsets[i] > o->shape_outline_offsets[i + 1])
+            return set_error(ctx, GUDNI_ERR_ARG, "shape outline offsets must be non-decreasing from 0");
+    n_pairs = n_outlines ? o->outline_pair_offsets[n_outlines] : 0;
+    for (int32_t j = 0; j < n_outlines; ++j)
+        if (o->outline_pair_offsets[j] < 0 || o->outline_pair_offsets[j] > o->outline_pair_offsets[j + 1])
+            return set_error(ctx, GUDNI_ERR_ARG, "outline pair offsets must be non-decreasing from 0");
+    if (n_pairs && !o->pairs) return set_error(ctx, GUDNI_ERR_ARG, "outlines without pairs");
+    return GUDNI_OK;
+}
+
+int serialize_outlines(gudni_ctx *ctx, const SerIn &in, const SerOut &out, SerCounts &cnt)
+{
+    const hipStream_t st = ctx->stream;
+    const int32_t n = in.n_shapes;
+    CHECK_HIP(ctx, ctx->ser_work.ensure(16u * (size_t)n + 8u * (4 * (size_t)n + 3) + 4u * 2 * (size_t)n +
+                                        8u * (devscan::kScanBlocks + 1) + 64));
+    ser::Args a;
+    std::memset(&a, 0, sizeof(a));
+    uint64_t *partials;
+    {
+        // float4 first, then 8-byte and 4-byte arrays: every array aligned
+        uint8_t *w = (uint8_t *)ctx->ser_work.p;
+        a.box = (float4 *)w;              w += 16u * (size_t)n;
+        a.units = (uint64_t *)w;          w += 8u * (size_t)n;
+        a.kidx = (uint64_t *)w;           w += 8u * ((size_t)n + 1);
+        a.ustart = (uint64_t *)w;         w += 8u * ((size_t)n + 1);
+        a.cstart = (uint64_t *)w;         w += 8u * ((size_t)n + 1);
+        a.keep = (uint32_t *)w;           w += 4u * (size_t)n;
+        a.strands = (uint32_t *)w;        w += 4u * (size_t)n;
+        a.status = (uint32_t *)w;         w += 16;
+        w = (uint8_t *)(((uintptr_t)w + 15) & ~(uintptr_t)15);
+        partials = (uint64_t *)w;
+    }
+    a.tags = in.tags;
+    a.soff = in.soff;
+    a.ooff = in.ooff;
+    a.pairs = in.pairs;
+    a.n_shapes = n;
+    a.fw = (float)in.width;
+    a.fh = (float)in.height;
+    a.subs = in.subs;
+    a.n_subs = in.n_subs;
+    a.n_uses = in.n_uses;
+    a.n_grads = in.n_grads;
+    hipLaunchKernelGGL(ser::ser_reset, dim3(1), dim3(64), 0, st, a.status);
+    hipLaunchKernelGGL(ser::ser_count, dim3(ser::grid_for(n)), dim3(ser::kThreads), 0, st, a);
+    devscan::excl_scan(ser::KeepFlags{a.keep, n}, partials, a.kidx, st);
+    devscan::excl_scan(ser::Units{a.units, n}, partials, a.ustart, st);
+    if (out.curves) devscan::excl_scan(ser::Curves{a.units, a.strands, n}, partials, a.cstart, st);
+    CHECK_HIP(ctx, hipGetLastError());
+    uint64_t totals[3] = {0, 0, 0};
+    uint32_t status = 0;
+    CHECK_HIP(ctx, hipMemcpyAsync(&totals[0], a.kidx + n, 8, hipMemcpyDeviceToHost, st));
+    CHECK_HIP(ctx, hipMemcpyAsync(&totals[1], a.ustart + n, 8, hipMemcpyDeviceToHost, st));
+    if (out.curves) CHECK_HIP(ctx, hipMemcpyAsync(&totals[2], a.cstart + n, 8, hipMemcpyDeviceToHost, st));
+    CHECK_HIP(ctx, hipMemcpyAsync(&status, a.status, 4, hipMemcpyDeviceToHost, st));
+    CHECK_HIP(ctx, hipStreamSynchronize(st));
+    if (status & ser::kErrSubstance)
+        return set_error(ctx, GUDNI_ERR_ARG, "shape substance id outside the substance table");
+    if (status & ser::kErrGradient)
+        return set_error(ctx, GUDNI_ERR_ARG, "gradient substance refers to a missing gradient");
+    if (status & ser::kErrPicture)
+        return set_error(ctx, GUDNI_ERR_ARG, "picture substance refers to a missing usage");
+    const uint64_t n_kept = totals[0], units = totals[1];
+    if (units >= (1ull << 32) || totals[2] >= (1ull << 31))
+        return set_error(ctx, GUDNI_ERR_ARG, "serialised geometry above 64 GiB or 2^31 curves");
+    CHECK_HIP(ctx, out.geo->ensure(16u * (size_t)units));
+    CHECK_HIP(ctx, out.shapes->ensure(16u * (size_t)n_kept));
+    CHECK_HIP(ctx, out.bbox->ensure(16u * (size_t)n_kept));
+    CHECK_HIP(ctx, out.bounds->ensure(16u * (size_t)n_kept));
+    if (out.curves) CHECK_HIP(ctx, out.curves->ensure(sizeof(CurveSrc) * (size_t)totals[2]));
+    a.heap = (uint8_t *)out.geo->p;
+    a.shapes = (ShapeRec *)out.shapes->p;
+    a.bbox = (float4 *)out.bbox->p;
+    a.bounds = (float4 *)out.bounds->p;
+    a.curves = out.curves ? (CurveSrc *)out.curves->p : nullptr;
+    hipLaunchKernelGGL(ser::ser_emit, dim3(ser::grid_for(n)), dim3(ser::kThreads), 0, st, a, ser::make_orders());
+    hipLaunchKernelGGL(ser::ser_extents, dim3(ser::grid_for((int)n_kept)), dim3(ser::kThreads), 0, st, a,
+                       (const uint64_t *)(a.kidx + n));
+    CHECK_HIP(ctx, hipGetLastError());
+    cnt.n_kept = n_kept;
+    cnt.geo_bytes = 16u * units;
+    cnt.n_curves = totals[2];
+    return GUDNI_OK;
+}
+
+}  // namespace gudni
+
 using namespace gudni;
 
 extern "C" {
@@ -392,22 +532,13 @@ int gudni_scene_upload_outlines(gudni_ctx *ctx, const gudni_outlines *o, const g
     if (std::max(sc->width, sc->height) > (1 << 15))
         return set_error(ctx, GUDNI_ERR_ARG, "gudni_scene_upload_outlines: canvas side above 32768");
     const int32_t n = o->n_shapes;
-    // offsets: non-decreasing, inside their arrays (the kernels index with them)
-    const int32_t n_outlines = n ? o->shape_outline_offsets[n] : 0;
-    for (int32_t i = 0; i < n; ++i)
-        if (o->shape_outline_offsets[i] < 0 || o->shape_outline_offsets[i] > o->shape_outline_offsets[i + 1])
-            return set_error(ctx, GUDNI_ERR_ARG, "gudni_scene_upload_outlines: shape outline offsets "
-                                                 "must be non-decreasing from 0");
-    const int32_t n_pairs = n_outlines ? o->outline_pair_offsets[n_outlines] : 0;
-    for (int32_t j = 0; j < n_outlines; ++j)
-        if (o->outline_pair_offsets[j] < 0 || o->outline_pair_offsets[j] > o->outline_pair_offsets[j + 1])
-            return set_error(ctx, GUDNI_ERR_ARG, "gudni_scene_upload_outlines: outline pair offsets "
-                                                 "must be non-decreasing from 0");
-    if (n_pairs && !o->pairs) return set_error(ctx, GUDNI_ERR_ARG, "gudni_scene_upload_outlines: no pairs");
+    int32_t n_outlines = 0, n_pairs = 0;
+    int rc = check_outline_offsets(ctx, o, n_outlines, n_pairs);
+    if (rc) return rc;
     ctx->scene_ready = false;
-    int rc = gudni_upload_common(ctx, nullptr, 0, sc->substances, sc->n_substances, sc->pict_data,
-                                 sc->pict_bytes, sc->pict_uses, sc->n_pict_uses, sc->random_field,
-                                 sc->n_random);
+    rc = gudni_upload_common(ctx, nullptr, 0, sc->substances, sc->n_substances, sc->pict_data,
+                             sc->pict_bytes, sc->pict_uses, sc->n_pict_uses, sc->random_field,
+                             sc->n_random);
     if (rc) return rc;
     rc = gudni_upload_gradients(ctx, sc->gradients, sc->n_gradients);
     if (rc) return rc;
@@ -420,75 +551,31 @@ int gudni_scene_upload_outlines(gudni_ctx *ctx, const gudni_outlines *o, const g
     CHECK_HIP(ctx, upload(ctx, ctx->ser_soff, o->shape_outline_offsets, 4u * ((size_t)n + 1)));
     CHECK_HIP(ctx, upload(ctx, ctx->ser_ooff, o->outline_pair_offsets, 4u * ((size_t)n_outlines + 1)));
     CHECK_HIP(ctx, upload(ctx, ctx->ser_pairs, o->pairs, 16u * (size_t)n_pairs));
-    CHECK_HIP(ctx, ctx->ser_work.ensure(8u * (3 * (size_t)n + 2) + 4u * 2 * (size_t)n + 16u * (size_t)n +
-                                        8u * (devscan::kScanBlocks + 1) + 64));
-    ser::Args a;
-    std::memset(&a, 0, sizeof(a));
-    {
-        // float4 first, then 8-byte and 4-byte arrays: every array aligned
-        uint8_t *w = (uint8_t *)ctx->ser_work.p;
-        a.box = (float4 *)w;              w += 16u * (size_t)n;
-        a.units = (uint64_t *)w;          w += 8u * (size_t)n;
-        a.kidx = (uint64_t *)w;           w += 8u * ((size_t)n + 1);
-        a.ustart = (uint64_t *)w;         w += 8u * ((size_t)n + 1);
-        a.keep = (uint32_t *)w;           w += 4u * (size_t)n;
-        a.strands = (uint32_t *)w;        w += 4u * (size_t)n;
-        a.status = (uint32_t *)w;         w += 16;
-        w = (uint8_t *)(((uintptr_t)w + 15) & ~(uintptr_t)15);
-        ctx->ser_partials = (uint64_t *)w;
-    }
-    a.tags = (const uint64_t *)ctx->ser_tags.p;
-    a.soff = (const int32_t *)ctx->ser_soff.p;
-    a.ooff = (const int32_t *)ctx->ser_ooff.p;
-    a.pairs = (const float *)ctx->ser_pairs.p;
-    a.n_shapes = n;
-    a.fw = (float)sc->width;
-    a.fh = (float)sc->height;
-    a.subs = (const float4 *)ctx->subs.p;
-    a.n_subs = ctx->n_substances;
-    a.n_uses = ctx->n_uses;
-    a.n_grads = ctx->n_grads;
+    SerIn in;
+    in.tags = (const uint64_t *)ctx->ser_tags.p;
+    in.soff = (const int32_t *)ctx->ser_soff.p;
+    in.ooff = (const int32_t *)ctx->ser_ooff.p;
+    in.pairs = (const float *)ctx->ser_pairs.p;
+    in.n_shapes = n;
+    in.width = sc->width;
+    in.height = sc->height;
+    in.subs = (const float4 *)ctx->subs.p;
+    in.n_subs = ctx->n_substances;
+    in.n_uses = ctx->n_uses;
+    in.n_grads = ctx->n_grads;
+    SerOut out{&ctx->geo, &ctx->scene_shapes, &ctx->scene_bbox, &ctx->scene_bounds, nullptr};
+    SerCounts cnt;
     if (timed) CHECK_HIP(ctx, hipEventRecord(ctx->ser_ev[0], st));
-    hipLaunchKernelGGL(ser::ser_reset, dim3(1), dim3(64), 0, st, a.status);
-    hipLaunchKernelGGL(ser::ser_count, dim3(ser::grid_for(n)), dim3(ser::kThreads), 0, st, a);
-    devscan::excl_scan(ser::KeepFlags{a.keep, n}, ctx->ser_partials, a.kidx, st);
-    devscan::excl_scan(ser::Units{a.units, n}, ctx->ser_partials, a.ustart, st);
-    CHECK_HIP(ctx, hipGetLastError());
-    uint64_t totals[2] = {0, 0};
-    uint32_t status = 0;
-    CHECK_HIP(ctx, hipMemcpyAsync(&totals[0], a.kidx + n, 8, hipMemcpyDeviceToHost, st));
-    CHECK_HIP(ctx, hipMemcpyAsync(&totals[1], a.ustart + n, 8, hipMemcpyDeviceToHost, st));
-    CHECK_HIP(ctx, hipMemcpyAsync(&status, a.status, 4, hipMemcpyDeviceToHost, st));
-    CHECK_HIP(ctx, hipStreamSynchronize(st));
-    if (status & ser::kErrSubstance)
-        return set_error(ctx, GUDNI_ERR_ARG, "shape substance id outside the substance table");
-    if (status & ser::kErrGradient)
-        return set_error(ctx, GUDNI_ERR_ARG, "gradient substance refers to a missing gradient");
-    if (status & ser::kErrPicture)
-        return set_error(ctx, GUDNI_ERR_ARG, "picture substance refers to a missing usage");
-    const uint64_t n_kept = totals[0], units = totals[1];
-    if (units >= (1ull << 32))
-        return set_error(ctx, GUDNI_ERR_ARG, "gudni_scene_upload_outlines: geometry heap above 64 GiB");
-    CHECK_HIP(ctx, ctx->geo.ensure(16u * (size_t)units));
-    CHECK_HIP(ctx, ctx->scene_shapes.ensure(16u * (size_t)n_kept));
-    CHECK_HIP(ctx, ctx->scene_bbox.ensure(16u * (size_t)n_kept));
-    CHECK_HIP(ctx, ctx->scene_bounds.ensure(16u * (size_t)n_kept));
-    a.heap = (uint8_t *)ctx->geo.p;
-    a.shapes = (ShapeRec *)ctx->scene_shapes.p;
-    a.bbox = (float4 *)ctx->scene_bbox.p;
-    a.bounds = (float4 *)ctx->scene_bounds.p;
-    hipLaunchKernelGGL(ser::ser_emit, dim3(ser::grid_for(n)), dim3(ser::kThreads), 0, st, a, ser::make_orders());
-    hipLaunchKernelGGL(ser::ser_extents, dim3(ser::grid_for((int)n_kept)), dim3(ser::kThreads), 0, st, a,
-                       (const uint64_t *)(a.kidx + n));
-    CHECK_HIP(ctx, hipGetLastError());
+    rc = serialize_outlines(ctx, in, out, cnt);
+    if (rc) return rc;
     if (timed) CHECK_HIP(ctx, hipEventRecord(ctx->ser_ev[1], st));
     CHECK_HIP(ctx, hipStreamSynchronize(st));
     if (timed) CHECK_HIP(ctx, hipEventElapsedTime(&ctx->last_ser_ms, ctx->ser_ev[0], ctx->ser_ev[1]));
     // the heap lives on the device only: host-side record checks of later
     // host-built jobs have nothing to check against
     ctx->geo_host.clear();
-    ctx->scene_geo_bytes = 16u * (size_t)units;
-    ctx->scene_n_shapes = (int32_t)n_kept;
+    ctx->scene_geo_bytes = (size_t)cnt.geo_bytes;
+    ctx->scene_n_shapes = (int32_t)cnt.n_kept;
     ctx->scene_w = sc->width;
     ctx->scene_h = sc->height;
     std::memcpy(ctx->scene_bg, sc->background, sizeof(ctx->scene_bg));

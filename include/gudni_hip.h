@@ -306,10 +306,19 @@ typedef struct gudni_adjoint_batch gudni_adjoint_batch;
 /* Serialises, bins and uploads n_scenes scenes with their targets. */
 int  gudni_adjoint_upload(gudni_ctx *ctx, int32_t n_scenes, const gudni_adjoint_scene *scenes,
                           gudni_adjoint_batch **out);
-/* New parameters for one scene (pairs and/or rgba may be NULL = unchanged):
- * re-serialises and re-bins on the host and re-uploads. */
+/* New parameters for one scene (pairs and/or rgba may be NULL = unchanged),
+ * from host memory: copied up, then the scene is re-serialised (with the
+ * curve provenance the point gradients need) and re-binned on the device,
+ * and the batch's tables are reassembled; synchronous.  The first update of
+ * a batch rebuilds every scene this way; the result is identical to the
+ * host build gudni_adjoint_upload makes.  If an update fails the batch
+ * refuses to run until uploaded again. */
 int  gudni_adjoint_update(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t scene, const float *pairs,
                           const float *rgba);
+/* The same with the new parameters in device memory (e.g. a training loop's
+ * tensors): no host round trip of the parameters. */
+int  gudni_adjoint_update_device(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t scene,
+                                 const float *pairs, const float *rgba);
 /* Forward + backward of every scene in one launch (stream-ordered). */
 int  gudni_adjoint_run(gudni_ctx *ctx, gudni_adjoint_batch *a);
 /* The same for one scene alone. */
@@ -317,6 +326,13 @@ int  gudni_adjoint_run_scene(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t sce
 /* Waits and copies one scene's loss, gradients and image (any may be NULL). */
 int  gudni_adjoint_result(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t scene, double *loss,
                           float *d_pairs, float *d_rgba, float *image);
+/* The same into device memory, stream-ordered, without waiting; kernel
+ * errors (queue overflow, section guard) surface at the next
+ * gudni_adjoint_result or gudni_adjoint_check. */
+int  gudni_adjoint_result_device(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t scene, double *loss,
+                                 float *d_pairs, float *d_rgba, float *image);
+/* Waits for the stream and reports the last run's kernel errors. */
+int  gudni_adjoint_check(gudni_ctx *ctx, gudni_adjoint_batch *a);
 void gudni_adjoint_free(gudni_adjoint_batch *a);
 /* One scene, synchronously: upload, run, result, free. */
 int  gudni_adjoint(gudni_ctx *ctx, const gudni_adjoint_scene *scene, double *loss, float *d_pairs,

@@ -111,3 +111,60 @@ def test_torch_autograd_wrapper_descends(rasterizer):
         assert torch.isfinite(pairs.grad).all() and torch.isfinite(rgba.grad).all()
         opt.step()
     assert losses[-1] < losses[0]
+
+
+def _same_result(a, b, what):
+    """Equal up to the order of atomic accumulation (the image bit for bit)."""
+    assert abs(a[0] - b[0]) <= 1e-6 * max(1.0, abs(a[0])), f"{what}: loss {a[0]} vs {b[0]}"
+    np.testing.assert_allclose(a[1], b[1], rtol=1e-4, atol=1e-6 * max(1e-30, np.abs(a[1]).max()), err_msg=what)
+    np.testing.assert_allclose(a[2], b[2], rtol=1e-4, atol=1e-6 * max(1e-30, np.abs(a[2]).max()), err_msg=what)
+    np.testing.assert_array_equal(a[3], b[3], err_msg=what)
+
+
+def test_device_update_equals_host_build(rasterizer):
+    """Parameter updates re-serialise and re-bin on the device (with the
+    curve provenance the point gradients need).  Updating a two-scene batch
+    to its own parameters reproduces the host build's results; updating one
+    scene to new parameters, from host memory or from a device tensor, gives
+    the results of a fresh upload (host build) of those parameters."""
+    import torch
+    items = [A.case(192, 90, 81), A.case(256, 160, 82)]
+    adj = G.Adjoint(rasterizer, items)
+    fresh = None
+    try:
+        say("host build")
+        adj.run()
+        base = [adj.result(i, image=True) for i in range(2)]
+        say("device rebuild, same parameters")
+        adj.update(1, pairs=items[1][0].pairs)
+        adj.run()
+        for i in range(2):
+            _same_result(base[i], adj.result(i, image=True), f"scene {i} after a same-parameter update")
+        moved = A.perturbed(items[1][0], 99, point_sigma=1.5, color_sigma=0.0)
+        say("device rebuild, new parameters (host pointer)")
+        adj.update(1, pairs=moved.pairs)
+        adj.run()
+        got_host = adj.result(1, image=True)
+        say("fresh upload of the new parameters (host build)")
+        fresh = G.Adjoint(rasterizer, [items[0], (moved, items[1][1])])
+        fresh.run()
+        want = fresh.result(1, image=True)
+        _same_result(want, got_host, "update from host memory")
+        _same_result(base[0], fresh.result(0, image=True), "untouched scene")
+        say("device rebuild, new parameters (device pointer)")
+        adj.update(1, pairs=items[1][0].pairs)
+        t = torch.tensor(moved.pairs.reshape(-1), dtype=torch.float32, device="cuda")
+        adj.update_device(1, pairs_ptr=t.data_ptr())
+        adj.run()
+        dp = torch.zeros(moved.pairs.size, dtype=torch.float32, device="cuda")
+        loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+        adj.result_device(1, loss_ptr=loss.data_ptr(), d_pairs_ptr=dp.data_ptr())
+        adj.check()
+        got_dev = adj.result(1, image=True)
+        _same_result(want, got_dev, "update from device memory")
+        assert abs(float(loss.item()) - got_dev[0]) == 0.0
+        np.testing.assert_array_equal(dp.cpu().numpy().reshape(got_dev[1].shape), got_dev[1])
+    finally:
+        adj.close()
+        if fresh is not None:
+            fresh.close()
