@@ -1230,7 +1230,7 @@ int device_update(gudni_ctx *ctx, gudni_adjoint_batch *a, int scene)
 {
     CHECK_HIP(ctx, hipSetDevice(ctx->device));
     int rc = GUDNI_OK;
-    if (a->dev.empty()) {
+    if (a->dev.empty() || scene < 0) {
         a->dev.resize(a->scenes.size());
         for (int s = 0; s < (int)a->scenes.size() && rc == GUDNI_OK; ++s) rc = device_rebuild_scene(ctx, a, s);
     } else {
@@ -1387,19 +1387,24 @@ int gudni_adjoint_upload(gudni_ctx *ctx, int32_t n_scenes, const gudni_adjoint_s
 int gudni_adjoint_update(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t scene, const float *pairs,
                          const float *rgba)
 {
-    if (!ctx || !a || scene < 0 || scene >= (int32_t)a->scenes.size()) return GUDNI_ERR_ARG;
-    gudni_adjoint_batch::Scene &sc = a->scenes[scene];
+    if (!ctx || !a || scene < -1 || scene >= (int32_t)a->scenes.size()) return GUDNI_ERR_ARG;
     CHECK_HIP(ctx, hipSetDevice(ctx->device));
     CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (pairs) {
-        std::memcpy(sc.pairs.data(), pairs, 4 * sc.pairs.size());
-        CHECK_HIP(ctx, hipMemcpyAsync((float *)a->points.p + 2 * a->points_off[scene], sc.pairs.data(),
-                                      4 * sc.pairs.size(), hipMemcpyHostToDevice, ctx->stream));
-    }
-    if (rgba) {
-        std::memcpy(sc.subs.data(), rgba, 16 * sc.subs.size());
-        CHECK_HIP(ctx, hipMemcpyAsync((float4 *)a->subs.p + a->subs_off[scene], sc.subs.data(),
-                                      16 * sc.subs.size(), hipMemcpyHostToDevice, ctx->stream));
+    const int s0 = scene < 0 ? 0 : scene, s1 = scene < 0 ? (int)a->scenes.size() : scene + 1;
+    for (int s = s0; s < s1; ++s) {   // with scene -1 the inputs are every scene's, concatenated
+        gudni_adjoint_batch::Scene &sc = a->scenes[s];
+        if (pairs) {
+            const float *src = pairs + (scene < 0 ? 2 * a->points_off[s] : 0);
+            std::memcpy(sc.pairs.data(), src, 4 * sc.pairs.size());
+            CHECK_HIP(ctx, hipMemcpyAsync((float *)a->points.p + 2 * a->points_off[s], sc.pairs.data(),
+                                          4 * sc.pairs.size(), hipMemcpyHostToDevice, ctx->stream));
+        }
+        if (rgba) {
+            const float *src = rgba + (scene < 0 ? 4 * a->subs_off[s] : 0);
+            std::memcpy(sc.subs.data(), src, 16 * sc.subs.size());
+            CHECK_HIP(ctx, hipMemcpyAsync((float4 *)a->subs.p + a->subs_off[s], sc.subs.data(),
+                                          16 * sc.subs.size(), hipMemcpyHostToDevice, ctx->stream));
+        }
     }
     if (pairs) return device_update(ctx, a, scene);
     CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1409,15 +1414,23 @@ int gudni_adjoint_update(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t scene, 
 int gudni_adjoint_update_device(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t scene, const float *pairs,
                                 const float *rgba)
 {
-    if (!ctx || !a || scene < 0 || scene >= (int32_t)a->scenes.size()) return GUDNI_ERR_ARG;
-    const gudni_adjoint_batch::Scene &sc = a->scenes[scene];
+    if (!ctx || !a || scene < -1 || scene >= (int32_t)a->scenes.size()) return GUDNI_ERR_ARG;
     CHECK_HIP(ctx, hipSetDevice(ctx->device));
-    if (pairs && !sc.pairs.empty())
-        CHECK_HIP(ctx, hipMemcpyAsync((float *)a->points.p + 2 * a->points_off[scene], pairs,
-                                      4 * sc.pairs.size(), hipMemcpyDeviceToDevice, ctx->stream));
-    if (rgba && !sc.subs.empty())
-        CHECK_HIP(ctx, hipMemcpyAsync((float4 *)a->subs.p + a->subs_off[scene], rgba, 16 * sc.subs.size(),
-                                      hipMemcpyDeviceToDevice, ctx->stream));
+    if (scene < 0) {   // every scene's parameters, concatenated in batch order
+        if (pairs && a->n_points)
+            CHECK_HIP(ctx, hipMemcpyAsync(a->points.p, pairs, 8 * a->n_points, hipMemcpyDeviceToDevice,
+                                          ctx->stream));
+        if (rgba && a->n_subs)
+            CHECK_HIP(ctx, hipMemcpyAsync(a->subs.p, rgba, 16 * a->n_subs, hipMemcpyDeviceToDevice, ctx->stream));
+    } else {
+        const gudni_adjoint_batch::Scene &sc = a->scenes[scene];
+        if (pairs && !sc.pairs.empty())
+            CHECK_HIP(ctx, hipMemcpyAsync((float *)a->points.p + 2 * a->points_off[scene], pairs,
+                                          4 * sc.pairs.size(), hipMemcpyDeviceToDevice, ctx->stream));
+        if (rgba && !sc.subs.empty())
+            CHECK_HIP(ctx, hipMemcpyAsync((float4 *)a->subs.p + a->subs_off[scene], rgba, 16 * sc.subs.size(),
+                                          hipMemcpyDeviceToDevice, ctx->stream));
+    }
     if (pairs) return device_update(ctx, a, scene);
     CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return GUDNI_OK;
@@ -1470,7 +1483,20 @@ int gudni_adjoint_result(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t scene, 
 int gudni_adjoint_result_device(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t scene, double *loss,
                                 float *d_pairs, float *d_rgba, float *image)
 {
-    if (!ctx || !a || scene < 0 || scene >= (int32_t)a->scenes.size()) return GUDNI_ERR_ARG;
+    if (!ctx || !a || scene < -1 || scene >= (int32_t)a->scenes.size()) return GUDNI_ERR_ARG;
+    if (scene < 0) {   // every scene: losses [n_scenes], gradients and images concatenated
+        const hipStream_t st = ctx->stream;
+        CHECK_HIP(ctx, hipSetDevice(ctx->device));
+        if (loss)
+            CHECK_HIP(ctx, hipMemcpyAsync(loss, a->loss.p, 8 * a->scenes.size(), hipMemcpyDeviceToDevice, st));
+        if (d_pairs && a->n_points)
+            CHECK_HIP(ctx, hipMemcpyAsync(d_pairs, a->d_points.p, 8 * a->n_points, hipMemcpyDeviceToDevice, st));
+        if (d_rgba && a->n_subs)
+            CHECK_HIP(ctx, hipMemcpyAsync(d_rgba, a->d_rgba.p, 16 * a->n_subs, hipMemcpyDeviceToDevice, st));
+        if (image && a->n_pix)
+            CHECK_HIP(ctx, hipMemcpyAsync(image, a->image.p, 16 * a->n_pix, hipMemcpyDeviceToDevice, st));
+        return GUDNI_OK;
+    }
     const gudni_adjoint_batch::Scene &sc = a->scenes[scene];
     const hipStream_t st = ctx->stream;
     CHECK_HIP(ctx, hipSetDevice(ctx->device));

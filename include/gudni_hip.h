@@ -316,7 +316,9 @@ int  gudni_adjoint_upload(gudni_ctx *ctx, int32_t n_scenes, const gudni_adjoint_
 int  gudni_adjoint_update(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t scene, const float *pairs,
                           const float *rgba);
 /* The same with the new parameters in device memory (e.g. a training loop's
- * tensors): no host round trip of the parameters. */
+ * tensors): no host round trip of the parameters.  For both update calls,
+ * scene -1 updates every scene at once from the concatenation of all
+ * scenes' pairs (and RGBA) in batch order, reassembling the tables once. */
 int  gudni_adjoint_update_device(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t scene,
                                  const float *pairs, const float *rgba);
 /* Forward + backward of every scene in one launch (stream-ordered). */
@@ -326,7 +328,8 @@ int  gudni_adjoint_run_scene(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t sce
 /* Waits and copies one scene's loss, gradients and image (any may be NULL). */
 int  gudni_adjoint_result(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t scene, double *loss,
                           float *d_pairs, float *d_rgba, float *image);
-/* The same into device memory, stream-ordered, without waiting; kernel
+/* The same into device memory, stream-ordered, without waiting (scene -1:
+ * every scene's losses, gradients and images, concatenated); kernel
  * errors (queue overflow, section guard) surface at the next
  * gudni_adjoint_result or gudni_adjoint_check. */
 int  gudni_adjoint_result_device(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t scene, double *loss,

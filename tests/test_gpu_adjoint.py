@@ -168,3 +168,31 @@ def test_device_update_equals_host_build(rasterizer):
         adj.close()
         if fresh is not None:
             fresh.close()
+
+
+def test_whole_batch_device_update(rasterizer):
+    """Scene -1 updates every scene from the concatenated parameters and
+    reads every scene's gradients back concatenated."""
+    import torch
+    items = [A.case(160, 70, 91), A.case(128, 50, 92)]
+    moved = [A.perturbed(sc, 7 + i, point_sigma=1.0, color_sigma=0.0) for i, (sc, _) in enumerate(items)]
+    adj = G.Adjoint(rasterizer, items)
+    fresh = G.Adjoint(rasterizer, [(m, t) for m, (_, t) in zip(moved, items)])
+    try:
+        t = torch.tensor(np.concatenate([m.pairs.reshape(-1) for m in moved]), dtype=torch.float32, device="cuda")
+        adj.update_device(-1, pairs_ptr=t.data_ptr())
+        adj.run()
+        fresh.run()
+        dp = torch.zeros_like(t)
+        adj.result_device(-1, d_pairs_ptr=dp.data_ptr())
+        adj.check()
+        off = 0
+        for i, m in enumerate(moved):
+            got, want = adj.result(i, image=True), fresh.result(i, image=True)
+            _same_result(want, got, f"scene {i}")
+            n = m.pairs.size
+            np.testing.assert_array_equal(dp[off:off + n].cpu().numpy(), got[1].reshape(-1))
+            off += n
+    finally:
+        adj.close()
+        fresh.close()
