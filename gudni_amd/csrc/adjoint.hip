@@ -1229,6 +1229,8 @@ int assemble(gudni_ctx *ctx, gudni_adjoint_batch *a)
 int device_update(gudni_ctx *ctx, gudni_adjoint_batch *a, int scene)
 {
     CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    // a launch may still read the per-scene buffers the rebuild can regrow
+    CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     int rc = GUDNI_OK;
     if (a->dev.empty() || scene < 0) {
         a->dev.resize(a->scenes.size());
