@@ -444,7 +444,24 @@ def batched_leg(spec, steps, log_fn):
     b.check()
     seq = np.array([evs[i].elapsed_time(evs[i + 1]) for i in range(len(scenes))])
     seq_lat = np.cumsum(seq)
+    sample = [0, len(scenes) // 2, len(scenes) - 1]
+    want = {i: b.frame(i) for i in sample}
     b.close()
+    # the same batch serialised and binned on the device from the outlines
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    bd = G.Batch.from_outlines(r, scenes, chosen, radices)
+    t_dev = time.perf_counter() - t
+    bd.render(use_graph=True)
+    bd.check()
+    same = all(np.array_equal(bd.frame(i), want[i]) for i in sample)
+    start.record(stream)
+    for _ in range(steps):
+        bd.render(use_graph=True)
+    stop.record(stream)
+    bd.check()
+    ms_dev = start.elapsed_time(stop) / steps
+    bd.close()
     r.close()
     return {
         "scenes": len(scenes), "pixels": pixels, "curves": curves,
@@ -459,6 +476,8 @@ def batched_leg(spec, steps, log_fn):
         "speedup_vs_sequential": float(seq.sum() / ms),
         "tuned_by_group": table, "graph": True,
         "host_s": {"generate": t_gen, "autotune": t_tune, "serialize_and_bin": t_prep, "upload": t_up},
+        "device_built": {"build_s": t_dev, "ms_per_batch": ms_dev, "frames_equal_host_built": same,
+                         "sampled_scenes": sample},
     }
 
 

@@ -367,10 +367,48 @@ class Batch:
 
     def __init__(self, rasterizer: Rasterizer, items, rnd: np.ndarray | None = None):
         """items: list of (scene, Geometry, RasterJobs)."""
+        self._setup(rasterizer, [s for s, _, _ in items], rnd)
+        arr = self._scene_array(items)
+        self.r._check(self.r.L.gudni_batch_upload(self.r.ctx, len(items), arr, _ptr(self._rnd),
+                                                  self._rnd.size, C.byref(self.handle)))
+
+    @classmethod
+    def from_outlines(cls, rasterizer: Rasterizer, scenes, tile_sizes=None, radix_bits=None,
+                      rnd: np.ndarray | None = None):
+        """The batch built on the device from each scene's outlines: serialised
+        and binned per scene with its own tile size and sort radix (lists, or
+        None for the spec's)."""
+        self = cls.__new__(cls)
+        self._setup(rasterizer, scenes, rnd)
+        arr = self._scene_array([(s, None, None) for s in scenes])
+        outl = (_lib.Outlines * len(scenes))()
+        for k, sc in enumerate(scenes):
+            arrs = [np.ascontiguousarray(sc.tags, np.uint64),
+                    np.ascontiguousarray(sc.shape_outline_offsets, np.int32),
+                    np.ascontiguousarray(sc.outline_pair_offsets, np.int32),
+                    np.ascontiguousarray(sc.pairs, np.float32)]
+            self._keep.append(arrs)
+            tags, soff, ooff, pairs = arrs
+            outl[k] = _lib.Outlines(sc.n_shapes, tags.ctypes.data, soff.ctypes.data, ooff.ctypes.data,
+                                    pairs.ctypes.data if pairs.size else None)
+        ts = None if tile_sizes is None else np.ascontiguousarray(tile_sizes, np.int32)
+        rb = None if radix_bits is None else np.ascontiguousarray(radix_bits, np.int32)
+        for a in (ts, rb):
+            if a is not None and a.shape != (len(scenes),):
+                raise ValueError("one tile size and one radix per scene")
+        self._keep += [ts, rb]
+        self.r._check(self.r.L.gudni_batch_upload_outlines(self.r.ctx, len(scenes), outl, arr, _ptr(ts), _ptr(rb),
+                                                           _ptr(self._rnd), self._rnd.size, C.byref(self.handle)))
+        return self
+
+    def _setup(self, rasterizer, scenes, rnd):
         self.r = rasterizer
-        L = rasterizer.L
-        rnd = random_field(rasterizer.spec.random_field_size) if rnd is None else rnd
-        self._keep = [rnd]
+        self._rnd = random_field(rasterizer.spec.random_field_size) if rnd is None else rnd
+        self._keep = [self._rnd]
+        self.sizes = [(s.width, s.height) for s in scenes]
+        self.handle = C.c_void_p()
+
+    def _scene_array(self, items):
         arr = (_lib.BatchScene * len(items))()
         for k, (scene, geo, jobs) in enumerate(items):
             subs = np.ascontiguousarray(scene.substances, np.float32)
@@ -379,24 +417,22 @@ class Batch:
             grads = _gradients(scene)
             self._keep += [geo, jobs, subs, pict, uses, grads]
             a = arr[k]
-            a.geo = geo.geo.ctypes.data if geo.geo.size else None
-            a.geo_bytes = geo.geo.size
+            if geo is not None:
+                a.geo = geo.geo.ctypes.data if geo.geo.size else None
+                a.geo_bytes = geo.geo.size
+                a.jobs = jobs.handle
             a.substances = subs.ctypes.data
             a.n_substances = subs.shape[0]
             a.pict_data = pict.ctypes.data if pict.size else None
             a.pict_bytes = pict.size
             a.pict_uses = uses.ctypes.data if uses.size else None
             a.n_pict_uses = uses.shape[0]
-            a.jobs = jobs.handle
             a.gradients = grads.ctypes.data if len(grads) else None
             a.n_gradients = len(grads)
             a.width, a.height = scene.width, scene.height
             for c in range(4):
                 a.background[c] = float(scene.background[c])
-        self.sizes = [(s.width, s.height) for s, _, _ in items]
-        self.handle = C.c_void_p()
-        rasterizer._check(L.gudni_batch_upload(rasterizer.ctx, len(items), arr, _ptr(rnd), rnd.size,
-                                               C.byref(self.handle)))
+        return arr
 
     def render(self, use_graph: bool = False):
         self.r._check(self.r.L.gudni_batch_render(self.r.ctx, self.handle, 1 if use_graph else 0))

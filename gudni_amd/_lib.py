@@ -103,6 +103,9 @@ SIGNATURES = {
                                  C.c_int32, C.c_int32, C.c_int32, C.POINTER(Dump)]),
     "gudni_batch_upload": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(BatchScene), C.c_void_p,
                                      C.c_int32, C.POINTER(C.c_void_p)]),
+    "gudni_batch_upload_outlines": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Outlines), C.POINTER(BatchScene),
+                                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                              C.POINTER(C.c_void_p)]),
     "gudni_batch_render": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "gudni_batch_render_scene": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
     "gudni_batch_check": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -146,6 +146,7 @@ struct BinIn {
     const ShapeRec *shapes;
     const float4 *bbox, *bounds;
     int32_t n_shapes, width, height;
+    int32_t max_tile_size = 0;     // 0: the spec's
 };
 struct BinOut {
     DevBuf *tiles, *tile_job, *refs, *ref_bounds, *work;

@@ -449,6 +449,7 @@ int gudni_jobs_build(const gudni_spec *spec, const gudni_geometry *g, int32_t cw
         return GUDNI_ERR_ARG;
     }
     if (spec->max_tile_size <= 0 || (spec->max_tile_size & (spec->max_tile_size - 1)) ||
+        spec->max_tile_size > spec->threads_per_tile ||
         spec->threads_per_tile <= 0 || spec->max_tiles_per_call <= 0 ||
         spec->sort_radix_bits < 0 || spec->sort_radix_bits > 5) {
         g_host_error = "gudni_jobs_build: bad spec";

@@ -26,6 +26,7 @@
 #include "bin.h"
 #include "context.h"
 #include "device.h"
+#include "serialize.h"
 
 namespace gudni {
 namespace dev {
@@ -382,7 +383,7 @@ int bin_scene(gudni_ctx *ctx, const BinIn &in, const BinOut &out, BinCounts &cnt
     binning::Params bp;
     bp.n_shapes = in.n_shapes;
     bp.canvas_depth = ceil_log2(std::max(in.width, in.height));
-    bp.tile_depth = ceil_log2(ctx->spec.max_tile_size);
+    bp.tile_depth = ceil_log2(in.max_tile_size > 0 ? in.max_tile_size : ctx->spec.max_tile_size);
     bp.cap_shapes = ctx->spec.max_shapes - 1;
     bp.cap_strands = (uint32_t)ctx->spec.max_strands_per_tile;
     bp.per_job = ctx->spec.threads_per_tile;
@@ -432,6 +433,47 @@ int bin_scene(gudni_ctx *ctx, const BinIn &in, const BinOut &out, BinCounts &cnt
 
 } // namespace gudni
 
+namespace {
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Appends `bytes` of device memory at an `align`-ed offset of a growing
+// device array (capacity doubles; the used prefix moves along).
+hipError_t append_dev(DevBuf &buf, size_t &used, const void *src, size_t bytes, size_t align, hipStream_t st,
+                      size_t &offset)
+{
+    const size_t at = align_up(used, align), need = at + bytes;
+    if (need > buf.bytes || !buf.p) {
+        const size_t cap = std::max(std::max(need, 2 * buf.bytes), (size_t)256);
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, cap);
+        if (e != hipSuccess) return e;
+        if (used && buf.p) {
+            e = hipMemcpyAsync(p, buf.p, used, hipMemcpyDeviceToDevice, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e != hipSuccess) { (void)hipFree(p); return e; }
+        }
+        if (buf.p) (void)hipFree(buf.p);
+        buf.p = p;
+        buf.bytes = cap;
+    }
+    offset = at;
+    used = need;
+    return bytes ? hipMemcpyAsync((uint8_t *)buf.p + at, src, bytes, hipMemcpyDeviceToDevice, st) : hipSuccess;
+}
+
+__global__ void rebase_work(uint32_t *work, int n, uint32_t tile_base)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        work[i] += tile_base << 2;
+}
+__global__ void fill_i32(int32_t *dst, int n, int32_t v)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = v;
+}
+
+}  // namespace
+
 extern "C" {
 
 int gudni_create(const gudni_spec *spec, int device, gudni_ctx **out)
@@ -447,13 +489,15 @@ int gudni_create(const gudni_spec *spec, int device, gudni_ctx **out)
     if (spec) ctx->spec = *spec;
     else gudni_default_spec(&ctx->spec);
     const gudni_spec &s = ctx->spec;
-    if (s.threads_per_tile != 256 || s.max_shapes < 1 || s.max_shapes > 127 ||
+    if (s.threads_per_tile != 256 || s.max_shapes < 1 || s.max_shapes > 127 || s.max_tile_size < 8 ||
+        s.max_tile_size > s.threads_per_tile || (s.max_tile_size & (s.max_tile_size - 1)) ||
         s.max_thresholds < 1 || s.random_field_size < 1 ||
         (s.coverage_mode != GUDNI_COVERAGE_ACCURATE && s.coverage_mode != GUDNI_COVERAGE_REFERENCE) ||
         (s.texture_filter != GUDNI_FILTER_NEAREST && s.texture_filter != GUDNI_FILTER_BILINEAR) ||
         s.sort_radix_bits < 0 || s.sort_radix_bits > dev::kRadixMaxBits ||
         (s.random_field_size & (s.random_field_size - 1))) {
         g_host_error = "gudni_create: unsupported spec (threads_per_tile must be 256, "
+                       "max_tile_size a power of two in 8..256, "
                        "max_shapes 1..127, random_field_size a power of two, "
                        "sort_radix_bits 0..5)";
         delete ctx;
@@ -913,7 +957,6 @@ struct gudni_batch {
     hipStream_t capture_stream = nullptr;   // capture needs a stream other than the null stream
 };
 
-static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 int gudni_batch_upload(gudni_ctx *ctx, int32_t n_scenes, const gudni_batch_scene *scenes,
                        const float *random_field, int32_t n_random, gudni_batch **out)
@@ -1057,6 +1100,195 @@ int gudni_batch_upload(gudni_ctx *ctx, int32_t n_scenes, const gudni_batch_scene
         gudni_batch_free(b);
         return GUDNI_ERR_HIP;
     }
+    *out = b;
+    return GUDNI_OK;
+}
+
+
+int gudni_batch_upload_outlines(gudni_ctx *ctx, int32_t n_scenes, const gudni_outlines *outlines,
+                                const gudni_batch_scene *scenes, const int32_t *tile_sizes,
+                                const int32_t *radix_bits, const float *random_field, int32_t n_random,
+                                gudni_batch **out)
+{
+    if (!ctx || !outlines || !scenes || !out || n_scenes <= 0) return GUDNI_ERR_ARG;
+    if (!random_field || n_random != ctx->spec.random_field_size)
+        return set_error(ctx, GUDNI_ERR_ARG, "gudni_batch_upload_outlines: random field must hold "
+                                             "random_field_size floats");
+    *out = nullptr;
+    std::vector<uint8_t> pict, uses;
+    std::vector<float4> subs;
+    std::vector<gudni_gradient> grads;
+    std::vector<size_t> subs_off, pict_off, uses_off, grads_off, frame_off;
+    size_t frame_words = 0;
+    for (int32_t s = 0; s < n_scenes; ++s) {
+        const gudni_batch_scene &sc = scenes[s];
+        const std::string where = "gudni_batch_upload_outlines: scene " + std::to_string(s);
+        if (sc.jobs || sc.geo || sc.geo_bytes || sc.width <= 0 || sc.height <= 0 || sc.n_substances < 0 ||
+            sc.n_pict_uses < 0 || (sc.n_substances && !sc.substances) || (sc.pict_bytes && !sc.pict_data) ||
+            (sc.n_pict_uses && !sc.pict_uses) || std::max(sc.width, sc.height) > (1 << 15))
+            return set_error(ctx, GUDNI_ERR_ARG, where + ": bad scene (geometry and jobs must be empty)");
+        const int32_t ts = tile_sizes ? tile_sizes[s] : ctx->spec.max_tile_size;
+        const int32_t rb = radix_bits ? radix_bits[s] : ctx->spec.sort_radix_bits;
+        if (ts < 8 || ts > 256 || (ts & (ts - 1)) || rb < 0 || rb > dev::kRadixMaxBits)
+            return set_error(ctx, GUDNI_ERR_ARG, where + ": tile size must be a power of two in 8..256 "
+                                                         "and the sort radix 0..5");
+        const gudni_outlines &o = outlines[s];
+        if (o.n_shapes < 0 || (o.n_shapes && (!o.tags || !o.shape_outline_offsets || !o.outline_pair_offsets)))
+            return set_error(ctx, GUDNI_ERR_ARG, where + ": bad outlines");
+        int32_t n_outlines = 0, n_pairs = 0;
+        int rc = check_outline_offsets(ctx, &o, n_outlines, n_pairs);
+        if (rc) return set_error(ctx, rc, where + ": " + ctx->error);
+        rc = validate_uses(ctx, sc.pict_uses, sc.n_pict_uses, sc.pict_bytes);
+        if (rc) return rc;
+        rc = validate_gradients(ctx, sc.gradients, sc.n_gradients);
+        if (rc) return rc;
+        subs_off.push_back(subs.size());
+        if (sc.n_substances)
+            subs.insert(subs.end(), (const float4 *)sc.substances, (const float4 *)sc.substances + sc.n_substances);
+        grads_off.push_back(grads.size());
+        if (sc.n_gradients)
+            grads.insert(grads.end(), (const gudni_gradient *)sc.gradients,
+                         (const gudni_gradient *)sc.gradients + sc.n_gradients);
+        pict_off.push_back(pict.size());
+        if (sc.pict_bytes)
+            pict.insert(pict.end(), (const uint8_t *)sc.pict_data, (const uint8_t *)sc.pict_data + sc.pict_bytes);
+        pict.resize(align_up(pict.size(), 16));
+        uses_off.push_back(uses.size() / 24);
+        if (sc.n_pict_uses)
+            uses.insert(uses.end(), (const uint8_t *)sc.pict_uses,
+                        (const uint8_t *)sc.pict_uses + 24 * (size_t)sc.n_pict_uses);
+        frame_off.push_back(frame_words);
+        frame_words += (size_t)sc.width * sc.height;
+    }
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    int rc = ensure_scratch(ctx);
+    if (rc) return rc;
+    const hipStream_t st = ctx->stream;
+    gudni_batch *b = new gudni_batch();
+    b->n_scenes = n_scenes;
+    b->device = ctx->device;
+    b->frame_off = frame_off;
+    for (int32_t s = 0; s < n_scenes; ++s) {
+        b->width.push_back(scenes[s].width);
+        b->height.push_back(scenes[s].height);
+    }
+    auto fail = [&](int code, const std::string &msg) {
+        if (!msg.empty()) ctx->error = msg;
+        gudni_batch_free(b);
+        return code;
+    };
+    auto hip_fail = [&](hipError_t e) {
+        return fail(GUDNI_ERR_HIP, std::string("gudni_batch_upload_outlines: ") + hipGetErrorString(e));
+    };
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = upload(ctx, b->subs, subs.data(), 16 * subs.size());
+    if (e == hipSuccess) e = upload(ctx, b->pict, pict.data(), pict.size());
+    if (e == hipSuccess) e = upload(ctx, b->uses, uses.data(), uses.size());
+    if (e == hipSuccess) e = upload(ctx, b->grads, grads.data(), sizeof(gudni_gradient) * grads.size());
+    if (e == hipSuccess) e = upload(ctx, b->random, random_field, 4 * (size_t)n_random);
+    if (e == hipSuccess) e = b->frames.ensure(4 * frame_words);
+    if (e == hipSuccess) e = hipMemsetAsync(b->frames.p, 0, 4 * frame_words, st);
+    if (e == hipSuccess) e = b->done.ensure(8 * ((size_t)n_scenes + 1));
+    if (e == hipSuccess) e = hipMemsetAsync(b->done.p, 0, 8 * ((size_t)n_scenes + 1), st);
+    if (e != hipSuccess) return hip_fail(e);
+    // every scene serialised and binned on the device, appended to the batch
+    size_t geo_used = 0, refs_used = 0, bounds_used = 0, tiles_used = 0, tj_used = 0, ts_used = 0, work_used = 0;
+    std::vector<size_t> geo_off, refs_off;
+    ctx->scene_ready = false;
+    ctx->frame_ready = false;
+    for (int32_t s = 0; s < n_scenes; ++s) {
+        const gudni_outlines &o = outlines[s];
+        const gudni_batch_scene &sc = scenes[s];
+        const int32_t n = o.n_shapes;
+        const int32_t n_outlines = n ? o.shape_outline_offsets[n] : 0;
+        const int32_t n_pairs = n_outlines ? o.outline_pair_offsets[n_outlines] : 0;
+        if (e == hipSuccess) e = upload(ctx, ctx->ser_tags, o.tags, 8u * (size_t)n);
+        if (e == hipSuccess) e = upload(ctx, ctx->ser_soff, o.shape_outline_offsets, 4u * ((size_t)n + 1));
+        if (e == hipSuccess) e = upload(ctx, ctx->ser_ooff, o.outline_pair_offsets, 4u * ((size_t)n_outlines + 1));
+        if (e == hipSuccess) e = upload(ctx, ctx->ser_pairs, o.pairs, 16u * (size_t)n_pairs);
+        if (e != hipSuccess) return hip_fail(e);
+        SerIn in;
+        in.tags = (const uint64_t *)ctx->ser_tags.p;
+        in.soff = (const int32_t *)ctx->ser_soff.p;
+        in.ooff = (const int32_t *)ctx->ser_ooff.p;
+        in.pairs = (const float *)ctx->ser_pairs.p;
+        in.n_shapes = n;
+        in.width = sc.width;
+        in.height = sc.height;
+        in.subs = (const float4 *)b->subs.p + subs_off[s];
+        in.n_subs = sc.n_substances;
+        in.n_uses = sc.n_pict_uses;
+        in.n_grads = sc.n_gradients;
+        const SerOut so{&ctx->geo, &ctx->scene_shapes, &ctx->scene_bbox, &ctx->scene_bounds, nullptr};
+        SerCounts sc_cnt;
+        rc = serialize_outlines(ctx, in, so, sc_cnt);
+        if (rc) return fail(rc, "scene " + std::to_string(s) + ": " + ctx->error);
+        BinIn bi{(const ShapeRec *)ctx->scene_shapes.p, (const float4 *)ctx->scene_bbox.p,
+                 (const float4 *)ctx->scene_bounds.p, (int32_t)sc_cnt.n_kept, sc.width, sc.height};
+        bi.max_tile_size = tile_sizes ? tile_sizes[s] : 0;
+        const BinOut bo{&ctx->tiles, &ctx->tile_job, &ctx->shapes, &ctx->xrange, &ctx->work};
+        BinCounts bc;
+        rc = bin_scene(ctx, bi, bo, bc);
+        if (rc) return fail(rc, "scene " + std::to_string(s) + ": " + ctx->error);
+        size_t off = 0;
+        if (e == hipSuccess) e = append_dev(b->geo, geo_used, ctx->geo.p, (size_t)sc_cnt.geo_bytes, 16, st, off);
+        geo_off.push_back(off);
+        if (e == hipSuccess) e = append_dev(b->shapes, refs_used, ctx->shapes.p, 16u * (size_t)bc.n_refs, 16, st, off);
+        refs_off.push_back(off / 16);
+        if (e == hipSuccess)
+            e = append_dev(b->bounds, bounds_used, ctx->xrange.p, 16u * (size_t)bc.n_refs, 16, st, off);
+        const size_t tile_base = tiles_used / 32;
+        if (e == hipSuccess) e = append_dev(b->tiles, tiles_used, ctx->tiles.p, 32u * (size_t)bc.n_tiles, 32, st, off);
+        if (e == hipSuccess) e = append_dev(b->tile_job, tj_used, ctx->tile_job.p, 4u * (size_t)bc.n_tiles, 4, st, off);
+        if (e == hipSuccess && bc.n_tiles) {
+            e = append_dev(b->tile_scene, ts_used, ctx->tile_job.p, 4u * (size_t)bc.n_tiles, 4, st, off);
+            if (e == hipSuccess)
+                hipLaunchKernelGGL(fill_i32, dim3(std::min(1024, (bc.n_tiles + 255) / 256)), dim3(256), 0, st,
+                                   (int32_t *)((uint8_t *)b->tile_scene.p + off), bc.n_tiles, (int32_t)s);
+        }
+        b->work_begin.push_back((int32_t)(work_used / 4));
+        b->work_count.push_back(bc.n_work);
+        if (e == hipSuccess && bc.n_work) {
+            e = append_dev(b->work, work_used, ctx->work.p, 4u * (size_t)bc.n_work, 4, st, off);
+            if (e == hipSuccess)
+                hipLaunchKernelGGL(rebase_work, dim3(std::min(1024, (bc.n_work + 255) / 256)), dim3(256), 0, st,
+                                   (uint32_t *)((uint8_t *)b->work.p + off), bc.n_work, (uint32_t)tile_base);
+        }
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e);
+        // the next scene's serialisation reuses the scratch: wait for the copies
+        e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return hip_fail(e);
+    }
+    if (tiles_used / 32 >= (1u << 30)) return fail(GUDNI_ERR_ARG, "gudni_batch_upload_outlines: too many tiles");
+    for (DevBuf *d : {&b->geo, &b->shapes, &b->bounds, &b->tiles, &b->tile_job, &b->tile_scene, &b->work})
+        if (e == hipSuccess && !d->p) e = d->ensure(16);
+    if (e != hipSuccess) return hip_fail(e);
+    b->n_work = (int32_t)(work_used / 4);
+    rc = ensure_control(ctx, b->n_work);
+    if (rc) return fail(rc, "");
+    std::vector<dev::SceneDesc> descs((size_t)n_scenes);
+    for (int32_t s = 0; s < n_scenes; ++s) {
+        dev::SceneDesc &d = descs[s];
+        std::memset(&d, 0, sizeof(d));
+        d.geo = (const uint8_t *)b->geo.p + geo_off[s];
+        d.shapes = (const ShapeRec *)b->shapes.p + refs_off[s];
+        d.bounds = (const float4 *)b->bounds.p + refs_off[s];
+        d.substances = (const float4 *)b->subs.p + subs_off[s];
+        d.pict = (const uint8_t *)b->pict.p + pict_off[s];
+        d.uses = (const PictUse *)b->uses.p + uses_off[s];
+        d.out = (uint32_t *)b->frames.p + frame_off[s];
+        d.out_f32 = nullptr;
+        d.grads = (const float4 *)((const gudni_gradient *)b->grads.p + grads_off[s]);
+        d.filter = ctx->spec.texture_filter;
+        d.sort_bits = radix_bits ? radix_bits[s] : ctx->spec.sort_radix_bits;
+        for (int i = 0; i < 4; ++i) d.bg[i] = scenes[s].background[i];
+        d.width = scenes[s].width;
+        d.height = scenes[s].height;
+    }
+    e = upload(ctx, b->scenes, descs.data(), sizeof(dev::SceneDesc) * descs.size());
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e);
     *out = b;
     return GUDNI_OK;
 }

@@ -39,7 +39,9 @@ extern "C" {
  * (OpenCL/Setup.hs:71-87) yields for a device reporting a maximum work-group
  * size of 256 and a 4 GiB maximum allocation. */
 typedef struct gudni_spec {
-    int32_t max_tile_size;        /* specMaxTileSize      = 256               */
+    int32_t max_tile_size;        /* specMaxTileSize      = 256 (a power of
+                                     two, at most threads_per_tile; 8..256
+                                     for a context)                          */
     int32_t threads_per_tile;     /* specThreadsPerTile   = 256               */
     int32_t max_tiles_per_call;   /* specMaxTilesPerCall  = 256               */
     int32_t max_thresholds;       /* specMaxThresholds    = 4096              */
@@ -383,6 +385,21 @@ int  gudni_batch_upload(gudni_ctx *ctx, int32_t n_scenes, const gudni_batch_scen
 /* Each scene's records are sorted with the sort_radix_bits of the spec its
  * jobs were built with (so a tuner may choose tile size and radix per
  * scene). */
+/* The same batch built from outlines on the device: each scene is
+ * serialised (as gudni_scene_upload_outlines) and binned (as
+ * gudni_scene_bin) with its own tile size tile_sizes[s] (a power of two in
+ * 8..256; NULL: the spec's max_tile_size) and sort radix radix_bits[s]
+ * (0..5; NULL: the spec's), then appended to the batch's device arrays.
+ * scenes[s] supplies canvas, background, substances, pictures and
+ * gradients; its geo and jobs must be empty.  Reuses the context's
+ * uploaded-frame and scene buffers, so a frame uploaded before must be
+ * uploaded again.  No counterpart in the reference, whose serialiser and
+ * tile builder run on the host for one frame at a time (Raster/Serialize.hs:
+ * 148-177, Raster/TileTree.hs:81-204, Raster/Job.hs:132-179). */
+int  gudni_batch_upload_outlines(gudni_ctx *ctx, int32_t n_scenes, const gudni_outlines *outlines,
+                                 const gudni_batch_scene *scenes, const int32_t *tile_sizes,
+                                 const int32_t *radix_bits, const float *random_field,
+                                 int32_t n_random, gudni_batch **out);
 /* Renders every scene (stream-ordered).  use_graph: on the first call the
  * launch sequence is captured into a hipGraph on a stream the batch owns,
  * then that graph is replayed on the context's stream (the null stream

@@ -71,3 +71,12 @@ def test_geometry_culls_offcanvas_and_empty_shapes():
     geo = G.Geometry(scene)
     assert list(geo.source_index) == [0, 3]
     assert list(geo.strand_count) == [4, 0]
+
+
+def test_jobs_reject_tiles_wider_than_the_threads():
+    L = _lib.load()
+    h = C.c_void_p()
+    geo = G.Geometry(S.uniform_512(5, 32, 1))
+    wide = G.default_spec(max_tile_size=512)
+    assert L.gudni_jobs_build(C.byref(wide), geo.handle, 32, 32, C.byref(h)) == -1
+    assert b"bad spec" in L.gudni_last_error(None)

@@ -105,3 +105,65 @@ def test_batch_other_tile_sizes(tile):
         b.close()
     finally:
         r.close()
+
+
+def test_batch_from_outlines_equals_host_batch():
+    """The batch serialised and binned on the device, each scene with its own
+    tile size and sort radix, renders the frames of the batch built on the
+    host with the same per-scene specs, and the oracle's."""
+    import torch
+    from tests.test_gpu_binning import empty_scene
+    from tests.test_gpu_serialize import awkward_scene
+    scenes = batch_scenes() + [empty_scene(), awkward_scene(with_nan=False), S.blobs_scaled(1024, 12500, 7)]
+    tiles = [256, 64, 128, 256, 64, 128, 256, 32, 128, 64, 256]
+    radix = [0, 3, 5, 4, 0, 3, 5, 2, 1, 0, 3]
+    assert len(tiles) == len(radix) == len(scenes)
+    r = G.Rasterizer()
+    try:
+        r.set_stream(torch.cuda.current_stream().cuda_stream)
+        items = [(sc,) + G.prepare(sc, G.default_spec(max_tile_size=t, sort_radix_bits=q))[1:]
+                 for sc, t, q in zip(scenes, tiles, radix)]
+        say(f"host-built batch of {len(items)} scenes")
+        host = G.Batch(r, items)
+        host.render()
+        host.check()
+        want = [host.frame(i) for i in range(len(scenes))]
+        host.close()
+        for rep in range(2):
+            say(f"device-built batch, upload {rep}")
+            b = G.Batch.from_outlines(r, scenes, tiles, radix)
+            for use_graph in (False, True, True):
+                b.render(use_graph=use_graph)
+                b.check()
+                for i, sc in enumerate(scenes):
+                    got = b.frame(i)
+                    assert np.array_equal(got, want[i]), \
+                        f"scene {i} ({sc.name}, tile {tiles[i]}) differs from the host-built batch"
+            b.close()
+        for i in (0, 2, 9):
+            spec = G.default_spec(max_tile_size=tiles[i])
+            ref = H.oracle_render(scenes[i], spec, want_f64=False)["out"]
+            bad = H.first_mismatch(want[i], ref)
+            assert bad is None, f"scene {scenes[i].name}: mismatch with the oracle {bad}"
+    finally:
+        r.close()
+
+
+def test_batch_from_outlines_rejects_bad_tiles():
+    r = G.Rasterizer()
+    try:
+        scenes = [H.tiny_square(), S.uniform_512(100, 100, 1)]
+        for tiles in ([256, 48], [512, 256], [256, 4]):
+            with pytest.raises(G.GudniError) as ei:
+                G.Batch.from_outlines(r, scenes, tiles, None)
+            assert "tile size" in str(ei.value)
+        with pytest.raises(G.GudniError):
+            G.Batch.from_outlines(r, scenes, None, [0, 6])
+        b = G.Batch.from_outlines(r, scenes)
+        b.render()
+        b.check()
+        ref = H.oracle_render(scenes[0], want_f64=False)["out"]
+        assert H.first_mismatch(b.frame(0), ref) is None
+        b.close()
+    finally:
+        r.close()
