@@ -403,9 +403,9 @@ def batched_adjoint_leg(spec, steps, log_fn, fixed=None):
 
 def batched_leg(spec, steps, log_fn):
     """The batched workload: 256 scenes of 256^2..2048^2 with 100..50k shapes,
-    tile size and sort radix per scene-size group chosen by the autotuner, all scenes in one
-    persistent launch replayed from a captured hipGraph; compared with the same
-    scenes rendered by one launch each."""
+    tile size and sort radix chosen by the autotuner on the whole batch, all
+    scenes in one persistent launch replayed from a captured hipGraph;
+    compared with the same scenes rendered by one launch each."""
     import torch
     import gudni_amd as G
     from gudni_amd import autotune, scenes as S
@@ -416,9 +416,9 @@ def batched_leg(spec, steps, log_fn):
     stream = torch.cuda.current_stream()
     r.set_stream(stream.cuda_stream)
     t = time.perf_counter()
-    table = autotune.tune(r, scenes, per_bucket=2, reps=2, log=log_fn)
+    tile, radix, timings = autotune.tune_batch(r, scenes, reps=2, log=log_fn)
     t_tune = time.perf_counter() - t
-    items, chosen, radices, t_prep = autotune.prepare_batch(scenes, table)
+    items, chosen, radices, t_prep = autotune.prepare_batch(scenes, default_tile=tile, default_radix=radix)
     b_alg = sum(algorithmic_bytes(sc, geo, jobs) for sc, geo, jobs in items)
     t = time.perf_counter()
     b = G.Batch(r, items)
@@ -474,7 +474,7 @@ def batched_leg(spec, steps, log_fn):
         "sequential_latency_p50_ms": float(np.percentile(seq_lat, 50)),
         "sequential_latency_p99_ms": float(np.percentile(seq_lat, 99)),
         "speedup_vs_sequential": float(seq.sum() / ms),
-        "tuned_by_group": table, "graph": True,
+        "tile_size": tile, "sort_radix_bits": radix, "autotune_ms": timings, "graph": True,
         "host_s": {"generate": t_gen, "autotune": t_tune, "serialize_and_bin": t_prep, "upload": t_up},
         "device_built": {"build_s": t_dev, "ms_per_batch": ms_dev, "frames_equal_host_built": same,
                          "sampled_scenes": sample},
@@ -498,6 +498,8 @@ def main():
     ap.add_argument("--adjoint-radix", type=int, default=4)
     ap.add_argument("--sort-radix", type=int, default=-1,
                     help="sort radix bits of the frame (default: chosen by the autotuner)")
+    ap.add_argument("--tile", type=int, default=0,
+                    help="tile size of the frame (default: chosen by the autotuner)")
     ap.add_argument("--texture-filter", type=int, default=1,
                     help="picture sampling of the benchmark scenes: 1 bilinear, 0 nearest")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -550,10 +552,10 @@ def main():
     npix = scene.width * scene.height
     out = torch.zeros(npix, dtype=torch.int32, device="cuda")
 
-    # sort radix for this frame (tile size stays the pinned 256)
-    def frame_ms(radix):
-        j = G.RasterJobs(G.default_spec(texture_filter=args.texture_filter, sort_radix_bits=radix), geo,
-                         scene.width, scene.height)
+    # tile size and sort radix for this frame
+    def frame_ms(tile, radix):
+        j = G.RasterJobs(G.default_spec(texture_filter=args.texture_filter, sort_radix_bits=radix,
+                                        max_tile_size=tile), geo, scene.width, scene.height)
         r.frame_upload(j, scene)
         r.frame_render(out.data_ptr())
         r.frame_check()
@@ -566,12 +568,15 @@ def main():
         r.frame_check()
         return e0.elapsed_time(e1) / 3
 
-    if args.sort_radix >= 0:
-        radix, radix_ms = args.sort_radix, None
+    if args.sort_radix >= 0 and args.tile > 0:
+        tile, radix, tune_ms = args.tile, args.sort_radix, None
     else:
-        radix, radix_ms = autotune.tune_frame(frame_ms)
-        log(f"frame sort radix: {json.dumps(radix_ms)} -> {radix}")
-    spec = G.default_spec(texture_filter=args.texture_filter, sort_radix_bits=radix)
+        tile, radix, tune_ms = autotune.tune_frame(
+            frame_ms, tiles=(args.tile,) if args.tile > 0 else autotune.CANDIDATES,
+            radices=(args.sort_radix,) if args.sort_radix >= 0 else (0, 3, 4, 5),
+            default_radix=max(args.sort_radix, 0))
+        log(f"frame tile size and sort radix: {json.dumps(tune_ms)} -> tile {tile}, radix {radix}")
+    spec = G.default_spec(texture_filter=args.texture_filter, sort_radix_bits=radix, max_tile_size=tile)
     jobs = G.RasterJobs(spec, geo, scene.width, scene.height)
     r.close()
     r = G.Rasterizer(spec)
@@ -620,6 +625,10 @@ def main():
 
     b_alg = algorithmic_bytes(scene, geo, jobs)
     achieved = b_alg / (fast_ms * 1e-3) / 1e9
+    # the same figure with the pinned 256-pixel tiles' (smaller) byte count:
+    # smaller tiles read the strands of a shape once per tile it meets
+    b_alg_pinned = algorithmic_bytes(scene, geo, G.RasterJobs(G.default_spec(texture_filter=args.texture_filter),
+                                                              geo, scene.width, scene.height))
     gpix = npix / (ms * 1e-3) / 1e9
     mcurves = scene.n_curves / (ms * 1e-3) / 1e6
 
@@ -660,13 +669,15 @@ def main():
                                + " sampling)",
                    "canvas": [scene.width, scene.height], "shapes": scene.n_shapes,
                    "curves": scene.n_curves, "tiles": jobs.n_tiles, "tile_shape_refs": jobs.n_refs,
-                   "coverage_mode": "accurate", "sort_radix_bits": radix,
-                   "sort_radix_autotune_ms": radix_ms},
+                   "coverage_mode": "accurate", "tile_size": tile, "sort_radix_bits": radix,
+                   "autotune_ms": tune_ms},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_detail": traffic_info,
                      "kernel": "raster_pass (fast pass)", "kernel_ms": fast_ms,
                      "algorithmic_bytes": b_alg, "retry_pass_ms": float(np.mean(retry)),
+                     "algorithmic_bytes_256_px_tiles": b_alg_pinned,
+                     "frac_with_256_px_tile_bytes": b_alg_pinned / (fast_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "peak_measured_copy_gbs": copy_gbs, "frac_of_measured_copy": achieved / copy_gbs,
                      "stage_cycle_share": stage_share},
         "device_binning": binning,

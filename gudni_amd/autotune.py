@@ -9,11 +9,16 @@ cost of more shape references.  The sort radix is the digit width of the
 radix pass ahead of each column's insertion sort: it pays where columns hold
 many records, and every width yields the same permutation.
 
-`tune` groups the scenes of a batch by canvas side and shape count, renders a
-few scenes of each group as a batch with each candidate tile size (at the
-default radix), then with each candidate radix at the chosen tile size, and
-keeps the fastest per group.  `tune_frame` and `tune_adjoint` do the same for
-one frame or one adjoint workload with a caller-supplied timing function.
+`tune_batch` picks one tile size and radix for a whole batch by timing the
+whole batch (throughput).  `tune` groups the scenes of a batch by canvas
+side and shape count (latency: each group's own best), renders a
+few scenes of each group, repeated until the batch holds `fill_pixels`
+pixels (so the device is as full as in the real batch and the timing
+measures throughput, not the latency of a few scenes), with each candidate
+tile size (at the default radix), then with each candidate radix at the
+chosen tile size, and keeps the fastest per group.  The timed batches are
+built on the device from the outlines.  `tune_frame` does the same for one
+frame with a caller-supplied timing function.
 """
 from __future__ import annotations
 
@@ -24,7 +29,7 @@ import numpy as np
 
 from . import Batch, default_spec, prepare
 
-CANDIDATES = (64, 128, 256)
+CANDIDATES = (32, 64, 128, 256)
 RADIX_CANDIDATES = (0, 3, 5)
 
 
@@ -36,9 +41,8 @@ def bucket(scene) -> str:
     return f"{sb}-{nb}"
 
 
-def _time_batch(rasterizer, items, reps: int) -> float:
+def _time_render(b, reps: int) -> float:
     import torch
-    b = Batch(rasterizer, items)
     try:
         b.render()
         b.check()
@@ -72,7 +76,7 @@ def search(time_fn, tiles=CANDIDATES, radices=RADIX_CANDIDATES, default_radix: i
 
 
 def tune(rasterizer, scenes, per_bucket: int = 3, reps: int = 3, candidates=CANDIDATES,
-         radices=RADIX_CANDIDATES, log=None) -> dict:
+         radices=RADIX_CANDIDATES, log=None, fill_pixels: int = 16 << 20) -> dict:
     """Returns {bucket: {"tile": size, "radix": bits}} measured on the
     rasterizer's device."""
     groups: dict[str, list] = {}
@@ -81,11 +85,12 @@ def tune(rasterizer, scenes, per_bucket: int = 3, reps: int = 3, candidates=CAND
     table = {}
     for key, members in sorted(groups.items()):
         sample = members[:per_bucket]
+        pixels = sum(sc.width * sc.height for sc in sample)
+        filled = sample * max(1, -(-fill_pixels // pixels))
 
         def time_fn(ts, rb):
-            spec = default_spec(max_tile_size=ts, sort_radix_bits=rb)
-            items = [(sc,) + prepare(sc, spec)[1:] for sc in sample]
-            return _time_batch(rasterizer, items, reps)
+            n = len(filled)
+            return _time_render(Batch.from_outlines(rasterizer, filled, [ts] * n, [rb] * n), reps)
 
         ts, rb, timings = search(time_fn, candidates, radices)
         table[key] = {"tile": ts, "radix": rb}
@@ -94,19 +99,36 @@ def tune(rasterizer, scenes, per_bucket: int = 3, reps: int = 3, candidates=CAND
     return table
 
 
-def tune_frame(time_fn, radices=(0, 3, 4, 5)):
-    """Radix for one frame at a fixed tile size.  time_fn(radix) -> ms."""
-    timings = {rb: time_fn(rb) for rb in radices}
-    best = min(timings, key=timings.get)
-    return best, {str(k): round(v, 3) for k, v in timings.items()}
+def tune_batch(rasterizer, scenes, reps: int = 2, tiles=CANDIDATES, radices=RADIX_CANDIDATES, log=None):
+    """One tile size and radix for every scene of a batch, timed on the whole
+    batch (built on the device): the throughput choice.  A small tile
+    shortens each scene's latency but re-reads more geometry, and in a
+    full batch the device is busy anyway.  Returns (tile, radix,
+    {"tile,radix": ms})."""
+    n = len(scenes)
+
+    def time_fn(ts, rb):
+        return _time_render(Batch.from_outlines(rasterizer, scenes, [ts] * n, [rb] * n), reps)
+
+    ts, rb, timings = search(time_fn, tiles, radices)
+    if log:
+        log(f"autotune whole batch: {json.dumps(timings)} -> tile {ts}, radix {rb}")
+    return ts, rb, timings
 
 
-def prepare_batch(scenes, table: dict | None = None, default_tile: int = 256):
+def tune_frame(time_fn, tiles=CANDIDATES, radices=(0, 3, 4, 5), default_radix: int = 0):
+    """Tile size (timed at default_radix) and then radix for one frame.
+    time_fn(tile, radix) -> ms.  Returns (tile, radix, {"tile,radix": ms})."""
+    return search(time_fn, tiles, radices, default_radix)
+
+
+def prepare_batch(scenes, table: dict | None = None, default_tile: int = 256, default_radix: int | None = None):
     """(scene, Geometry, RasterJobs) per scene with the tuned tile size and
-    radix; returns (items, tile sizes, radices, host seconds)."""
+    radix (per group from `table`, else the defaults); returns (items, tile
+    sizes, radices, host seconds)."""
     items, tiles, radices = [], [], []
     t0 = time.perf_counter()
-    r0 = default_spec().sort_radix_bits
+    r0 = default_spec().sort_radix_bits if default_radix is None else default_radix
     for sc in scenes:
         choice = (table or {}).get(bucket(sc), {"tile": default_tile, "radix": r0})
         spec = default_spec(max_tile_size=choice["tile"], sort_radix_bits=choice["radix"])

@@ -128,6 +128,14 @@ CommonView ctx_view(gudni_ctx *ctx);
 // Active (tile, quarter) work units of a tile list (isActiveThread).
 void build_work(const gudni_spec &spec, const TileRec *tiles, int n_tiles, int width, int height,
                 std::vector<uint32_t> &work);
+// The work list of a whole-batch launch: the long units (estimated cost
+// rows per lane x ((tile shapes + 1) / tile width)^2 within a factor 8 of the
+// longest) sorted longest first and dealt tile by tile, round robin, to the
+// starts of the `parts` contiguous ranges the claimers begin with, so none is
+// claimed last; every other unit follows in its original order, which keeps
+// a scene's tiles together on one XCD.
+void order_longest_first(int threads_per_tile, const TileRec *tiles, const uint32_t *work, int n, int parts,
+                         uint32_t *out);
 // Scratch rings of the raster passes, sized on first use.
 int ensure_scratch(gudni_ctx *ctx);
 // Pass counters, status word and retry list for n_work units.

@@ -135,6 +135,55 @@ void build_work(const gudni_spec &spec, const TileRec *tiles, int n_tiles, int w
     }
 }
 
+void order_longest_first(int threads_per_tile, const TileRec *tiles, const uint32_t *work, int n, int parts,
+                         uint32_t *out)
+{
+    const int cd = ceil_log2(threads_per_tile);
+    // a lane sweeps `rows` pixels whose cost grows with the tile's shape
+    // density per column: rows x ((shapes + 1) / width)^2
+    std::vector<double> cost((size_t)n);
+    double top = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const TileRec &r = tiles[work[i] >> 2];
+        const int rows = std::max(1, (r.box[3] - r.box[1]) >> std::max(0, cd - (int)r.hdepth));
+        const double d = (double)(std::max(0, r.shape_count) + 1) / (double)(1 << std::max(0, (int)r.hdepth));
+        cost[i] = rows * d * d;
+        top = std::max(top, cost[i]);
+    }
+    // the long units: within kSpread of the longest
+    constexpr double kSpread = 8.0;
+    std::vector<int> heavy;
+    for (int i = 0; i < n; ++i)
+        if (cost[i] * kSpread >= top) heavy.push_back(i);
+    std::stable_sort(heavy.begin(), heavy.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<long long> begin((size_t)parts), size((size_t)parts), fill((size_t)parts, 0);
+    for (int p = 0; p < parts; ++p) {
+        begin[p] = (long long)n * p / parts;
+        size[p] = (long long)n * (p + 1) / parts - begin[p];
+    }
+    // long units first, dealt tile by tile round robin to the range starts
+    std::vector<char> placed((size_t)n, 0);
+    int p = 0;
+    for (size_t i = 0; i < heavy.size();) {
+        size_t j = i + 1;
+        while (j < heavy.size() && (work[heavy[j]] >> 2) == (work[heavy[i]] >> 2)) ++j;
+        for (size_t k = i; k < j; ++k) {
+            for (int tries = 0; fill[p] == size[p] && tries < parts; ++tries) p = (p + 1) % parts;
+            out[begin[p] + fill[p]++] = work[heavy[k]];
+            placed[heavy[k]] = 1;
+        }
+        p = (p + 1) % parts;
+        i = j;
+    }
+    // then every other unit in its original order, range after range
+    p = 0;
+    for (int i = 0; i < n; ++i) {
+        if (placed[i]) continue;
+        while (fill[p] == size[p]) ++p;
+        out[begin[p] + fill[p]++] = work[i];
+    }
+}
+
 int ensure_scratch(gudni_ctx *ctx)
 {
     if (ctx->fast_blocks) return GUDNI_OK;
@@ -947,6 +996,7 @@ struct gudni_batch {
     int device = 0;
     DevBuf geo, subs, pict, uses, shapes, bounds, tiles, tile_job, tile_scene, work, scenes, done,
         frames, random, grads;
+    DevBuf work_all;   // the whole-batch launch's work list, longest units first
     std::vector<size_t> frame_off;
     std::vector<int32_t> width, height;
     std::vector<int32_t> work_begin, work_count;
@@ -1069,6 +1119,12 @@ int gudni_batch_upload(gudni_ctx *ctx, int32_t n_scenes, const gudni_batch_scene
     if (e == hipSuccess) e = up(b->tile_job, tile_job.data(), 4 * tile_job.size());
     if (e == hipSuccess) e = up(b->tile_scene, tile_scene.data(), 4 * tile_scene.size());
     if (e == hipSuccess) e = up(b->work, work.data(), 4 * work.size());
+    if (e == hipSuccess) {
+        std::vector<uint32_t> all(work.size());
+        order_longest_first(ctx->spec.threads_per_tile, tiles.data(), work.data(), (int)work.size(), dev::kParts,
+                            all.data());
+        e = up(b->work_all, all.data(), 4 * all.size());
+    }
     if (e == hipSuccess) e = up(b->random, random_field, 4 * (size_t)n_random);
     if (e == hipSuccess) e = up(b->grads, grads.data(), sizeof(gudni_gradient) * grads.size());
     if (e == hipSuccess) e = b->frames.ensure(4 * frame_words);
@@ -1265,6 +1321,16 @@ int gudni_batch_upload_outlines(gudni_ctx *ctx, int32_t n_scenes, const gudni_ou
         if (e == hipSuccess && !d->p) e = d->ensure(16);
     if (e != hipSuccess) return hip_fail(e);
     b->n_work = (int32_t)(work_used / 4);
+    {
+        std::vector<TileRec> th(tiles_used / 32);
+        std::vector<uint32_t> wh((size_t)b->n_work), all((size_t)b->n_work);
+        if (!th.empty()) e = hipMemcpy(th.data(), b->tiles.p, tiles_used, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && !wh.empty()) e = hipMemcpy(wh.data(), b->work.p, work_used, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(e);
+        order_longest_first(ctx->spec.threads_per_tile, th.data(), wh.data(), b->n_work, dev::kParts, all.data());
+        e = upload(ctx, b->work_all, all.data(), 4 * all.size());
+        if (e != hipSuccess) return hip_fail(e);
+    }
     rc = ensure_control(ctx, b->n_work);
     if (rc) return fail(rc, "");
     std::vector<dev::SceneDesc> descs((size_t)n_scenes);
@@ -1301,7 +1367,7 @@ static dev::Frame batch_frame(gudni_ctx *ctx, gudni_batch *b)
     f.scene_done = (unsigned long long *)b->done.p + 1;
     f.tiles = (const TileRec *)b->tiles.p;
     f.tile_job = (const int32_t *)b->tile_job.p;
-    f.work = (const uint32_t *)b->work.p;
+    f.work = (const uint32_t *)b->work_all.p;
     f.n_work = b->n_work;
     f.random = (const float *)b->random.p;
     return f;
@@ -1400,6 +1466,18 @@ int gudni_batch_frame(gudni_ctx *ctx, gudni_batch *b, int32_t scene, uint32_t *h
     return GUDNI_OK;
 }
 
+int gudni_order_work(const void *tiles, int32_t n_tiles, const uint32_t *work, int32_t n_work,
+                     int32_t threads_per_tile, int32_t parts, uint32_t *out)
+{
+    if (n_tiles < 0 || n_work < 0 || parts < 1 || threads_per_tile < 1 || (n_work && (!tiles || !work || !out)))
+        return GUDNI_ERR_ARG;
+    const TileRec *t = (const TileRec *)tiles;
+    for (int32_t i = 0; i < n_work; ++i)
+        if ((int64_t)(work[i] >> 2) >= n_tiles) return GUDNI_ERR_ARG;
+    order_longest_first(threads_per_tile, t, work, n_work, parts, out);
+    return GUDNI_OK;
+}
+
 void gudni_batch_free(gudni_batch *b)
 {
     if (!b) return;
@@ -1409,7 +1487,7 @@ void gudni_batch_free(gudni_batch *b)
     if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
     DevBuf *bufs[] = {&b->geo, &b->subs, &b->pict, &b->uses, &b->shapes, &b->bounds, &b->tiles,
                       &b->tile_job, &b->tile_scene, &b->work, &b->scenes, &b->done, &b->frames,
-                      &b->random, &b->grads};
+                      &b->random, &b->grads, &b->work_all};
     for (DevBuf *d : bufs) d->release();
     delete b;
 }

@@ -414,6 +414,19 @@ int  gudni_batch_check(gudni_ctx *ctx, gudni_batch *b, float *latency_ms);
 /* Copies one scene's BGRA frame (width*height words) to host memory. */
 int  gudni_batch_frame(gudni_ctx *ctx, gudni_batch *b, int32_t scene, uint32_t *host_out);
 void gudni_batch_free(gudni_batch *b);
+/* The order a whole-batch launch hands out its work units (tile << 2 |
+ * quarter, over `tiles`, TileInfo records).  A unit's estimated cost is the
+ * rows each of its threads sweeps x ((the tile's shape count + 1) / tile
+ * width)^2; the units within a factor 8 of the costliest go first, longest
+ * first, dealt tile by tile, round robin, to the starts of the `parts`
+ * contiguous ranges the launch's claimers begin with (range p = [n p / parts,
+ * n (p+1) / parts)), so that none is claimed last; the others follow in
+ * their original order.  Batches built by
+ * gudni_batch_upload and gudni_batch_upload_outlines use it; per-scene
+ * launches keep each scene's own order.  An extension: the reference launches
+ * one job at a time (OpenCL/CallKernels.hs:244-255). */
+int  gudni_order_work(const void *tiles, int32_t n_tiles, const uint32_t *work, int32_t n_work,
+                      int32_t threads_per_tile, int32_t parts, uint32_t *out);
 
 /* Test hook: per-column threshold records after generate (queue order) and
  * after sort, plus the initial shape state, for one RasterJob.  Slot =

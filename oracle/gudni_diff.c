@@ -23,8 +23,13 @@
  *     front-to-back composite of the set shapes (front = lowest scene index,
  *     stopping at alpha 1) over the background, straightened
  *   -> pixel = sum(area * colour) / sum(area), loss = sum (pixel - target)^2.
- * A column segment here is the window's rows; tiles only cut columns into
- * segments, which (up to the 127-shape cap, not modelled) changes nothing.
+ * A column segment here is the window's rows (go_diff_loss) or `seg` rows
+ * aligned to multiples of seg (go_diff_grad).  Tiles cut columns into such
+ * segments; each segment trims the records crossing its top and sorts its
+ * own, and since the sweep never reorders active records that cross inside a
+ * span (a section of negative width follows), the forward depends on where
+ * the columns are cut: checks of a tiled forward sum go_diff_loss over the
+ * same segments.
  *
  * Every discrete decision (culling, knob splits, strand runs, the curves
  * found, bisection paths, record drops, the number of sections per pixel)

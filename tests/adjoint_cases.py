@@ -53,12 +53,51 @@ def shape_window(scene, s, pad=3):
     return (max(0, x0), max(0, y0), min(scene.width, x1), min(scene.height, y1))
 
 
-def fd_samples(scene, target, n=64, seed=7, h_point=1e-2, h_color=1e-3, max_tries=600):
+def segment_rows(scene, spec) -> int:
+    """Rows per column segment of the tile tree `spec` builds for the scene:
+    a tile 2^hd wide and 2^vd tall is rendered by threads_per_tile threads,
+    2^(log2 threads - hd) of them stacked in each column (initTileState,
+    Kernels.cl:1692-1722).  Each segment trims the records crossing its top
+    and starts its own sort, and the reference's sweep does not reorder
+    active records that cross inside a span, so the forward (and its
+    gradient) depends on where columns are cut.  The cases here use one
+    segment height everywhere; anything else is an error."""
+    import gudni_amd as G
+    _, _, jobs = G.prepare(scene, spec)
+    t = np.frombuffer(jobs.tiles.tobytes(), np.int32).reshape(-1, 8)
+    hd = (t[:, 4] & 0xFFFF).astype(np.int64)
+    vd = (t[:, 4] >> 16).astype(np.int64)
+    log_threads = int(spec.threads_per_tile).bit_length() - 1
+    rows = {max(0, int(v) - max(0, log_threads - int(h))) for h, v in zip(hd, vd)}
+    if len(rows) != 1:
+        raise ValueError(f"column segments of several heights: {sorted(1 << r for r in rows)}")
+    return 1 << rows.pop()
+
+
+def segmented_loss(scene, pairs, rgba, target, window, seg=None):
+    """The FP64 forward's loss over a window with the columns cut into
+    segments of `seg` rows aligned to multiples of seg (None: one segment
+    per window), and the decision signature of every segment."""
+    if seg is None:
+        return O.diff_loss(scene, pairs, rgba, target, window)
+    x0, y0, x1, y1 = window
+    y0 = (y0 // seg) * seg
+    y1 = min(scene.height, -(-y1 // seg) * seg)
+    total, sig = 0.0, []
+    for ys in range(y0, y1, seg):
+        l, _, s = O.diff_loss(scene, pairs, rgba, target, (x0, ys, x1, min(ys + seg, scene.height)))
+        total += l
+        sig.append(s)
+    return total, None, tuple(sig)
+
+
+def fd_samples(scene, target, n=64, seed=7, h_point=1e-2, h_color=1e-3, max_tries=600, seg=None):
     """n accepted samples: (kind, index, component, fd) with kind 'point'
     (index = pair, component 0..3) or 'rgba' (index = substance).  A sample
     is rejected when the decision signature at +h or -h differs from the
     base's, or when the central differences at h and h/2 disagree (a kink
-    inside the step that the signature does not name)."""
+    inside the step that the signature does not name).  seg: the column
+    segment height of the forward being checked (segment_rows)."""
     rng = np.random.default_rng(seed)
     owner = pair_shape(scene)
     pairs = scene.pairs.reshape(-1, 4).astype(np.float64)
@@ -85,7 +124,7 @@ def fd_samples(scene, target, n=64, seed=7, h_point=1e-2, h_color=1e-3, max_trie
             else:
                 cc = rgba.copy()
                 cc[p, comp] += delta
-            return O.diff_loss(scene, pp, cc, target, win)
+            return segmented_loss(scene, pp, cc, target, win, seg)
 
         l0, _, s0 = loss(0.0)
         lp, _, sp = loss(h)

@@ -80,3 +80,52 @@ def test_jobs_reject_tiles_wider_than_the_threads():
     wide = G.default_spec(max_tile_size=512)
     assert L.gudni_jobs_build(C.byref(wide), geo.handle, 32, 32, C.byref(h)) == -1
     assert b"bad spec" in L.gudni_last_error(None)
+
+
+def test_work_order_longest_first():
+    """gudni_order_work: a permutation of the work list; the units within a
+    factor 8 of the costliest (rows per lane x ((shapes + 1) / width)^2) open
+    the eight claimer ranges, longest first, a tile's quarters together; all
+    other units keep their original order."""
+    rng = np.random.default_rng(4)
+    n_tiles = 300
+    tiles = np.zeros((n_tiles, 8), np.int32)
+    hd = rng.integers(3, 9, n_tiles)
+    tiles[:, 2] = 1 << hd                                   # box: 0 0 w h
+    tiles[:, 3] = 1 << rng.integers(3, 9, n_tiles)
+    tiles[:, 4] = (hd | (hd << 16)).astype(np.int32)        # hdepth, vdepth
+    tiles[:, 7] = rng.integers(0, 300, n_tiles)             # shape count
+    work = np.array([(t << 2) | q for t in range(n_tiles) for q in range(4) if q < max(1, (1 << hd[t]) // 64)],
+                    np.uint32)
+    out = np.zeros_like(work)
+    L = _lib.load()
+    assert L.gudni_order_work(tiles.ctypes.data, n_tiles, work.ctypes.data, work.size, 256, 8,
+                              out.ctypes.data) == 0
+    assert sorted(out.tolist()) == sorted(work.tolist())
+
+    def cost(units):
+        t = units >> 2
+        rows = np.maximum(1, tiles[t, 3] >> np.maximum(0, 8 - hd[t]))
+        return rows * ((tiles[t, 7] + 1.0) / (1 << hd[t])) ** 2
+
+    c = cost(out)
+    heavy = c * 8 >= c.max()
+    n = work.size
+    bounds = [n * p // 8 for p in range(9)]
+    for p in range(8):
+        h = heavy[bounds[p]:bounds[p + 1]]
+        k = int(h.sum())
+        assert h[:k].all() and not h[k:].any(), "long units open the range"
+        assert np.all(np.diff(c[bounds[p]:bounds[p] + k]) <= 0), "longest first"
+    assert heavy[bounds[:8]].all()
+    # the others in their original order
+    pos = {int(u): i for i, u in enumerate(work)}
+    rest = [pos[int(u)] for u, hv in zip(out, heavy) if not hv]
+    assert rest == sorted(rest)
+    # the quarters of a tile stay adjacent, in order
+    t = out >> 2
+    for i in range(1, n):
+        if t[i] == t[i - 1]:
+            assert (out[i] & 3) == (out[i - 1] & 3) + 1
+    bad = np.zeros(4, np.uint32)
+    assert L.gudni_order_work(tiles.ctypes.data, 1, work.ctypes.data, 4, 256, 8, bad.ctypes.data) != 0

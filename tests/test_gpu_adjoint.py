@@ -39,11 +39,26 @@ def gpu_adjoint(r, sc, target):
         adj.close()
 
 
-def test_adjoint_forward_matches_oracle(rasterizer, adjoint_small):
+def tiled(rasterizer, tile):
+    """The shared context for the pinned tile size, else a context whose
+    device tile tree uses `tile` (the autotuner's smaller sizes)."""
+    if tile == 256:
+        return rasterizer, None
+    r = G.Rasterizer(G.default_spec(max_tile_size=tile))
+    return r, r
+
+
+@pytest.mark.parametrize("tile", [256, 64, 32])
+def test_adjoint_forward_matches_oracle(rasterizer, adjoint_small, tile):
     sc, target = adjoint_small
-    say("adjoint forward vs oracle")
-    loss, dp, dc, img = gpu_adjoint(rasterizer, sc, target)
-    ref = H.oracle_render(sc, want_f64=True)
+    say(f"adjoint forward vs oracle, tile {tile}")
+    r, own = tiled(rasterizer, tile)
+    try:
+        loss, dp, dc, img = gpu_adjoint(r, sc, target)
+    finally:
+        if own:
+            own.close()
+    ref = H.oracle_render(sc, G.default_spec(max_tile_size=tile), want_f64=True)
     err = np.abs(img.astype(np.float64) - ref["f64"].reshape(img.shape)).max()
     assert err <= COVERAGE_TOL, f"forward image differs from the oracle by {err}"
     l64, _, _ = O.diff_loss(sc, sc.pairs.reshape(-1, 4), np.asarray(sc.substances, np.float64), target,
@@ -54,19 +69,26 @@ def test_adjoint_forward_matches_oracle(rasterizer, adjoint_small):
     assert np.all(np.isfinite(dp)) and np.all(np.isfinite(dc))
 
 
-def test_adjoint_gradients_match_finite_differences(rasterizer, adjoint_small):
+@pytest.mark.parametrize("tile", [256, 64, 32])
+def test_adjoint_gradients_match_finite_differences(rasterizer, adjoint_small, tile):
     sc, target = adjoint_small
-    say("adjoint gradients")
-    loss, dp, dc, _ = gpu_adjoint(rasterizer, sc, target)
-    say("finite differences (64 samples, FP64)")
-    samples = A.fd_samples(sc, target, n=64, seed=5)
+    say(f"adjoint gradients, tile {tile}")
+    r, own = tiled(rasterizer, tile)
+    try:
+        loss, dp, dc, _ = gpu_adjoint(r, sc, target)
+    finally:
+        if own:
+            own.close()
+    seg = A.segment_rows(sc, G.default_spec(max_tile_size=tile))
+    say(f"finite differences (64 samples, FP64, {seg}-row column segments)")
+    samples = A.fd_samples(sc, target, n=64, seed=5, seg=seg)
     assert len(samples) == 64
     got, fd, err, bad = A.compare(samples, dp, dc)
     for (kind, i, k, v), g, e, b in zip(samples, got, err, bad):
         say(f"{kind:5s} {i:5d}.{k} fd {v:+.6e} adjoint {g:+.6e} err {e:.2e}{' BAD' if b else ''}")
     assert bad.sum() == 0, f"{bad.sum()} of 64 gradients off"
     # and the same algorithm in FP64 over every parameter
-    _, dp64, dc64 = O.diff_grad(sc, sc.pairs, sc.substances, target)
+    _, dp64, dc64 = O.diff_grad(sc, sc.pairs, sc.substances, target, seg=seg)
     scale_p = np.sqrt(np.mean(dp64 ** 2))
     assert np.abs(dp - dp64).max() <= 2e-2 * scale_p + 0.0, np.abs(dp - dp64).max() / scale_p
     np.testing.assert_allclose(dc, dc64, rtol=1e-3, atol=1e-4 * np.sqrt(np.mean(dc64 ** 2)))

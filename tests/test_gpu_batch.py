@@ -83,7 +83,7 @@ def test_batch_sequential_scenes(batch_and_refs):
     check_all(b, scenes, refs, "sequential")
 
 
-@pytest.mark.parametrize("tile", [64, 128])
+@pytest.mark.parametrize("tile", [32, 64, 128])
 def test_batch_other_tile_sizes(tile):
     """The autotuner's tile sizes: frames equal the oracle's render under the
     same spec (its own tile tree)."""

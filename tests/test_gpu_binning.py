@@ -61,7 +61,7 @@ def test_device_binning_equals_host(rasterizer, name):
     compare_binning(rasterizer, scene, rasterizer.spec)
 
 
-@pytest.mark.parametrize("tile", [64, 128])
+@pytest.mark.parametrize("tile", [32, 64, 128])
 def test_device_binning_other_tile_sizes(tile):
     spec = G.default_spec(max_tile_size=tile)
     r = G.Rasterizer(spec)

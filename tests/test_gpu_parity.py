@@ -188,15 +188,23 @@ def test_blobs_1024(rasterizer):
     say(f"blobs-1024 coverage max err {err:.3g}")
 
 
-def test_blobs_4096_full_frame(rasterizer):
+@pytest.mark.parametrize("tile,filt", [(256, 0), (32, 1)])
+def test_blobs_4096_full_frame(rasterizer, tile, filt):
     """The full 4096^2, 200k-shape workload: every BGRA word equals the
-    oracle's."""
+    oracle's, with the pinned spec and with the benchmark's (32-pixel tiles,
+    bilinear texture sampling)."""
     scene = S.blobs_4096()
-    say("blobs 4096 oracle (16 threads)")
-    ref = H.oracle_render(scene, want_f64=False, nthreads=16)
+    spec = G.default_spec(max_tile_size=tile, texture_filter=filt)
+    say(f"blobs 4096 oracle (16 threads), tile {tile} filter {filt}")
+    ref = H.oracle_render(scene, spec, want_f64=False, nthreads=16)
     assert ref["rc"] == 0
     say("blobs 4096 gpu")
-    u2, _ = H.gpu_render_level2(rasterizer, scene)
+    r = rasterizer if tile == 256 and filt == 0 else G.Rasterizer(spec)
+    try:
+        u2, _ = H.gpu_render_level2(r, scene, spec)
+    finally:
+        if r is not rasterizer:
+            r.close()
     bad = H.first_mismatch(u2, ref["out"])
     assert bad is None, f"blobs 4096 first mismatch {bad}"
 

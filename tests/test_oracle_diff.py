@@ -111,6 +111,50 @@ def test_fp64_adjoint_matches_finite_differences():
     assert not bad.any(), [(s, g) for s, g, b in zip(samples, got, bad) if b]
 
 
+def test_forward_depends_on_column_segments():
+    """Two lines of one shape cross inside the span y in [20, 20.26] of
+    column 12.  A column that starts above the crossing keeps them in the
+    order they became active (the reference's sweep does not reorder active
+    records, Kernels.cl:1007-1124) and takes a section of negative width;
+    a segment starting at y = 20 trims both at its top and sorts them by top
+    x, the geometric order.  That pixel, and only it, differs; every other
+    pixel is the same for 4-, 8- and 32-row segments."""
+    sc, _ = A.case(32, 1, 33)
+    target = np.zeros((32, 32, 4), np.float32)
+    pairs = sc.pairs.reshape(-1, 4).astype(np.float64)
+    rgba = np.asarray(sc.substances, np.float64)
+    _, whole, _ = O.diff_loss(sc, pairs, rgba, target, (0, 0, 32, 32), want_image=True)
+    for seg, differing in ((8, set()), (4, {(12, 20)})):
+        img = np.zeros_like(whole)
+        for y0 in range(0, 32, seg):
+            img[y0:y0 + seg] = O.diff_loss(sc, pairs, rgba, target, (0, y0, 32, y0 + seg), want_image=True)[1]
+        ys, xs = np.nonzero(np.abs(img - whole).max(axis=2) > 1e-12)
+        assert set(zip(xs.tolist(), ys.tolist())) == differing, seg
+    # a negative-width section in the whole column, none in the 4-row segment
+    assert whole[20, 12, 0] < 0.95 < img[20, 12, 0]
+
+
+def test_fp64_adjoint_on_short_segments():
+    """With 4-row column segments (32-pixel tiles rendered by 256 threads)
+    the adjoint's FP64 restatement is the gradient of the segmented forward:
+    central differences of that forward agree on 32 sampled parameters,
+    those of the one-segment forward do not."""
+    sc, target = A.case(256, 150, 41)
+    _, dp, dc = O.diff_grad(sc, sc.pairs, sc.substances, target, seg=4)
+    samples = A.fd_samples(sc, target, n=32, seed=5, seg=4)
+    assert len(samples) == 32
+    got, fd, err, bad = A.compare(samples, dp, dc)
+    assert not bad.any(), [(s, g) for s, g, b in zip(samples, got, bad) if b]
+    whole = A.fd_samples(sc, target, n=32, seed=5)
+    assert A.compare(whole, dp, dc)[3].any()
+
+
+def test_segment_rows_of_the_tile_tree():
+    import gudni_amd as G
+    sc, _ = A.case(256, 150, 41)
+    assert [A.segment_rows(sc, G.default_spec(max_tile_size=t)) for t in (256, 64, 32)] == [128, 16, 4]
+
+
 def test_fp64_adjoint_closed_forms():
     """A translucent square over white: the colour gradient of one pixel is
     2 (pix - t) a per channel; moving the right edge by dx changes the loss by
