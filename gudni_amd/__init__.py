@@ -48,6 +48,9 @@ def _gradients(scene) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(g, np.float32).reshape(-1, 28))
 
 
+FORMAT_BGRA8, FORMAT_RGBA8 = 0, 1
+
+
 def default_spec(**overrides) -> RasterSpec:
     s = RasterSpec()
     _lib.load().gudni_default_spec(C.byref(s))
@@ -334,6 +337,10 @@ class Rasterizer:
     def set_blocks_per_cu(self, n: int):
         """Fast-pass blocks per CU (0 = as many as fit)."""
         self._check(self.L.gudni_set_blocks_per_cu(self.ctx, int(n)))
+
+    def set_output_format(self, fmt: int):
+        """FORMAT_BGRA8 (the reference's pixel word, default) or FORMAT_RGBA8."""
+        self._check(self.L.gudni_set_output_format(self.ctx, int(fmt)))
 
     def enable_timing(self, on=True):
         self._check(self.L.gudni_enable_timing(self.ctx, 1 if on else 0))

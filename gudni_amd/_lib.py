@@ -97,6 +97,7 @@ SIGNATURES = {
     "gudni_frame_check": (C.c_int, [C.c_void_p]),
     "gudni_enable_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "gudni_set_blocks_per_cu": (C.c_int, [C.c_void_p, C.c_int32]),
+    "gudni_set_output_format": (C.c_int, [C.c_void_p, C.c_int32]),
     "gudni_last_timings": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
     "gudni_last_stage_cycles": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
     "gudni_dump_job": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,

@@ -47,6 +47,7 @@ struct gudni_ctx {
     int cus = 0;
     int fast_blocks = 0, slow_blocks = 0;
     int blocks_per_cu = 0;        // fast-pass blocks per CU (0: as many as fit, at most 8)
+    int32_t out_format = GUDNI_FORMAT_BGRA8;   // pixel word byte order
     // common buffers
     gudni::DevBuf geo, subs, pict, uses, random, grads;
     int32_t n_substances = 0, n_uses = 0, n_grads = 0;

@@ -151,6 +151,7 @@ struct Frame {
     int32_t compute_depth, max_shapes, max_thresholds, random_mask;
     float stochastic;
     int32_t coverage_mode;
+    int32_t out_rgba;             // pixel words in R,G,B,A byte order, else the reference's B,G,R,A
     // scratch
     float4 *q_thr;                // [slots][cap][64]
     uint32_t *q_hdr;              // [slots][cap][64]
@@ -945,15 +946,16 @@ __device__ float4 determine_color(const Frame &f, const SceneDesc &s, const Stac
     return base;
 }
 
-__device__ __forceinline__ uint32_t bgra_word(float4 c)
-{   // colorToSolidPixel_Word32_BGRA (Kernels.cl:842-844): rtz, alpha 255
+__device__ __forceinline__ uint32_t pixel_word(float4 c, bool rgba)
+{   // colorToSolidPixel_Word32_BGRA (Kernels.cl:842-844): rtz, alpha 255; the
+    // same bytes in R,G,B,A memory order when `rgba`
     const float ch[3] = {c.z * 255.0f, c.y * 255.0f, c.x * 255.0f};
     uint32_t w = 0xFF000000u;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         const float v = ch[i];
         const uint32_t u = (v == v) ? (v >= 255.0f ? 255u : (v <= 0.0f ? 0u : (uint32_t)v)) : 0u;
-        w |= u << (8 * i);
+        w |= u << (8 * (rgba ? 2 - i : i));
     }
     return w;
 }
@@ -1043,7 +1045,7 @@ __device__ uint32_t render(const Frame &f, const SceneDesc &s, Queue &q, const C
         const float4 color = make_float4(acc.x / area_sum, acc.y / area_sum, acc.z / area_sum,
                                          acc.w / area_sum);
         const long long pix = (long long)(c.dy + yint) * s.width + c.dx;
-        s.out[pix] = bgra_word(color);
+        s.out[pix] = pixel_word(color, f.out_rgba != 0);
         if (s.out_f32) s.out_f32[pix] = color;
         p.sx0 = 0.0f;
         p.sy0 = p.pixel_y;

@@ -233,6 +233,7 @@ dev::Frame base_frame(gudni_ctx *ctx)
     f.random_mask = ctx->spec.random_field_size - 1;
     f.stochastic = ctx->spec.stochastic_factor;
     f.coverage_mode = ctx->spec.coverage_mode;
+    f.out_rgba = ctx->out_format == GUDNI_FORMAT_RGBA8 ? 1 : 0;
     f.threads_per_tile = ctx->spec.threads_per_tile;
     return f;
 }
@@ -955,6 +956,15 @@ int gudni_raster_scene(gudni_ctx *ctx, const gudni_scene *sc, uint32_t *host_out
     return GUDNI_OK;
 }
 
+int gudni_set_output_format(gudni_ctx *ctx, int32_t format)
+{
+    if (!ctx) return GUDNI_ERR_ARG;
+    if (format != GUDNI_FORMAT_BGRA8 && format != GUDNI_FORMAT_RGBA8)
+        return set_error(ctx, GUDNI_ERR_ARG, "gudni_set_output_format: GUDNI_FORMAT_BGRA8 or GUDNI_FORMAT_RGBA8");
+    ctx->out_format = format;
+    return GUDNI_OK;
+}
+
 int gudni_set_blocks_per_cu(gudni_ctx *ctx, int32_t blocks_per_cu)
 {
     if (!ctx) return GUDNI_ERR_ARG;
@@ -1004,6 +1014,7 @@ struct gudni_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     void *captured_control = nullptr;
+    int32_t captured_format = 0;
     hipStream_t capture_stream = nullptr;   // capture needs a stream other than the null stream
 };
 
@@ -1390,7 +1401,7 @@ int gudni_batch_render(gudni_ctx *ctx, gudni_batch *b, int use_graph)
     if (rc) return rc;
     ctx->timed_render = false;
     if (!use_graph) return batch_launch(ctx, b);
-    if (b->exec && b->captured_control != ctx->control.p) {
+    if (b->exec && (b->captured_control != ctx->control.p || b->captured_format != ctx->out_format)) {
         (void)hipGraphExecDestroy(b->exec);
         (void)hipGraphDestroy(b->graph);
         b->exec = nullptr;
@@ -1421,6 +1432,7 @@ int gudni_batch_render(gudni_ctx *ctx, gudni_batch *b, int use_graph)
         b->graph = g;
         CHECK_HIP(ctx, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
         b->captured_control = ctx->control.p;
+        b->captured_format = ctx->out_format;
     }
     CHECK_HIP(ctx, hipGraphLaunch(b->exec, ctx->stream));
     return GUDNI_OK;

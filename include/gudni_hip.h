@@ -351,6 +351,15 @@ int  gudni_adjoint(gudni_ctx *ctx, const gudni_adjoint_scene *scene, double *los
  * latency.  No counterpart in the reference (its launch shape is fixed by
  * the OpenCL runtime). */
 int gudni_set_blocks_per_cu(gudni_ctx *ctx, int32_t blocks_per_cu);
+/* Byte order of the 32-bit pixel words every render of the context writes:
+ * GUDNI_FORMAT_BGRA8 (default) is the reference's word, bytes B, G, R, A in
+ * memory (colorToSolidPixel_Word32_BGRA, Kernels.cl:842-844; the ARGB8888
+ * texture of Interface/InterfaceSDL.hs:118); GUDNI_FORMAT_RGBA8 stores the
+ * same channel values as bytes R, G, B, A (an RGBA8 image).  Alpha is 255
+ * in both.  A batch's captured graph is re-captured when this changes. */
+#define GUDNI_FORMAT_BGRA8 0
+#define GUDNI_FORMAT_RGBA8 1
+int gudni_set_output_format(gudni_ctx *ctx, int32_t format);
 int gudni_enable_timing(gudni_ctx *ctx, int enable);
 int gudni_last_timings(gudni_ctx *ctx, float *ms, int32_t n);
 /* Shader-clock cycles of the last timed render's fast pass, summed over

@@ -292,3 +292,37 @@ def test_gradients_and_texture_filters(filt):
             check_frames(scene, u2, f2, ref)
     finally:
         r.close()
+
+
+def test_rgba8_output_format():
+    """GUDNI_FORMAT_RGBA8 writes the reference's BGRA words' channel bytes in
+    R,G,B,A order, on the whole-frame, per-job and batch paths (the batch's
+    captured graph follows a format change); BGRA stays the oracle's."""
+    import torch
+    scene = textured_scene()
+    ref = H.oracle_render(scene, want_f64=False)
+    assert ref["rc"] == 0
+    bgra = np.asarray(ref["out"], np.uint32)
+    want = (bgra & 0xFF00FF00) | ((bgra >> 16) & 0xFF) | ((bgra & 0xFF) << 16)
+    r = G.Rasterizer()
+    try:
+        say("rgba8: whole frame")
+        r.set_output_format(G.FORMAT_RGBA8)
+        u2, _ = H.gpu_render_level2(r, scene)
+        assert H.first_mismatch(u2, want) is None
+        say("rgba8: per job")
+        u1 = H.gpu_render_level1(r, scene)
+        assert H.first_mismatch(u1, want) is None
+        say("rgba8: batch with graph, format switched between replays")
+        r.set_stream(torch.cuda.current_stream().cuda_stream)
+        b = G.Batch.from_outlines(r, [scene, H.tiny_square()])
+        for fmt, expect in ((G.FORMAT_RGBA8, want), (G.FORMAT_BGRA8, bgra), (G.FORMAT_RGBA8, want)):
+            r.set_output_format(fmt)
+            b.render(use_graph=True)
+            b.check()
+            assert H.first_mismatch(b.frame(0), expect) is None, fmt
+        b.close()
+        with pytest.raises(G.GudniError):
+            r.set_output_format(7)
+    finally:
+        r.close()
