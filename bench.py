@@ -257,7 +257,8 @@ def adjoint_leg(spec, steps, log_fn, fixed=None):
     if fixed:
         (tile, radix), timings = fixed, None
     else:
-        tile, radix, timings = autotune.search(lambda ts, rb: _time_adjoint(_adjoint_spec(spec, ts, rb), items))
+        tile, radix, timings = autotune.search(lambda ts, rb: _time_adjoint(_adjoint_spec(spec, ts, rb), items),
+                                               tiles=autotune.ADJOINT_CANDIDATES)
         log_fn(f"adjoint autotune {json.dumps(timings)} -> tile {tile}, radix {radix}")
     sp = _adjoint_spec(spec, tile, radix)
     r = G.Rasterizer(sp)
@@ -342,7 +343,8 @@ def batched_adjoint_leg(spec, steps, log_fn, fixed=None):
         (tile, radix), timings = fixed, None
     else:
         tile, radix, timings = autotune.search(
-            lambda ts, rb: _time_adjoint(_adjoint_spec(spec, ts, rb), items_of(len(bases)), runs=1))
+            lambda ts, rb: _time_adjoint(_adjoint_spec(spec, ts, rb), items_of(len(bases)), runs=1),
+            tiles=autotune.ADJOINT_CANDIDATES)
         log_fn(f"batched adjoint autotune {json.dumps(timings)} -> tile {tile}, radix {radix}")
     sp = _adjoint_spec(spec, tile, radix)
     r = G.Rasterizer(sp)

@@ -30,6 +30,8 @@ import numpy as np
 from . import Batch, default_spec, prepare
 
 CANDIDATES = (32, 64, 128, 256)
+# the adjoint's scenes are sparse: a 16-pixel tile still has work for every lane
+ADJOINT_CANDIDATES = (16, 32, 64, 128, 256)
 RADIX_CANDIDATES = (0, 3, 5)
 
 

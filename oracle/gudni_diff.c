@@ -400,9 +400,14 @@ static void section_color(const dcol *c, int n_shapes, const dshape *shapes, con
 
 int go_diff_loss(int32_t n_shapes, const uint64_t *tags, const int32_t *soff, const int32_t *ooff,
                  const double *pairs, const double *rgba, const double *bg, int32_t W, int32_t H,
-                 const float *target, int32_t x0, int32_t y0, int32_t x1, int32_t y1,
+                 const float *target, int32_t x0, int32_t y0, int32_t x1, int32_t y1, int32_t seg,
                  double *loss_out, double *image_out, uint64_t *signature)
 {
+    /* seg > 0: columns cut into segments of seg rows at multiples of seg
+     * (the window must start on one and end on one or at the canvas edge);
+     * seg == 0: one segment over the window's rows */
+    if (seg < 0 || (seg > 0 && (y0 % seg != 0 || (y1 % seg != 0 && y1 != H)))) return -3;
+    const int step = seg > 0 ? seg : (y1 - y0 > 0 ? y1 - y0 : 1);
     uint64_t sig = 0xcbf29ce484222325ull;
     dshape *shapes = calloc((size_t)(n_shapes > 0 ? n_shapes : 1), sizeof(dshape));
     for (int i = 0; i < n_shapes; i++) {
@@ -414,11 +419,13 @@ int go_diff_loss(int32_t n_shapes, const uint64_t *tags, const int32_t *soff, co
     c.q = malloc(sizeof(drec) * D_QCAP);
     c.stack = malloc(sizeof(uint64_t) * nwords);
     c.nwords = nwords;
-    c.height_i = y1 - y0; c.fh = (double)(y1 - y0);
     double loss = 0.0;
     int rc = 0;
-    for (int x = x0; x < x1 && rc == 0; x++) {
-        const double dx = x, dy = y0;
+    for (int x = x0; x < x1 && rc == 0; x++)
+    for (int sy = y0; sy < y1 && rc == 0; sy += step) {
+        const int sy1 = sy + step < y1 ? sy + step : y1;
+        c.height_i = sy1 - sy; c.fh = (double)(sy1 - sy);
+        const double dx = x, dy = sy;
         c.qstart = D_QCAP; c.qlen = 0; c.overflow = 0;
         memset(c.stack, 0, sizeof(uint64_t) * nwords);
         /* shapes newest first, as the tile lists hold them */
@@ -486,12 +493,12 @@ int go_diff_loss(int32_t n_shapes, const uint64_t *tags, const int32_t *soff, co
                 sections++;
             }
             sig = fnv(sig, (uint64_t)sections);
-            long pix = (long)(y0 + yi) * W + x;
+            long pix = (long)(sy + yi) * W + x;
             for (int i = 0; i < 4; i++) {
                 double v = acc[i] / area_sum;
                 double e = v - (double)target[4 * pix + i];
                 loss += e * e;
-                if (image_out) image_out[4 * ((long)yi * (x1 - x0) + (x - x0)) + i] = v;
+                if (image_out) image_out[4 * ((long)(sy - y0 + yi) * (x1 - x0) + (x - x0)) + i] = v;
             }
             p.sx0 = 0.0; p.sy0 = p.pixel_y;
         }

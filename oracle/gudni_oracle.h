@@ -139,7 +139,7 @@ void go_default_spec(go_spec *spec);
 int go_diff_loss(int32_t n_shapes, const uint64_t *tags, const int32_t *shape_outline_offsets,
                  const int32_t *outline_pair_offsets, const double *pairs, const double *rgba,
                  const double *background, int32_t width, int32_t height, const float *target,
-                 int32_t x0, int32_t y0, int32_t x1, int32_t y1, double *loss,
+                 int32_t x0, int32_t y0, int32_t x1, int32_t y1, int32_t seg, double *loss,
                  double *image, uint64_t *signature);
 
 /* The adjoint's algorithm restated in FP64 over the whole canvas, columns

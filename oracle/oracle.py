@@ -87,7 +87,7 @@ def lib():
         L.go_diff_loss.restype = C.c_int
         L.go_diff_loss.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
-                                   C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                   C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                    C.POINTER(C.c_double), C.c_void_p, C.POINTER(C.c_uint64)]
         L.go_diff_grad.restype = C.c_int
         L.go_diff_grad.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -276,9 +276,11 @@ def render_jobs(scene, spec, geometry, jobs, job_ids, out, nthreads=1, rnd=None)
     return pixels
 
 
-def diff_loss(scene, pairs, rgba, target, window, want_image=False):
+def diff_loss(scene, pairs, rgba, target, window, want_image=False, seg=0):
     """FP64 forward of the differentiable path (gudni_diff.c) over a window
     (x0, y0, x1, y1) of pixel columns: (loss, image or None, signature).
+    seg > 0 cuts the columns into segments of seg rows at multiples of seg
+    (y0 must be one, y1 one or the canvas height); 0: one segment.
     pairs: (n_pairs, 4) outline points, rgba: (n_substances, 4) solid
     colours, target: (H, W, 4) float32."""
     x0, y0, x1, y1 = [int(v) for v in window]
@@ -294,7 +296,7 @@ def diff_loss(scene, pairs, rgba, target, window, want_image=False):
     sig = C.c_uint64()
     rc = lib().go_diff_loss(scene.n_shapes, _ptr(tags), _ptr(soff), _ptr(ooff), _ptr(pairs),
                             _ptr(rgba), _ptr(bg), scene.width, scene.height, _ptr(target),
-                            x0, y0, x1, y1, C.byref(loss), _ptr(img), C.byref(sig))
+                            x0, y0, x1, y1, int(seg), C.byref(loss), _ptr(img), C.byref(sig))
     if rc != 0:
         raise RuntimeError(f"go_diff_loss failed ({rc})")
     return loss.value, img, sig.value

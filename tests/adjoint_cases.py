@@ -83,12 +83,7 @@ def segmented_loss(scene, pairs, rgba, target, window, seg=None):
     x0, y0, x1, y1 = window
     y0 = (y0 // seg) * seg
     y1 = min(scene.height, -(-y1 // seg) * seg)
-    total, sig = 0.0, []
-    for ys in range(y0, y1, seg):
-        l, _, s = O.diff_loss(scene, pairs, rgba, target, (x0, ys, x1, min(ys + seg, scene.height)))
-        total += l
-        sig.append(s)
-    return total, None, tuple(sig)
+    return O.diff_loss(scene, pairs, rgba, target, (x0, y0, x1, y1), seg=seg)
 
 
 def fd_samples(scene, target, n=64, seed=7, h_point=1e-2, h_color=1e-3, max_tries=600, seg=None):

@@ -48,7 +48,7 @@ def tiled(rasterizer, tile):
     return r, r
 
 
-@pytest.mark.parametrize("tile", [256, 64, 32])
+@pytest.mark.parametrize("tile", [256, 64, 32, 16])
 def test_adjoint_forward_matches_oracle(rasterizer, adjoint_small, tile):
     sc, target = adjoint_small
     say(f"adjoint forward vs oracle, tile {tile}")
@@ -69,7 +69,7 @@ def test_adjoint_forward_matches_oracle(rasterizer, adjoint_small, tile):
     assert np.all(np.isfinite(dp)) and np.all(np.isfinite(dc))
 
 
-@pytest.mark.parametrize("tile", [256, 64, 32])
+@pytest.mark.parametrize("tile", [256, 64, 32, 16])
 def test_adjoint_gradients_match_finite_differences(rasterizer, adjoint_small, tile):
     sc, target = adjoint_small
     say(f"adjoint gradients, tile {tile}")

@@ -130,6 +130,9 @@ def test_forward_depends_on_column_segments():
             img[y0:y0 + seg] = O.diff_loss(sc, pairs, rgba, target, (0, y0, 32, y0 + seg), want_image=True)[1]
         ys, xs = np.nonzero(np.abs(img - whole).max(axis=2) > 1e-12)
         assert set(zip(xs.tolist(), ys.tolist())) == differing, seg
+        # the forward's own segmenting gives the same image in one call
+        np.testing.assert_array_equal(O.diff_loss(sc, pairs, rgba, target, (0, 0, 32, 32), want_image=True,
+                                                  seg=seg)[1], img)
     # a negative-width section in the whole column, none in the 4-row segment
     assert whole[20, 12, 0] < 0.95 < img[20, 12, 0]
 
