@@ -441,7 +441,11 @@ int bin_scene(gudni_ctx *ctx, const BinIn &in, const BinOut &out, BinCounts &cnt
     bp.threads_per_tile = ctx->spec.threads_per_tile;
     bp.width = in.width;
     bp.height = in.height;
-    bp.max_levels = 2 * bp.canvas_depth + 4;
+    // The root is a 2^cd square and the levels halve height and width in
+    // turn; a node whose side in its split direction is 8 or less is a leaf,
+    // so the deepest level is 2 (cd - 3).  One level more for margin (a
+    // deeper tree stops with an error, never silently).
+    bp.max_levels = std::max(2, 2 * (bp.canvas_depth - 3) + 2);
     if (!ctx->bin_cap_pairs) {
         ctx->bin_cap_pairs = std::max(1 << 16, 8 * bp.n_shapes);
         ctx->bin_cap_nodes = std::max(1 << 15, bp.n_shapes / 4);
