@@ -924,7 +924,7 @@ struct gudni_adjoint_batch {
         DevBuf tags, soff, ooff;
         DevBuf geo, shapes, bbox, sbounds, curves;
         DevBuf tiles, tile_job, refs, ref_bounds, work;
-        int32_t n_tiles = 0, n_work = 0;
+        int32_t n_tiles = 0, n_work = 0, n_kept = 0;
         void release()
         {
             DevBuf *bufs[] = {&tags, &soff, &ooff, &geo, &shapes, &bbox, &sbounds, &curves,
@@ -1171,14 +1171,45 @@ int device_rebuild_scene(gudni_ctx *ctx, gudni_adjoint_batch *a, int s)
     SerCounts cnt;
     int rc = serialize_outlines(ctx, in, out, cnt);
     if (rc) return rc;
-    const BinIn bin{(const ShapeRec *)v.shapes.p, (const float4 *)v.bbox.p, (const float4 *)v.sbounds.p,
-                    (int32_t)cnt.n_kept, sc.width, sc.height};
-    const BinOut bo{&v.tiles, &v.tile_job, &v.refs, &v.ref_bounds, &v.work};
-    BinCounts bc;
-    rc = bin_scene(ctx, bin, bo, bc);
+    v.n_kept = (int32_t)cnt.n_kept;
+    return GUDNI_OK;
+}
+
+// Bins the listed scenes' serialised shapes: one scene with the device-wide
+// level launches (a whole GPU on one tree), several in one launch of a block
+// per scene (the level launches would cost more than the work).
+int device_bin_scenes(gudni_ctx *ctx, gudni_adjoint_batch *a, const std::vector<int> &which)
+{
+    const size_t n = which.size();
+    if (n == 1) {
+        const int s = which[0];
+        gudni_adjoint_batch::DevScene &v = a->dev[s];
+        const BinIn in{(const ShapeRec *)v.shapes.p, (const float4 *)v.bbox.p, (const float4 *)v.sbounds.p,
+                       v.n_kept, a->scenes[s].width, a->scenes[s].height};
+        const BinOut out{&v.tiles, &v.tile_job, &v.refs, &v.ref_bounds, &v.work};
+        BinCounts c;
+        const int rc = bin_scene(ctx, in, out, c);
+        if (rc) return rc;
+        v.n_tiles = c.n_tiles;
+        v.n_work = c.n_work;
+        return GUDNI_OK;
+    }
+    std::vector<BinIn> in(n);
+    std::vector<BinOut> out(n);
+    std::vector<BinCounts> cnt(n);
+    for (size_t i = 0; i < n; ++i) {
+        const int s = which[i];
+        gudni_adjoint_batch::DevScene &v = a->dev[s];
+        in[i] = BinIn{(const ShapeRec *)v.shapes.p, (const float4 *)v.bbox.p, (const float4 *)v.sbounds.p,
+                      v.n_kept, a->scenes[s].width, a->scenes[s].height};
+        out[i] = BinOut{&v.tiles, &v.tile_job, &v.refs, &v.ref_bounds, &v.work};
+    }
+    const int rc = bin_scenes(ctx, in.data(), out.data(), cnt.data(), (int)n);
     if (rc) return rc;
-    v.n_tiles = bc.n_tiles;
-    v.n_work = bc.n_work;
+    for (size_t i = 0; i < n; ++i) {
+        a->dev[which[i]].n_tiles = cnt[i].n_tiles;
+        a->dev[which[i]].n_work = cnt[i].n_work;
+    }
     return GUDNI_OK;
 }
 
@@ -1232,12 +1263,15 @@ int device_update(gudni_ctx *ctx, gudni_adjoint_batch *a, int scene)
     // a launch may still read the per-scene buffers the rebuild can regrow
     CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     int rc = GUDNI_OK;
+    std::vector<int> which;
     if (a->dev.empty() || scene < 0) {
         a->dev.resize(a->scenes.size());
-        for (int s = 0; s < (int)a->scenes.size() && rc == GUDNI_OK; ++s) rc = device_rebuild_scene(ctx, a, s);
+        for (int s = 0; s < (int)a->scenes.size(); ++s) which.push_back(s);
     } else {
-        rc = device_rebuild_scene(ctx, a, scene);
+        which.push_back(scene);
     }
+    for (size_t i = 0; i < which.size() && rc == GUDNI_OK; ++i) rc = device_rebuild_scene(ctx, a, which[i]);
+    if (rc == GUDNI_OK) rc = device_bin_scenes(ctx, a, which);
     if (rc) {
         // the parameters on the device no longer match any complete set of
         // tables: the batch refuses to run until it is uploaded again

@@ -59,5 +59,23 @@ hipError_t enqueue(const Params &p, const Workspace &w, const ShapeRec *shapes,
                    const float4 *bbox, const float4 *shape_bounds, const Output &out,
                    hipStream_t stream);
 
+// One scene of a many-scene binning: its parameters, its own workspace and
+// outputs.
+struct SceneJob {
+    Params p;
+    Workspace w;
+    const ShapeRec *shapes;
+    const float4 *bbox, *shape_bounds;
+    Output out;
+};
+
+constexpr int kSceneBlockThreads = 512;
+
+// Bins n scenes in one launch, one block per scene: every level's steps and
+// scans run in the block between barriers, so a small scene costs one
+// block's sweep instead of a dozen launches per level.  `jobs` is in device
+// memory; each job's status[] holds its counts once the stream reaches it.
+hipError_t enqueue_scenes(const SceneJob *jobs, int n, hipStream_t stream);
+
 }  // namespace binning
 }  // namespace gudni

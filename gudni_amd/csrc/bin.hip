@@ -125,18 +125,28 @@ struct Dev {
     int32_t cap_pairs, cap_nodes;
 };
 
+__device__ __forceinline__ int ceil_log2_dev(int x)
+{
+    int k = 0;
+    while ((1 << k) < x) ++k;
+    return k;
+}
+
 __device__ __forceinline__ int gtid() { return (int)(blockIdx.x * blockDim.x + threadIdx.x); }
 __device__ __forceinline__ int gstride() { return (int)(gridDim.x * blockDim.x); }
 
-__global__ void __launch_bounds__(kThreads) bin_init(Dev d)
+// Each step is a device function over (first thread, stride): the grid-stride
+// kernels below call them once per launch, and the one-block-per-scene kernel
+// calls them all in turn with block barriers in between.
+__device__ void init_body(const Dev &d, int t0, int ts)
 {
-    for (int i = gtid(); i < d.p.n_shapes; i += gstride()) {
+    for (int i = t0; i < d.p.n_shapes; i += ts) {
         d.pair_shape[i] = i;
         d.pair_node[i] = 0;
     }
-    for (int l = gtid(); l < d.p.max_levels; l += gstride())
+    for (int l = t0; l < d.p.max_levels; l += ts)
         d.levels[l] = l == 0 ? make_int4(0, 1, 0, d.p.n_shapes) : make_int4(0, 0, 0, 0);
-    if (gtid() == 0) {
+    if (t0 == 0) {
         const int side = 1 << d.p.canvas_depth;
         const int cd = d.p.canvas_depth;
         d.node_box[0] = make_int4(0, 0, side, side);
@@ -148,11 +158,13 @@ __global__ void __launch_bounds__(kThreads) bin_init(Dev d)
     }
 }
 
-__global__ void __launch_bounds__(kThreads) bin_decide(Dev d, const ShapeRec *shapes, int level)
+__global__ void __launch_bounds__(kThreads) bin_init(Dev d) { init_body(d, gtid(), gstride()); }
+
+__device__ void decide_body(const Dev &d, const ShapeRec *shapes, int level, int t0, int ts)
 {
     if (d.status[0]) return;
     const int4 lv = d.levels[level];
-    for (int i = gtid(); i < lv.y; i += gstride()) {
+    for (int i = t0; i < lv.y; i += ts) {
         const int n = lv.x + i;
         const int4 m = d.node_meta[n];
         uint32_t flags = (uint32_t)m.y;
@@ -178,8 +190,13 @@ __global__ void __launch_bounds__(kThreads) bin_decide(Dev d, const ShapeRec *sh
     }
 }
 
+__global__ void __launch_bounds__(kThreads) bin_decide(Dev d, const ShapeRec *shapes, int level)
+{
+    decide_body(d, shapes, level, gtid(), gstride());
+}
+
 // Next level's extent from the two scans; checks the capacities.
-__global__ void bin_advance(Dev d, int level)
+__device__ void advance_body(const Dev &d, int level)
 {
     if (d.status[0]) return;
     const int4 lv = d.levels[level];
@@ -198,12 +215,14 @@ __global__ void bin_advance(Dev d, int level)
     d.levels[level + 1] = make_int4(nb, nc, pb, pc);
 }
 
-__global__ void __launch_bounds__(kThreads) bin_children(Dev d, int level)
+__global__ void bin_advance(Dev d, int level) { advance_body(d, level); }
+
+__device__ void children_body(const Dev &d, int level, int t0, int ts)
 {
     if (d.status[0]) return;
     const int4 lv = d.levels[level];
     const int nb = lv.x + lv.y, pb = lv.z + lv.w;
-    for (int i = gtid(); i < lv.y; i += gstride()) {
+    for (int i = t0; i < lv.y; i += ts) {
         const int n = lv.x + i;
         const int4 m = d.node_meta[n];
         const uint32_t flags = (uint32_t)m.y;
@@ -251,11 +270,13 @@ __global__ void __launch_bounds__(kThreads) bin_children(Dev d, int level)
     }
 }
 
-__global__ void __launch_bounds__(kThreads) bin_scatter(Dev d, const float4 *bbox, int level)
+__global__ void __launch_bounds__(kThreads) bin_children(Dev d, int level) { children_body(d, level, gtid(), gstride()); }
+
+__device__ void scatter_body(const Dev &d, const float4 *bbox, int level, int t0, int ts)
 {
     if (d.status[0]) return;
     const int4 lv = d.levels[level];
-    for (int i = gtid(); i < lv.w; i += gstride()) {
+    for (int i = t0; i < lv.w; i += ts) {
         const int p = lv.z + i;
         const int n = d.pair_node[p];
         const int4 m = d.node_meta[n];
@@ -278,11 +299,16 @@ __global__ void __launch_bounds__(kThreads) bin_scatter(Dev d, const float4 *bbo
     }
 }
 
-__global__ void __launch_bounds__(kThreads) bin_count_up(Dev d, int level)
+__global__ void __launch_bounds__(kThreads) bin_scatter(Dev d, const float4 *bbox, int level)
+{
+    scatter_body(d, bbox, level, gtid(), gstride());
+}
+
+__device__ void count_up_body(const Dev &d, int level, int t0, int ts)
 {
     if (d.status[0]) return;
     const int4 lv = d.levels[level];
-    for (int i = gtid(); i < lv.y; i += gstride()) {
+    for (int i = t0; i < lv.y; i += ts) {
         const int n = lv.x + i;
         const int c = d.node_child[n];
         if (c < 0) {
@@ -295,11 +321,13 @@ __global__ void __launch_bounds__(kThreads) bin_count_up(Dev d, int level)
     }
 }
 
-__global__ void __launch_bounds__(kThreads) bin_offsets_down(Dev d, int level)
+__global__ void __launch_bounds__(kThreads) bin_count_up(Dev d, int level) { count_up_body(d, level, gtid(), gstride()); }
+
+__device__ void offsets_down_body(const Dev &d, int level, int t0, int ts)
 {
     if (d.status[0]) return;
     const int4 lv = d.levels[level];
-    for (int i = gtid(); i < lv.y; i += gstride()) {
+    for (int i = t0; i < lv.y; i += ts) {
         const int n = lv.x + i;
         const int c = d.node_child[n];
         if (c < 0) continue;
@@ -310,7 +338,12 @@ __global__ void __launch_bounds__(kThreads) bin_offsets_down(Dev d, int level)
     }
 }
 
-__global__ void bin_finish(Dev d)
+__global__ void __launch_bounds__(kThreads) bin_offsets_down(Dev d, int level)
+{
+    offsets_down_body(d, level, gtid(), gstride());
+}
+
+__device__ void finish_body(const Dev &d)
 {
     if (d.status[0]) return;
     int tn = 0, tp = 0;
@@ -325,13 +358,15 @@ __global__ void bin_finish(Dev d)
     d.status[5] = (uint32_t)tp;
 }
 
-__global__ void __launch_bounds__(kThreads) bin_emit_tiles(Dev d, TileRec *tiles, int32_t *tile_job)
+__global__ void bin_finish(Dev d) { finish_body(d); }
+
+__device__ void emit_tiles_body(const Dev &d, TileRec *tiles, int32_t *tile_job, int t0, int ts)
 {
     if (d.status[0]) return;
     const int tn = (int)d.status[4];
     const int n_tiles = (int)d.status[1];
     const int n_jobs = (n_tiles + d.p.per_job - 1) / d.p.per_job;
-    for (int n = gtid(); n < tn; n += gstride()) {
+    for (int n = t0; n < tn; n += ts) {
         if (d.node_child[n] >= 0) continue;
         const int t = d.node_toff[n];
         const int4 b = d.node_box[n];
@@ -348,13 +383,17 @@ __global__ void __launch_bounds__(kThreads) bin_emit_tiles(Dev d, TileRec *tiles
     }
 }
 
-__global__ void __launch_bounds__(kThreads) bin_emit_refs(Dev d, const ShapeRec *shapes,
-                                                          const float4 *shape_bounds, ShapeRec *refs,
-                                                          float4 *ref_bounds)
+__global__ void __launch_bounds__(kThreads) bin_emit_tiles(Dev d, TileRec *tiles, int32_t *tile_job)
+{
+    emit_tiles_body(d, tiles, tile_job, gtid(), gstride());
+}
+
+__device__ void emit_refs_body(const Dev &d, const ShapeRec *shapes, const float4 *shape_bounds,
+                               ShapeRec *refs, float4 *ref_bounds, int t0, int ts)
 {
     if (d.status[0]) return;
     const int tp = (int)d.status[5];
-    for (int p = gtid(); p < tp; p += gstride()) {
+    for (int p = t0; p < tp; p += ts) {
         const int n = d.pair_node[p];
         if (d.node_child[n] >= 0) continue;
         const int4 m = d.node_meta[n];
@@ -365,18 +404,97 @@ __global__ void __launch_bounds__(kThreads) bin_emit_refs(Dev d, const ShapeRec 
     }
 }
 
-__global__ void __launch_bounds__(kThreads) bin_emit_work(Dev d, WorkFlags f, uint32_t *work)
+__global__ void __launch_bounds__(kThreads) bin_emit_refs(Dev d, const ShapeRec *shapes,
+                                                          const float4 *shape_bounds, ShapeRec *refs,
+                                                          float4 *ref_bounds)
+{
+    emit_refs_body(d, shapes, shape_bounds, refs, ref_bounds, gtid(), gstride());
+}
+
+__device__ void emit_work_body(const Dev &d, WorkFlags f, uint32_t *work, int t0, int ts)
 {
     if (d.status[0]) return;
     f.load();
     const int n = f.count();
-    for (int t = gtid(); t < n; t += gstride()) {
+    for (int t = t0; t < n; t += ts) {
         const TileRec r = f.tiles[t];
         uint32_t k = (uint32_t)d.scan[t];
         for (int q = 0; q < f.quarters; ++q)
             if (quarter_active(r, q, f.cd, f.width, f.height)) work[k++] = ((uint32_t)t << 2) | (uint32_t)q;
     }
-    if (gtid() == 0) d.status[3] = (uint32_t)d.scan[n];
+    if (t0 == 0) d.status[3] = (uint32_t)d.scan[n];
+}
+
+__global__ void __launch_bounds__(kThreads) bin_emit_work(Dev d, WorkFlags f, uint32_t *work)
+{
+    emit_work_body(d, f, work, gtid(), gstride());
+}
+
+// One scene per block: the steps of enqueue() in order, with block barriers
+// for the kernel boundaries and block scans for the device-wide ones.
+__global__ void __launch_bounds__(kSceneBlockThreads) bin_scene_block(const SceneJob *jobs)
+{
+    const SceneJob &j = jobs[blockIdx.x];
+    Dev d;
+    d.p = j.p;
+    d.pair_shape = j.w.pair_shape;
+    d.pair_node = j.w.pair_node;
+    d.scan = j.w.scan;
+    d.node_scan = j.w.scan + (((size_t)8 * ((size_t)j.w.cap_pairs + 1) + 255) & ~(size_t)255) / 8;
+    d.node_box = j.w.node_box;
+    d.node_meta = j.w.node_meta;
+    d.node_child = j.w.node_child;
+    d.node_leaves = j.w.node_leaves;
+    d.node_refs = j.w.node_refs;
+    d.node_toff = j.w.node_toff;
+    d.node_roff = j.w.node_roff;
+    d.levels = j.w.levels;
+    d.status = j.w.status;
+    d.cap_pairs = j.w.cap_pairs;
+    d.cap_nodes = j.w.cap_nodes;
+    const int t0 = (int)threadIdx.x, ts = (int)blockDim.x;
+    init_body(d, t0, ts);
+    __syncthreads();
+    int depth = d.p.max_levels;
+    for (int l = 0; l < d.p.max_levels; ++l) {
+        decide_body(d, j.shapes, l, t0, ts);
+        __syncthreads();
+        devscan::block_range_excl_scan(SplitFlags{d.levels, d.node_meta, l, 0, 0}, d.node_scan);
+        __syncthreads();
+        devscan::block_range_excl_scan(
+            PairFlags{d.levels, d.node_meta, d.node_box, d.pair_shape, d.pair_node, j.bbox, l, 0, 0}, d.scan);
+        __syncthreads();
+        if (t0 == 0) advance_body(d, l);
+        __syncthreads();
+        children_body(d, l, t0, ts);
+        __syncthreads();
+        scatter_body(d, j.bbox, l, t0, ts);
+        __syncthreads();
+        // the levels below a level without nodes are empty: their steps
+        // would do nothing (the same for every thread of the block)
+        if (d.status[0] || l + 1 >= d.p.max_levels || d.levels[l + 1].y == 0) {
+            depth = l + 1;
+            break;
+        }
+    }
+    for (int l = depth - 1; l >= 0; --l) {
+        count_up_body(d, l, t0, ts);
+        __syncthreads();
+    }
+    for (int l = 0; l < depth; ++l) {
+        offsets_down_body(d, l, t0, ts);
+        __syncthreads();
+    }
+    if (t0 == 0) finish_body(d);
+    __syncthreads();
+    emit_tiles_body(d, j.out.tiles, j.out.tile_job, t0, ts);
+    emit_refs_body(d, j.shapes, j.shape_bounds, j.out.refs, j.out.ref_bounds, t0, ts);
+    __syncthreads();
+    WorkFlags wf{j.out.tiles, d.status, ceil_log2_dev(d.p.threads_per_tile), d.p.threads_per_tile / 64,
+                 d.p.width, d.p.height, 0};
+    devscan::block_range_excl_scan(wf, d.scan);
+    __syncthreads();
+    emit_work_body(d, wf, j.out.work, t0, ts);
 }
 
 }  // namespace
@@ -470,6 +588,13 @@ hipError_t enqueue(const Params &p, const Workspace &w, const ShapeRec *shapes, 
                  p.width, p.height, 0};
     devscan::excl_scan(wf, w.partials, w.scan, stream);
     hipLaunchKernelGGL(bin_emit_work, g, b, 0, stream, d, wf, out.work);
+    return hipGetLastError();
+}
+
+hipError_t enqueue_scenes(const SceneJob *jobs, int n, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(bin_scene_block, dim3(n), dim3(kSceneBlockThreads), 0, stream, jobs);
     return hipGetLastError();
 }
 

@@ -70,6 +70,7 @@ struct gudni_ctx {
     bool frame_ready = false;
     // Level 2 scene: shapes in traversal order, binned on the device
     gudni::DevBuf scene_shapes, scene_bbox, scene_bounds, bin_ws;
+    gudni::DevBuf bins_ws, bins_jobs;   // many-scene binning: workspaces, job records
     int32_t scene_n_shapes = 0, scene_w = 0, scene_h = 0;
     float scene_bg[4] = {1, 1, 1, 1};
     int32_t bin_cap_pairs = 0, bin_cap_nodes = 0, scene_n_refs = 0;
@@ -164,6 +165,10 @@ struct BinCounts {
     int32_t n_tiles = 0, n_refs = 0, n_work = 0;
 };
 int bin_scene(gudni_ctx *ctx, const BinIn &in, const BinOut &out, BinCounts &cnt);
+// The same for n scenes at once, one block per scene (for scenes of a few
+// thousand shapes, where the per-level launches of bin_scene cost more than
+// the work): one launch and one synchronisation while the capacities hold.
+int bin_scenes(gudni_ctx *ctx, const BinIn *in, const BinOut *out, BinCounts *cnt, int n);
 // Bounds-checks shape records (and tiles) against the common buffers and
 // fills each shape's strand extent (xmin, ymin, xmax, ymax).
 int validate_job(gudni_ctx *ctx, const CommonView &cv, const ShapeRec *shapes, int32_t n_shapes,

@@ -485,6 +485,90 @@ int bin_scene(gudni_ctx *ctx, const BinIn &in, const BinOut &out, BinCounts &cnt
     return GUDNI_OK;
 }
 
+int bin_scenes(gudni_ctx *ctx, const BinIn *in, const BinOut *out, BinCounts *cnt, int n)
+{
+    if (n <= 0) return GUDNI_OK;
+    std::vector<binning::Params> params((size_t)n);
+    std::vector<int32_t> cap_pairs((size_t)n), cap_nodes((size_t)n);
+    for (int s = 0; s < n; ++s) {
+        binning::Params &bp = params[s];
+        bp.n_shapes = in[s].n_shapes;
+        bp.canvas_depth = ceil_log2(std::max(in[s].width, in[s].height));
+        bp.tile_depth = ceil_log2(in[s].max_tile_size > 0 ? in[s].max_tile_size : ctx->spec.max_tile_size);
+        bp.cap_shapes = ctx->spec.max_shapes - 1;
+        bp.cap_strands = (uint32_t)ctx->spec.max_strands_per_tile;
+        bp.per_job = ctx->spec.threads_per_tile;
+        bp.col_step = ctx->spec.max_tiles_per_call;
+        bp.threads_per_tile = ctx->spec.threads_per_tile;
+        bp.width = in[s].width;
+        bp.height = in[s].height;
+        bp.max_levels = std::max(2, 2 * (bp.canvas_depth - 3) + 2);
+        // every node of a tree down to the 8-pixel floor fits
+        const int64_t cells = (int64_t)std::max(1, (1 << bp.canvas_depth) / 8);
+        cap_nodes[s] = (int32_t)std::min<int64_t>(2 * cells * cells + 16, 1 << 26);
+        // the work list's scan reuses the pair scan's words: room for a
+        // count per tile, and tiles never outnumber nodes
+        cap_pairs[s] = std::max(std::max(1 << 14, 8 * bp.n_shapes), cap_nodes[s]);
+    }
+    std::vector<int> todo((size_t)n);
+    for (int s = 0; s < n; ++s) todo[s] = s;
+    for (int attempt = 0; !todo.empty(); ++attempt) {
+        if (attempt > 8) return set_error(ctx, GUDNI_ERR_ARG, "scene binning: capacities keep overflowing");
+        const size_t m = todo.size();
+        // workspaces after one status array of 8 words per scene
+        std::vector<size_t> off(m);
+        size_t total = (32 * m + 255) & ~(size_t)255;
+        for (size_t i = 0; i < m; ++i) {
+            const int s = todo[i];
+            off[i] = total;
+            total += binning::workspace_bytes(cap_pairs[s], cap_nodes[s], params[s].max_levels);
+        }
+        CHECK_HIP(ctx, ctx->bins_ws.ensure(total));
+        std::vector<binning::SceneJob> jobs(m);
+        for (size_t i = 0; i < m; ++i) {
+            const int s = todo[i];
+            const BinOut &o = out[s];
+            CHECK_HIP(ctx, o.tiles->ensure(32u * (size_t)cap_nodes[s]));
+            CHECK_HIP(ctx, o.tile_job->ensure(4u * (size_t)cap_nodes[s]));
+            CHECK_HIP(ctx, o.refs->ensure(16u * (size_t)cap_pairs[s]));
+            CHECK_HIP(ctx, o.ref_bounds->ensure(16u * (size_t)cap_pairs[s]));
+            CHECK_HIP(ctx, o.work->ensure(16u * (size_t)cap_nodes[s]));
+            binning::SceneJob &j = jobs[i];
+            j.p = params[s];
+            binning::carve(j.w, (uint8_t *)ctx->bins_ws.p + off[i], cap_pairs[s], cap_nodes[s], params[s].max_levels);
+            j.w.status = (uint32_t *)ctx->bins_ws.p + 8 * i;
+            j.shapes = in[s].shapes;
+            j.bbox = in[s].bbox;
+            j.shape_bounds = in[s].bounds;
+            j.out = binning::Output{(TileRec *)o.tiles->p, (int32_t *)o.tile_job->p, (ShapeRec *)o.refs->p,
+                                    (float4 *)o.ref_bounds->p, (uint32_t *)o.work->p};
+        }
+        CHECK_HIP(ctx, upload(ctx, ctx->bins_jobs, jobs.data(), sizeof(binning::SceneJob) * m));
+        CHECK_HIP(ctx, binning::enqueue_scenes((const binning::SceneJob *)ctx->bins_jobs.p, (int)m, ctx->stream));
+        std::vector<uint32_t> status(8 * m);
+        CHECK_HIP(ctx, hipMemcpyAsync(status.data(), ctx->bins_ws.p, 4 * status.size(), hipMemcpyDeviceToHost,
+                                      ctx->stream));
+        CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        std::vector<int> again;
+        for (size_t i = 0; i < m; ++i) {
+            const int s = todo[i];
+            const uint32_t *st = status.data() + 8 * i;
+            if (st[0] == 0) {
+                cnt[s].n_tiles = (int32_t)st[1];
+                cnt[s].n_refs = (int32_t)st[2];
+                cnt[s].n_work = (int32_t)st[3];
+            } else if (st[0] == 1) {
+                cap_pairs[s] = (int32_t)std::min<int64_t>(4 * (int64_t)cap_pairs[s], 1 << 28);
+                again.push_back(s);
+            } else {
+                return set_error(ctx, GUDNI_ERR_ARG, "scene binning: tile tree deeper than the canvas allows");
+            }
+        }
+        todo.swap(again);
+    }
+    return GUDNI_OK;
+}
+
 } // namespace gudni
 
 namespace {

@@ -84,6 +84,26 @@ __global__ void __launch_bounds__(kScanThreads) scan_down(F f, const uint64_t *p
     if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = partials[kScanBlocks];
 }
 
+// The same scan by one block over the whole range (a kernel that handles a
+// problem per block): out[i] for i < n and out[n] = the total.  Every thread
+// of the block calls it.
+template <class F>
+__device__ void block_range_excl_scan(F f, uint64_t *out)
+{
+    f.load();
+    const int n = f.count();
+    uint64_t carry = 0;
+    for (int base = 0; base < n; base += (int)blockDim.x) {
+        const int i = base + (int)threadIdx.x;
+        const uint64_t v = i < n ? f(i) : 0;
+        uint64_t total;
+        const uint64_t ex = block_excl_scan(v, total);
+        if (i < n) out[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) out[n] = carry;
+}
+
 // out[i] = sum of f(j) for j < i, out[n] = the total; partials holds
 // kScanBlocks + 1 words.
 template <class F>
