@@ -114,9 +114,11 @@ def test_batch_from_outlines_equals_host_batch():
     import torch
     from tests.test_gpu_binning import empty_scene
     from tests.test_gpu_serialize import awkward_scene
-    scenes = batch_scenes() + [empty_scene(), awkward_scene(with_nan=False), S.blobs_scaled(1024, 12500, 7)]
-    tiles = [256, 64, 128, 256, 64, 128, 256, 32, 128, 64, 256]
-    radix = [0, 3, 5, 4, 0, 3, 5, 2, 1, 0, 3]
+    # scenes from empty to 1400^2 with 20,000 shapes
+    scenes = batch_scenes() + [empty_scene(), awkward_scene(with_nan=False), S.blobs_scaled(1024, 12500, 7),
+                               S.blobs_scaled(1400, 20000, 9)]
+    tiles = [256, 64, 128, 256, 64, 128, 256, 32, 128, 64, 256, 64]
+    radix = [0, 3, 5, 4, 0, 3, 5, 2, 1, 0, 3, 3]
     assert len(tiles) == len(radix) == len(scenes)
     r = G.Rasterizer()
     try:
