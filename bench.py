@@ -634,6 +634,18 @@ def main():
     gpix = npix / (ms * 1e-3) / 1e9
     mcurves = scene.n_curves / (ms * 1e-3) / 1e6
 
+    # Level 1, the reference's generateCall boundary: every raster job with
+    # its piles in host memory and the pixels back in a host frame
+    # (PCIe-inclusive; not the bench value)
+    log("Level 1 (per job, host buffers) of the same frame")
+    host_frame = np.zeros(npix, np.uint32)
+    t = time.perf_counter()
+    r.queue_jobs(jobs, scene, host_frame)
+    t_level1 = time.perf_counter() - t
+    level1 = {"ms": t_level1 * 1e3, "gpixels_per_s": npix / t_level1 / 1e9, "jobs": jobs.n_jobs,
+              "frame_equals_device_frame": bool(np.array_equal(host_frame, out.cpu().numpy().view(np.uint32)))}
+    del host_frame
+
     log("device binning (stage a) of the same frame")
     binning = device_binning_leg(r, scene, geo, stream, max(3, min(args.steps, 10)))
     traffic, traffic_info = measured_traffic(fast_ms)
@@ -682,6 +694,7 @@ def main():
                      "frac_with_256_px_tile_bytes": b_alg_pinned / (fast_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "peak_measured_copy_gbs": copy_gbs, "frac_of_measured_copy": achieved / copy_gbs,
                      "stage_cycle_share": stage_share},
+        "level1_host_buffers": level1,
         "device_binning": binning,
         "device_pipeline": pipeline,
         "batched": batched,

@@ -762,8 +762,21 @@ static int run_job(gudni_ctx *ctx, const void *shapes, int32_t n_shapes, const v
     CHECK_HIP(ctx, upload(ctx, ctx->work, work.data(), 4u * work.size()));
     const size_t npix = (size_t)w * (size_t)h;
     CHECK_HIP(ctx, ctx->out.ensure(4 * npix));
-    if (host_out)
-        CHECK_HIP(ctx, hipMemcpyAsync(ctx->out.p, host_out, 4 * npix, hipMemcpyHostToDevice, ctx->stream));
+    // the pixels a job can write lie in the rows its tiles span (clipped to
+    // the canvas): only those rows of the host frame travel, both ways and in
+    // one piece each, so every other pixel stays as it was
+    int32_t by0 = h, by1 = 0;
+    for (int32_t t = 0; t < n_tiles; ++t) {
+        const TileRec &r = ((const TileRec *)tiles)[t];
+        if (r.box[0] >= w || r.box[2] <= 0) continue;
+        by0 = std::min(by0, std::max(0, r.box[1]));
+        by1 = std::max(by1, std::min(h, r.box[3]));
+    }
+    const bool window = host_out && by0 < by1;
+    const size_t win_off = 4 * (size_t)by0 * (size_t)w, win_bytes = 4 * (size_t)(by1 - by0) * (size_t)w;
+    if (window)
+        CHECK_HIP(ctx, hipMemcpyAsync((uint8_t *)ctx->out.p + win_off, (const uint8_t *)host_out + win_off, win_bytes,
+                                      hipMemcpyHostToDevice, ctx->stream));
     dev::Frame f = base_frame(ctx);
     f.tiles = (const TileRec *)ctx->tiles.p;
     f.tile_job = (const int32_t *)ctx->tile_job.p;
@@ -807,8 +820,9 @@ static int run_job(gudni_ctx *ctx, const void *shapes, int32_t n_shapes, const v
     if (rc) return rc;
     rc = read_status(ctx);
     if (rc) return rc;
-    if (host_out)
-        CHECK_HIP(ctx, hipMemcpyAsync(host_out, ctx->out.p, 4 * npix, hipMemcpyDeviceToHost, ctx->stream));
+    if (window)
+        CHECK_HIP(ctx, hipMemcpyAsync((uint8_t *)host_out + win_off, (const uint8_t *)ctx->out.p + win_off, win_bytes,
+                                      hipMemcpyDeviceToHost, ctx->stream));
     if (dump) {
         std::vector<uint8_t> hb(total);
         CHECK_HIP(ctx, hipMemcpyAsync(hb.data(), ctx->dump.p, total, hipMemcpyDeviceToHost, ctx->stream));

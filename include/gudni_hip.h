@@ -139,7 +139,8 @@ int  gudni_upload_gradients(gudni_ctx *ctx, const void *gradients, int32_t n_gra
 /* Replaces generateCall (OpenCL/CallKernels.hs:88-179): runs the
  * generateThresholds / sortThresholds / renderThresholds work of one
  * RasterJob (Raster/Job.hs:72-77) and writes the job's pixels into `out`
- * (host memory, width*height words; other pixels are left untouched).
+ * (host memory, width*height words; other pixels are left untouched: only
+ * the rows the job's tiles span are copied to the device and back).
  * Synchronous, like the reference's KernelSync launches. */
 int gudni_raster_job(gudni_ctx *ctx,
                      const void *shapes, int32_t n_shapes,

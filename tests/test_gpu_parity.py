@@ -326,3 +326,27 @@ def test_rgba8_output_format():
             r.set_output_format(7)
     finally:
         r.close()
+
+
+def test_level1_job_touches_only_its_tiles(rasterizer):
+    """gudni_raster_job writes a job's pixels into the host frame and leaves
+    every other pixel as it was (it moves only the window of the job's
+    tiles): one job of the 512^2 uniform scene into a frame of sentinels."""
+    scene = S.uniform_512()
+    _, geo, jobs = G.prepare(scene)
+    rasterizer.upload_common(geo.geo, scene)
+    ref = H.oracle_render(scene, want_f64=False)["out"]
+    sentinel = np.uint32(0xDEADBEEF)
+    for j in (0, jobs.n_jobs // 2, jobs.n_jobs - 1):
+        tiles, shapes = jobs.job(j)
+        say(f"level1 single job {j} of {jobs.n_jobs}")
+        out = np.full(scene.width * scene.height, sentinel, np.uint32)
+        rasterizer.raster_job(shapes, tiles, scene.background, scene.width, scene.height, 0,
+                              jobs.n_jobs - 1 - j, out)
+        frame = out.reshape(scene.height, scene.width)
+        mine = np.zeros(frame.shape, bool)
+        for b in np.frombuffer(tiles.tobytes(), np.int32).reshape(-1, 8)[:, :4]:
+            mine[max(0, b[1]):min(scene.height, b[3]), max(0, b[0]):min(scene.width, b[2])] = True
+        assert mine.any()
+        assert np.array_equal(frame[mine], ref[mine]), f"job {j}: its pixels differ from the oracle"
+        assert np.all(frame[~mine] == sentinel), f"job {j}: pixels outside its tiles changed"
