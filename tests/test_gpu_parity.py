@@ -188,18 +188,19 @@ def test_blobs_1024(rasterizer):
     say(f"blobs-1024 coverage max err {err:.3g}")
 
 
-@pytest.mark.parametrize("tile,filt", [(256, 0), (32, 1)])
-def test_blobs_4096_full_frame(rasterizer, tile, filt):
+@pytest.mark.parametrize("tile,filt,mode", [(256, 0, 0), (32, 1, 0), (256, 0, 1)])
+def test_blobs_4096_full_frame(rasterizer, tile, filt, mode):
     """The full 4096^2, 200k-shape workload: every BGRA word equals the
-    oracle's, with the pinned spec and with the benchmark's (32-pixel tiles,
-    bilinear texture sampling)."""
+    oracle's, with the pinned spec, with the benchmark's (32-pixel tiles,
+    bilinear texture sampling), and in the reference's own arithmetic
+    (coverage mode REFERENCE)."""
     scene = S.blobs_4096()
-    spec = G.default_spec(max_tile_size=tile, texture_filter=filt)
-    say(f"blobs 4096 oracle (16 threads), tile {tile} filter {filt}")
+    spec = G.default_spec(max_tile_size=tile, texture_filter=filt, coverage_mode=mode)
+    say(f"blobs 4096 oracle (16 threads), tile {tile} filter {filt} mode {mode}")
     ref = H.oracle_render(scene, spec, want_f64=False, nthreads=16)
     assert ref["rc"] == 0
     say("blobs 4096 gpu")
-    r = rasterizer if tile == 256 and filt == 0 else G.Rasterizer(spec)
+    r = rasterizer if tile == 256 and filt == 0 and mode == 0 else G.Rasterizer(spec)
     try:
         u2, _ = H.gpu_render_level2(r, scene, spec)
     finally:
