@@ -1145,7 +1145,8 @@ int fill_descs(gudni_ctx *ctx, gudni_adjoint_batch *a)
 
 // Serialises (with curve provenance) and bins scene s on the device from the
 // parameters in a->points, into a->dev[s].
-int device_rebuild_scene(gudni_ctx *ctx, gudni_adjoint_batch *a, int s)
+// Scene s's serialiser inputs (its outline offsets go to the device once).
+int scene_ser_in(gudni_ctx *ctx, gudni_adjoint_batch *a, int s, SerIn &in)
 {
     const gudni_adjoint_batch::Scene &sc = a->scenes[s];
     gudni_adjoint_batch::DevScene &v = a->dev[s];
@@ -1155,7 +1156,6 @@ int device_rebuild_scene(gudni_ctx *ctx, gudni_adjoint_batch *a, int s)
         CHECK_HIP(ctx, upload(ctx, v.soff, sc.shape_off.data(), 4u * sc.shape_off.size()));
         CHECK_HIP(ctx, upload(ctx, v.ooff, sc.outline_off.data(), 4u * sc.outline_off.size()));
     }
-    SerIn in;
     in.tags = (const uint64_t *)v.tags.p;
     in.soff = (const int32_t *)v.soff.p;
     in.ooff = (const int32_t *)v.ooff.p;
@@ -1167,11 +1167,27 @@ int device_rebuild_scene(gudni_ctx *ctx, gudni_adjoint_batch *a, int s)
     in.n_subs = (int32_t)sc.subs.size();
     in.n_uses = (int32_t)(sc.uses.size() / 24);
     in.n_grads = (int32_t)sc.grads.size();
-    const SerOut out{&v.geo, &v.shapes, &v.bbox, &v.sbounds, &v.curves};
-    SerCounts cnt;
-    int rc = serialize_outlines(ctx, in, out, cnt);
+    return GUDNI_OK;
+}
+
+// Re-serialises the listed scenes from the parameters on the device (one
+// scene alone, several in one many-scene pass), with curve provenance.
+int device_serialize_scenes(gudni_ctx *ctx, gudni_adjoint_batch *a, const std::vector<int> &which)
+{
+    const size_t m = which.size();
+    std::vector<SerIn> in(m);
+    std::vector<SerOut> out(m);
+    std::vector<SerCounts> cnt(m);
+    for (size_t i = 0; i < m; ++i) {
+        const int rc = scene_ser_in(ctx, a, which[i], in[i]);
+        if (rc) return rc;
+        gudni_adjoint_batch::DevScene &v = a->dev[which[i]];
+        out[i] = SerOut{&v.geo, &v.shapes, &v.bbox, &v.sbounds, &v.curves};
+    }
+    const int rc = m == 1 ? serialize_outlines(ctx, in[0], out[0], cnt[0])
+                          : serialize_outlines_many(ctx, in.data(), out.data(), cnt.data(), (int)m);
     if (rc) return rc;
-    v.n_kept = (int32_t)cnt.n_kept;
+    for (size_t i = 0; i < m; ++i) a->dev[which[i]].n_kept = (int32_t)cnt[i].n_kept;
     return GUDNI_OK;
 }
 
@@ -1270,7 +1286,7 @@ int device_update(gudni_ctx *ctx, gudni_adjoint_batch *a, int scene)
     } else {
         which.push_back(scene);
     }
-    for (size_t i = 0; i < which.size() && rc == GUDNI_OK; ++i) rc = device_rebuild_scene(ctx, a, which[i]);
+    rc = device_serialize_scenes(ctx, a, which);
     if (rc == GUDNI_OK) rc = device_bin_scenes(ctx, a, which);
     if (rc) {
         // the parameters on the device no longer match any complete set of

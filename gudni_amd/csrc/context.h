@@ -79,6 +79,7 @@ struct gudni_ctx {
     float last_bin_ms = 0;
     // device serialisation (gudni_scene_upload_outlines)
     gudni::DevBuf ser_tags, ser_soff, ser_ooff, ser_pairs, ser_work;
+    gudni::DevBuf ser_many_ws, ser_many_args;   // many-scene serialisation
     size_t scene_geo_bytes = 0;
     hipEvent_t ser_ev[2] = {nullptr, nullptr};
     float last_ser_ms = 0;

@@ -667,7 +667,8 @@ void gudni_destroy(gudni_ctx *ctx)
                       &ctx->slow_thr, &ctx->slow_hdr, &ctx->slow_sidx, &ctx->control, &ctx->dump,
                       &ctx->cycles, &ctx->scene_table, &ctx->scene_shapes, &ctx->scene_bbox,
                       &ctx->scene_bounds, &ctx->bin_ws, &ctx->ser_tags, &ctx->ser_soff,
-                      &ctx->ser_ooff, &ctx->ser_pairs, &ctx->ser_work};
+                      &ctx->ser_ooff, &ctx->ser_pairs, &ctx->ser_work, &ctx->bins_ws, &ctx->bins_jobs,
+                      &ctx->ser_many_ws, &ctx->ser_many_args};
     for (DevBuf *b : bufs) b->release();
     for (hipEvent_t &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);

@@ -33,4 +33,8 @@ int check_outline_offsets(gudni_ctx *ctx, const gudni_outlines *o, int32_t &n_ou
 // the output buffers as needed.  Offsets must already be validated.
 int serialize_outlines(gudni_ctx *ctx, const SerIn &in, const SerOut &out, SerCounts &cnt);
 
+// The same for m scenes in a handful of launches (a grid row per scene, a
+// block per scene for the scans) and one synchronisation in all.
+int serialize_outlines_many(gudni_ctx *ctx, const SerIn *in, const SerOut *out, SerCounts *cnt, int m);
+
 }  // namespace gudni

@@ -259,9 +259,12 @@ __device__ void emit_outline(const float *p, int k, const Runs &r, uint8_t *heap
 __device__ __forceinline__ int gtid() { return (int)(blockIdx.x * blockDim.x + threadIdx.x); }
 __device__ __forceinline__ int gstride() { return (int)(gridDim.x * blockDim.x); }
 
-__global__ void __launch_bounds__(kThreads) ser_count(Args a)
+// Kernel bodies over (first thread, stride): the single-scene kernels run
+// them over the grid, the many-scene kernels over the blocks of a grid row
+// (blockIdx.y = scene).
+__device__ void count_body(const Args &a, int t0, int ts)
 {
-    for (int s = gtid(); s < a.n_shapes; s += gstride()) {
+    for (int s = t0; s < a.n_shapes; s += ts) {
         const int o0 = a.soff[s], o1 = a.soff[s + 1];
         const int p0 = a.ooff[o0], p1 = a.ooff[o1];
         bool keep = false;
@@ -318,9 +321,9 @@ __global__ void __launch_bounds__(kThreads) ser_count(Args a)
     }
 }
 
-__global__ void __launch_bounds__(kThreads) ser_emit(Args a, Orders ord)
+__device__ void emit_body(const Args &a, const Orders &ord, int t0, int ts)
 {
-    for (int s = gtid(); s < a.n_shapes; s += gstride()) {
+    for (int s = t0; s < a.n_shapes; s += ts) {
         if (!a.keep[s]) continue;
         const uint64_t idx = a.kidx[s];
         uint64_t base = a.ustart[s];
@@ -345,10 +348,9 @@ __global__ void __launch_bounds__(kThreads) ser_emit(Args a, Orders ord)
 
 // Extent of a shape's strands: x from the end points, y over every point; a
 // NaN y disables the y test (the host's record validation computes the same).
-__global__ void __launch_bounds__(kThreads) ser_extents(Args a, const uint64_t *n_kept)
+__device__ void extents_body(const Args &a, int n, int t0, int ts)
 {
-    const int n = (int)*n_kept;
-    for (int i = gtid(); i < n; i += gstride()) {
+    for (int i = t0; i < n; i += ts) {
         const ShapeRec rec = a.shapes[i];
         size_t pos = 16u * (size_t)rec.geo_start;
         float xlo = INFINITY, xhi = -INFINITY, ylo = INFINITY, yhi = -INFINITY;
@@ -372,7 +374,41 @@ __global__ void __launch_bounds__(kThreads) ser_extents(Args a, const uint64_t *
     }
 }
 
+__global__ void __launch_bounds__(kThreads) ser_count(Args a) { count_body(a, gtid(), gstride()); }
+
+__global__ void __launch_bounds__(kThreads) ser_emit(Args a, Orders ord) { emit_body(a, ord, gtid(), gstride()); }
+
+__global__ void __launch_bounds__(kThreads) ser_extents(Args a, const uint64_t *n_kept)
+{
+    extents_body(a, (int)*n_kept, gtid(), gstride());
+}
+
 __global__ void ser_reset(uint32_t *status) { if (threadIdx.x < 4) status[threadIdx.x] = 0u; }
+
+__device__ __forceinline__ int row_tid() { return (int)(blockIdx.x * blockDim.x + threadIdx.x); }
+__device__ __forceinline__ int row_stride() { return (int)(gridDim.x * blockDim.x); }
+
+__global__ void ser_reset_many(const Args *args, int n)
+{
+    for (int s = row_tid(); s < n; s += row_stride())
+        for (int k = 0; k < 4; ++k) args[s].status[k] = 0u;
+}
+
+__global__ void __launch_bounds__(kThreads) ser_count_many(const Args *args)
+{
+    count_body(args[blockIdx.y], row_tid(), row_stride());
+}
+
+__global__ void __launch_bounds__(kThreads) ser_emit_many(const Args *args, Orders ord)
+{
+    emit_body(args[blockIdx.y], ord, row_tid(), row_stride());
+}
+
+__global__ void __launch_bounds__(kThreads) ser_extents_many(const Args *args)
+{
+    const Args &a = args[blockIdx.y];
+    extents_body(a, (int)a.kidx[a.n_shapes], row_tid(), row_stride());
+}
 
 // Scan inputs.
 struct KeepFlags {
@@ -398,6 +434,28 @@ struct Curves {
     __device__ int count() const { return n; }
     __device__ uint64_t operator()(int i) const { return units[i] - strands[i]; }
 };
+
+// One block per scene: its three scans, then its totals and status.
+__global__ void __launch_bounds__(kThreads) ser_scans_many(const Args *args, uint64_t *totals)
+{
+    const Args &a = args[blockIdx.x];
+    const int n = a.n_shapes;
+    devscan::block_range_excl_scan(KeepFlags{a.keep, n}, a.kidx);
+    __syncthreads();
+    devscan::block_range_excl_scan(Units{a.units, n}, a.ustart);
+    __syncthreads();
+    if (a.cstart) {
+        devscan::block_range_excl_scan(Curves{a.units, a.strands, n}, a.cstart);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        uint64_t *t = totals + 4 * (size_t)blockIdx.x;
+        t[0] = a.kidx[n];
+        t[1] = a.ustart[n];
+        t[2] = a.cstart ? a.cstart[n] : 0;
+        t[3] = a.status[0];
+    }
+}
 
 Orders make_orders()
 {
@@ -511,6 +569,99 @@ int serialize_outlines(gudni_ctx *ctx, const SerIn &in, const SerOut &out, SerCo
     cnt.n_kept = n_kept;
     cnt.geo_bytes = 16u * units;
     cnt.n_curves = totals[2];
+    return GUDNI_OK;
+}
+
+int serialize_outlines_many(gudni_ctx *ctx, const SerIn *in, const SerOut *out, SerCounts *cnt, int m)
+{
+    if (m <= 0) return GUDNI_OK;
+    const hipStream_t st = ctx->stream;
+    // per scene: box 16n | units, kidx, ustart, cstart 8(n+1) | keep, strands 4n | status 16
+    auto ws_bytes = [](size_t n) {
+        return (16u * n + 8u * (4 * n + 3) + 4u * 2 * n + 16 + 255) & ~(size_t)255;
+    };
+    std::vector<size_t> off((size_t)m);
+    size_t total = 0;
+    int32_t most = 0;
+    for (int s = 0; s < m; ++s) {
+        off[s] = total;
+        total += ws_bytes((size_t)in[s].n_shapes);
+        most = std::max(most, in[s].n_shapes);
+    }
+    CHECK_HIP(ctx, ctx->ser_many_ws.ensure(total + 32u * (size_t)m));
+    std::vector<ser::Args> args((size_t)m);
+    for (int s = 0; s < m; ++s) {
+        ser::Args &a = args[s];
+        std::memset(&a, 0, sizeof(a));
+        const size_t n = (size_t)in[s].n_shapes;
+        uint8_t *w = (uint8_t *)ctx->ser_many_ws.p + off[s];
+        a.box = (float4 *)w;              w += 16u * n;
+        a.units = (uint64_t *)w;          w += 8u * n;
+        a.kidx = (uint64_t *)w;           w += 8u * (n + 1);
+        a.ustart = (uint64_t *)w;         w += 8u * (n + 1);
+        a.cstart = (uint64_t *)w;         w += 8u * (n + 1);
+        a.keep = (uint32_t *)w;           w += 4u * n;
+        a.strands = (uint32_t *)w;        w += 4u * n;
+        a.status = (uint32_t *)w;
+        if (!out[s].curves) a.cstart = nullptr;   // no curve tracking: no curve scan
+        a.tags = in[s].tags;
+        a.soff = in[s].soff;
+        a.ooff = in[s].ooff;
+        a.pairs = in[s].pairs;
+        a.n_shapes = in[s].n_shapes;
+        a.fw = (float)in[s].width;
+        a.fh = (float)in[s].height;
+        a.subs = in[s].subs;
+        a.n_subs = in[s].n_subs;
+        a.n_uses = in[s].n_uses;
+        a.n_grads = in[s].n_grads;
+    }
+    uint64_t *totals_dev = (uint64_t *)((uint8_t *)ctx->ser_many_ws.p + total);
+    CHECK_HIP(ctx, upload(ctx, ctx->ser_many_args, args.data(), sizeof(ser::Args) * args.size()));
+    const ser::Args *dargs = (const ser::Args *)ctx->ser_many_args.p;
+    const int gx = std::min(64, ser::grid_for(most));
+    hipLaunchKernelGGL(ser::ser_reset_many, dim3(1), dim3(ser::kThreads), 0, st, dargs, m);
+    hipLaunchKernelGGL(ser::ser_count_many, dim3(gx, m), dim3(ser::kThreads), 0, st, dargs);
+    hipLaunchKernelGGL(ser::ser_scans_many, dim3(m), dim3(ser::kThreads), 0, st, dargs, totals_dev);
+    CHECK_HIP(ctx, hipGetLastError());
+    std::vector<uint64_t> totals(4 * (size_t)m);
+    CHECK_HIP(ctx, hipMemcpyAsync(totals.data(), totals_dev, 8 * totals.size(), hipMemcpyDeviceToHost, st));
+    CHECK_HIP(ctx, hipStreamSynchronize(st));
+    int32_t most_kept = 0;
+    for (int s = 0; s < m; ++s) {
+        const uint64_t *t = totals.data() + 4 * (size_t)s;
+        const uint32_t status = (uint32_t)t[3];
+        const std::string where = "scene " + std::to_string(s) + ": ";
+        if (status & ser::kErrSubstance)
+            return set_error(ctx, GUDNI_ERR_ARG, where + "shape substance id outside the substance table");
+        if (status & ser::kErrGradient)
+            return set_error(ctx, GUDNI_ERR_ARG, where + "gradient substance refers to a missing gradient");
+        if (status & ser::kErrPicture)
+            return set_error(ctx, GUDNI_ERR_ARG, where + "picture substance refers to a missing usage");
+        if (t[1] >= (1ull << 32) || t[2] >= (1ull << 31))
+            return set_error(ctx, GUDNI_ERR_ARG, where + "serialised geometry above 64 GiB or 2^31 curves");
+        const SerOut &o = out[s];
+        CHECK_HIP(ctx, o.geo->ensure(16u * (size_t)t[1]));
+        CHECK_HIP(ctx, o.shapes->ensure(16u * (size_t)t[0]));
+        CHECK_HIP(ctx, o.bbox->ensure(16u * (size_t)t[0]));
+        CHECK_HIP(ctx, o.bounds->ensure(16u * (size_t)t[0]));
+        if (o.curves) CHECK_HIP(ctx, o.curves->ensure(sizeof(CurveSrc) * (size_t)t[2]));
+        ser::Args &a = args[s];
+        a.heap = (uint8_t *)o.geo->p;
+        a.shapes = (ShapeRec *)o.shapes->p;
+        a.bbox = (float4 *)o.bbox->p;
+        a.bounds = (float4 *)o.bounds->p;
+        a.curves = o.curves ? (CurveSrc *)o.curves->p : nullptr;
+        cnt[s].n_kept = t[0];
+        cnt[s].geo_bytes = 16u * t[1];
+        cnt[s].n_curves = t[2];
+        most_kept = std::max(most_kept, (int32_t)t[0]);
+    }
+    CHECK_HIP(ctx, upload(ctx, ctx->ser_many_args, args.data(), sizeof(ser::Args) * args.size()));
+    hipLaunchKernelGGL(ser::ser_emit_many, dim3(gx, m), dim3(ser::kThreads), 0, st, dargs, ser::make_orders());
+    hipLaunchKernelGGL(ser::ser_extents_many, dim3(std::min(64, ser::grid_for(most_kept)), m), dim3(ser::kThreads),
+                       0, st, dargs);
+    CHECK_HIP(ctx, hipGetLastError());
     return GUDNI_OK;
 }
 
