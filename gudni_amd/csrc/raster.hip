@@ -489,7 +489,7 @@ int bin_scenes(gudni_ctx *ctx, const BinIn *in, const BinOut *out, BinCounts *cn
 {
     if (n <= 0) return GUDNI_OK;
     std::vector<binning::Params> params((size_t)n);
-    std::vector<int32_t> cap_pairs((size_t)n), cap_nodes((size_t)n);
+    std::vector<int32_t> cap_pairs((size_t)n), cap_nodes((size_t)n), max_nodes((size_t)n);
     for (int s = 0; s < n; ++s) {
         binning::Params &bp = params[s];
         bp.n_shapes = in[s].n_shapes;
@@ -503,9 +503,11 @@ int bin_scenes(gudni_ctx *ctx, const BinIn *in, const BinOut *out, BinCounts *cn
         bp.width = in[s].width;
         bp.height = in[s].height;
         bp.max_levels = std::max(2, 2 * (bp.canvas_depth - 3) + 2);
-        // every node of a tree down to the 8-pixel floor fits
+        // a tree down to the 8-pixel floor has at most 2 (side / 8)^2 nodes;
+        // start below that for large canvases and grow on overflow
         const int64_t cells = (int64_t)std::max(1, (1 << bp.canvas_depth) / 8);
-        cap_nodes[s] = (int32_t)std::min<int64_t>(2 * cells * cells + 16, 1 << 26);
+        max_nodes[s] = (int32_t)std::min<int64_t>(2 * cells * cells + 16, 1 << 28);
+        cap_nodes[s] = std::min(max_nodes[s], std::max(1 << 15, 4 * bp.n_shapes));
         // the work list's scan reuses the pair scan's words: room for a
         // count per tile, and tiles never outnumber nodes
         cap_pairs[s] = std::max(std::max(1 << 14, 8 * bp.n_shapes), cap_nodes[s]);
@@ -558,7 +560,8 @@ int bin_scenes(gudni_ctx *ctx, const BinIn *in, const BinOut *out, BinCounts *cn
                 cnt[s].n_refs = (int32_t)st[2];
                 cnt[s].n_work = (int32_t)st[3];
             } else if (st[0] == 1) {
-                cap_pairs[s] = (int32_t)std::min<int64_t>(4 * (int64_t)cap_pairs[s], 1 << 28);
+                cap_nodes[s] = (int32_t)std::min<int64_t>(4 * (int64_t)cap_nodes[s], max_nodes[s]);
+                cap_pairs[s] = std::max(cap_nodes[s], (int32_t)std::min<int64_t>(4 * (int64_t)cap_pairs[s], 1 << 28));
                 again.push_back(s);
             } else {
                 return set_error(ctx, GUDNI_ERR_ARG, "scene binning: tile tree deeper than the canvas allows");

@@ -218,3 +218,35 @@ def test_whole_batch_device_update(rasterizer):
     finally:
         adj.close()
         fresh.close()
+
+
+def test_whole_batch_update_regrows_binning():
+    """A whole-batch update whose one-block-per-scene binning outgrows its
+    first capacities: 40 large shapes on 2048^2 at 16-pixel tiles make 16,384
+    tiles and 146,895 (tile, shape) pairs against a first capacity of 32,768
+    pairs, beside a small scene.  After the update every scene equals a fresh
+    host-built upload."""
+    import torch
+    from gudni_amd import scenes as S
+    big = S.batch_scene(2048, 40, 31, gradient_share=0)
+    small, small_target = A.case(128, 40, 93)
+    r = G.Rasterizer(G.default_spec(max_tile_size=16))
+    try:
+        items = [(A.perturbed(big, 5), np.zeros((big.height, big.width, 4), np.float32)), (small, small_target)]
+        moved = [A.perturbed(sc, 17 + i, point_sigma=1.0, color_sigma=0.0) for i, (sc, _) in enumerate(items)]
+        adj = G.Adjoint(r, items)
+        fresh = G.Adjoint(r, [(m, t) for m, (_, t) in zip(moved, items)])
+        try:
+            t = torch.tensor(np.concatenate([m.pairs.reshape(-1) for m in moved]), dtype=torch.float32,
+                             device="cuda")
+            say("whole-batch update with a 2048^2 scene at 16-px tiles")
+            adj.update_device(-1, pairs_ptr=t.data_ptr())
+            adj.run()
+            fresh.run()
+            for i in range(2):
+                _same_result(fresh.result(i, image=True), adj.result(i, image=True), f"scene {i}")
+        finally:
+            adj.close()
+            fresh.close()
+    finally:
+        r.close()
