@@ -403,7 +403,7 @@ def batched_adjoint_leg(spec, steps, log_fn, fixed=None):
             "shapes": int(sum(s.n_shapes for s, _ in items))}
 
 
-def batched_leg(spec, steps, log_fn):
+def batched_leg(spec, steps, log_fn, fixed=None):
     """The batched workload: 256 scenes of 256^2..2048^2 with 100..50k shapes,
     tile size and sort radix chosen by the autotuner on the whole batch, all
     scenes in one persistent launch replayed from a captured hipGraph;
@@ -418,7 +418,10 @@ def batched_leg(spec, steps, log_fn):
     stream = torch.cuda.current_stream()
     r.set_stream(stream.cuda_stream)
     t = time.perf_counter()
-    tile, radix, timings = autotune.tune_batch(r, scenes, reps=2, log=log_fn)
+    if fixed:
+        (tile, radix), timings = fixed, None
+    else:
+        tile, radix, timings = autotune.tune_batch(r, scenes, reps=2, log=log_fn)
     t_tune = time.perf_counter() - t
     items, chosen, radices, t_prep = autotune.prepare_batch(scenes, default_tile=tile, default_radix=radix)
     b_alg = sum(algorithmic_bytes(sc, geo, jobs) for sc, geo, jobs in items)
@@ -502,6 +505,9 @@ def main():
                     help="sort radix bits of the frame (default: chosen by the autotuner)")
     ap.add_argument("--tile", type=int, default=0,
                     help="tile size of the frame (default: chosen by the autotuner)")
+    ap.add_argument("--batch-tile", type=int, default=0,
+                    help="tile size of every batched scene (with --batch-radix: no batch autotuning)")
+    ap.add_argument("--batch-radix", type=int, default=-1)
     ap.add_argument("--texture-filter", type=int, default=1,
                     help="picture sampling of the benchmark scenes: 1 bilinear, 0 nearest")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -665,7 +671,8 @@ def main():
     batched = None
     if not args.no_batch:
         log("batched workload (256 scenes)")
-        batched = batched_leg(spec, args.batch_steps, log)
+        fixed = (args.batch_tile, args.batch_radix) if args.batch_tile > 0 and args.batch_radix >= 0 else None
+        batched = batched_leg(spec, args.batch_steps, log, fixed)
 
     cpu = None
     if not args.no_cpu_baseline:
