@@ -1590,8 +1590,12 @@ int gudni_order_work(const void *tiles, int32_t n_tiles, const uint32_t *work, i
     if (n_tiles < 0 || n_work < 0 || parts < 1 || threads_per_tile < 1 || (n_work && (!tiles || !work || !out)))
         return GUDNI_ERR_ARG;
     const TileRec *t = (const TileRec *)tiles;
-    for (int32_t i = 0; i < n_work; ++i)
+    for (int32_t i = 0; i < n_work; ++i) {
         if ((int64_t)(work[i] >> 2) >= n_tiles) return GUDNI_ERR_ARG;
+        // the cost estimate shifts by the tile's depth: a tile is at most 2^15 wide
+        const TileRec &r = t[work[i] >> 2];
+        if (r.hdepth < 0 || r.hdepth > 15) return GUDNI_ERR_ARG;
+    }
     order_longest_first(threads_per_tile, t, work, n_work, parts, out);
     return GUDNI_OK;
 }

@@ -129,3 +129,7 @@ def test_work_order_longest_first():
             assert (out[i] & 3) == (out[i - 1] & 3) + 1
     bad = np.zeros(4, np.uint32)
     assert L.gudni_order_work(tiles.ctypes.data, 1, work.ctypes.data, 4, 256, 8, bad.ctypes.data) != 0
+    deep = tiles.copy()
+    deep[:, 4] = 40                                         # hdepth beyond any tile width
+    assert L.gudni_order_work(deep.ctypes.data, n_tiles, work.ctypes.data, work.size, 256, 8,
+                              out.ctypes.data) != 0
