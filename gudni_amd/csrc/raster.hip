@@ -447,7 +447,7 @@ int bin_scene(gudni_ctx *ctx, const BinIn &in, const BinOut &out, BinCounts &cnt
     // deeper tree stops with an error, never silently).
     bp.max_levels = std::max(2, 2 * (bp.canvas_depth - 3) + 2);
     if (!ctx->bin_cap_pairs) {
-        ctx->bin_cap_pairs = std::max(1 << 16, 8 * bp.n_shapes);
+        ctx->bin_cap_pairs = (int32_t)std::min<int64_t>(std::max<int64_t>(1 << 16, 8 * (int64_t)bp.n_shapes), 1 << 28);
         ctx->bin_cap_nodes = std::max(1 << 15, bp.n_shapes / 4);
     }
     if (ctx->timing)
@@ -473,10 +473,14 @@ int bin_scene(gudni_ctx *ctx, const BinIn &in, const BinOut &out, BinCounts &cnt
                                       ctx->stream));
         CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
         if (status[0] == 0) break;
-        if (status[0] != 1 || attempt >= 8)
+        if (status[0] != 1)
             return set_error(ctx, GUDNI_ERR_ARG, "scene binning: tile tree deeper than the canvas allows");
-        ctx->bin_cap_pairs = 4 * cp;
-        ctx->bin_cap_nodes = 4 * cn;
+        // capacities grow 4x up to 2^28 entries each (32-bit indices throughout)
+        constexpr int32_t kCapMax = 1 << 28;
+        if (attempt >= 8 || (cp >= kCapMax && cn >= kCapMax))
+            return set_error(ctx, GUDNI_ERR_ARG, "scene binning: tile tree needs more than 2^28 nodes or references");
+        ctx->bin_cap_pairs = (int32_t)std::min<int64_t>(4 * (int64_t)cp, kCapMax);
+        ctx->bin_cap_nodes = (int32_t)std::min<int64_t>(4 * (int64_t)cn, kCapMax);
     }
     if (ctx->timing) CHECK_HIP(ctx, hipEventElapsedTime(&ctx->last_bin_ms, ctx->bin_ev[0], ctx->bin_ev[1]));
     cnt.n_tiles = (int32_t)status[1];
@@ -507,10 +511,11 @@ int bin_scenes(gudni_ctx *ctx, const BinIn *in, const BinOut *out, BinCounts *cn
         // start below that for large canvases and grow on overflow
         const int64_t cells = (int64_t)std::max(1, (1 << bp.canvas_depth) / 8);
         max_nodes[s] = (int32_t)std::min<int64_t>(2 * cells * cells + 16, 1 << 28);
-        cap_nodes[s] = std::min(max_nodes[s], std::max(1 << 15, 4 * bp.n_shapes));
+        cap_nodes[s] = (int32_t)std::min<int64_t>(max_nodes[s], std::max<int64_t>(1 << 15, 4 * (int64_t)bp.n_shapes));
         // the work list's scan reuses the pair scan's words: room for a
         // count per tile, and tiles never outnumber nodes
-        cap_pairs[s] = std::max(std::max(1 << 14, 8 * bp.n_shapes), cap_nodes[s]);
+        cap_pairs[s] = (int32_t)std::max<int64_t>(
+            std::min<int64_t>(std::max<int64_t>(1 << 14, 8 * (int64_t)bp.n_shapes), 1 << 28), cap_nodes[s]);
     }
     std::vector<int> todo((size_t)n);
     for (int s = 0; s < n; ++s) todo[s] = s;
