@@ -771,15 +771,24 @@ static int run_job(gudni_ctx *ctx, const void *shapes, int32_t n_shapes, const v
     CHECK_HIP(ctx, upload(ctx, ctx->work, work.data(), 4u * work.size()));
     const size_t npix = (size_t)w * (size_t)h;
     CHECK_HIP(ctx, ctx->out.ensure(4 * npix));
-    // the pixels a job can write lie in the rows its tiles span (clipped to
-    // the canvas): only those rows of the host frame travel, both ways and in
-    // one piece each, so every other pixel stays as it was
+    // the pixels a job can write lie in the rows its tiles' columns reach:
+    // only those rows of the host frame travel, both ways and in one piece
+    // each, so every other pixel stays as it was.  A column is active while
+    // its row origin lies inside the box and the canvas, and it writes 2^dd
+    // rows from there, clipped to the canvas but not to the box
+    // (initTileState / isActiveThread, Kernels.cl:1692-1722): a box shorter
+    // than its depths say is written below its bottom.  Origins are >= 0.
+    const int cd = ceil_log2(ctx->spec.threads_per_tile);
     int32_t by0 = h, by1 = 0;
     for (int32_t t = 0; t < n_tiles; ++t) {
         const TileRec &r = ((const TileRec *)tiles)[t];
-        if (r.box[0] >= w || r.box[2] <= 0) continue;
-        by0 = std::min(by0, std::max(0, r.box[1]));
-        by1 = std::max(by1, std::min(h, r.box[3]));
+        const int64_t bh = (int64_t)r.box[3] - r.box[1];
+        if (r.box[0] >= w || r.box[1] >= h || bh <= 0) continue;
+        const int dd = std::max(0, (int)r.vdepth - (cd - (int)r.hdepth));
+        const int64_t seg = (int64_t)1 << dd;
+        const int64_t rows = std::min<int64_t>((bh + seg - 1) / seg, ctx->spec.threads_per_tile >> r.hdepth);
+        by0 = std::min(by0, r.box[1]);
+        by1 = (int32_t)std::max<int64_t>(by1, std::min<int64_t>(h, r.box[1] + rows * seg));
     }
     const bool window = host_out && by0 < by1;
     const size_t win_off = 4 * (size_t)by0 * (size_t)w, win_bytes = 4 * (size_t)(by1 - by0) * (size_t)w;

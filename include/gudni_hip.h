@@ -140,7 +140,9 @@ int  gudni_upload_gradients(gudni_ctx *ctx, const void *gradients, int32_t n_gra
  * generateThresholds / sortThresholds / renderThresholds work of one
  * RasterJob (Raster/Job.hs:72-77) and writes the job's pixels into `out`
  * (host memory, width*height words; other pixels are left untouched: only
- * the rows the job's tiles span are copied to the device and back).
+ * the rows the job's columns reach are copied to the device and back -- a
+ * column writes 2^dd rows from its origin, clipped to the canvas but not to
+ * its tile's box, Kernels.cl:1692-1722).
  * Synchronous, like the reference's KernelSync launches. */
 int gudni_raster_job(gudni_ctx *ctx,
                      const void *shapes, int32_t n_shapes,

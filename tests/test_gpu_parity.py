@@ -351,3 +351,39 @@ def test_level1_job_touches_only_its_tiles(rasterizer):
         assert mine.any()
         assert np.array_equal(frame[mine], ref[mine]), f"job {j}: its pixels differ from the oracle"
         assert np.all(frame[~mine] == sentinel), f"job {j}: pixels outside its tiles changed"
+
+
+def test_level1_job_writes_below_a_short_tile_box(rasterizer):
+    """A column writes 2^dd rows from its origin, clipped to the canvas but
+    not to the tile box (initTileState, Kernels.cl:1692-1722).  A 256-pixel
+    tile whose box is cut to 100 rows still writes all 256 rows of each
+    column, and gudni_raster_job returns every one of them to the host
+    frame, as the oracle writes them."""
+    scene = S.build_scene(256, 256, [(S.solid_tag(0), [S.rectangle_pairs(10.5, 20.25, 200.5, 240.75)])],
+                          [[0.2, 0.6, 1.0, 1.0]], background=(0.9, 0.8, 0.7, 1.0), name="short box")
+    _, geo, jobs = G.prepare(scene)
+    assert jobs.n_jobs == 1
+    tiles, shapes = jobs.job(0)
+    t = np.frombuffer(tiles.tobytes(), H.TILE_DT).copy()
+    assert t.size == 1 and tuple(t["box"][0]) == (0, 0, 256, 256) and t["hd"][0] == 8 and t["vd"][0] == 8
+    t["box"][0][3] = 100
+    short = t.view(np.uint8)
+    say("level1 job with a tile box cut to 100 of its 256 rows")
+    rasterizer.upload_common(geo.geo, scene)
+    out = np.full(scene.width * scene.height, np.uint32(0xDEADBEEF), np.uint32)
+    rasterizer.raster_job(shapes, short, scene.background, scene.width, scene.height, 0, 0, out)
+    ospec = H.oracle_spec(G.default_spec())
+    ogeo = O.Geometry(scene)
+    assert np.array_equal(np.frombuffer(ogeo.geo.tobytes(), np.uint8), np.frombuffer(geo.geo.tobytes(), np.uint8))
+
+    class OneJob:
+        n_jobs = 1
+
+        def job(self, i):
+            return short, np.ascontiguousarray(shapes, np.uint8)
+
+    rc, ref, _ = O.raster_job_dump(scene, ospec, ogeo, OneJob(), 0, cap=1)
+    assert rc == 0
+    frame = out.reshape(scene.height, scene.width)
+    assert np.array_equal(frame, ref), H.first_mismatch(frame, ref)
+    assert not np.any(frame[100:] == np.uint32(0xDEADBEEF)), "rows below the box did not reach the host"
