@@ -64,3 +64,54 @@ def test_random_scene_matches_oracle(i):
         assert H.first_mismatch(u32, ref["out"]) is None
         np.testing.assert_array_equal(f32.view(np.uint32), np.asarray(ref["f32"], np.float32).reshape(f32.shape).view(np.uint32))
         assert np.abs(f32.astype(np.float64) - ref["f64"]).max() <= REFERENCE_MODE_TOL
+
+
+N_CUT_CASES = 12
+SENTINEL = np.uint32(0xDEADBEEF)     # alpha 0xDE: never a pixel the path writes
+
+
+@pytest.mark.parametrize("i", range(N_CUT_CASES))
+def test_random_level1_jobs_with_cut_tile_boxes(i):
+    """Level 1 with hand-made TileInfo: the sweep's scenes and specs, every
+    job's tile boxes cut at random to 1..height rows (half of the tiles).  A
+    column is active while its origin lies in the cut box and writes its full
+    segment regardless (initTileState / isActiveThread, Kernels.cl:1692-1722).
+    Per job, the pixels gudni_raster_job changes in a frame of sentinels are
+    exactly those the oracle writes, with the oracle's BGRA words."""
+    from oracle import oracle as O
+    scene, spec = case(i)
+    rng = np.random.default_rng(7000 + i)
+    _, geo, jobs = G.prepare(scene, spec)
+    ogeo = O.Geometry(scene)
+    assert np.array_equal(np.frombuffer(ogeo.geo.tobytes(), np.uint8), np.frombuffer(geo.geo.tobytes(), np.uint8))
+    ospec = H.oracle_spec(spec)
+    say(f"cut boxes {i}: {scene.name} {scene.width}x{scene.height}, {jobs.n_jobs} jobs, tile {spec.max_tile_size}")
+    r = G.Rasterizer(spec)
+    try:
+        r.upload_common(geo.geo, scene)
+        for j in range(jobs.n_jobs):
+            tiles, shapes = jobs.job(j)
+            t = np.frombuffer(np.ascontiguousarray(tiles).tobytes(), H.TILE_DT).copy()
+            bh = t["box"][:, 3] - t["box"][:, 1]
+            cut = rng.random(t.size) < 0.5
+            t["box"][cut, 3] = t["box"][cut, 1] + rng.integers(1, bh[cut] + 1)
+            cut_tiles = t.view(np.uint8)
+            shapes = np.ascontiguousarray(shapes, np.uint8)
+
+            class OneJob:
+                n_jobs = jobs.n_jobs
+
+                def job(self, _):
+                    return cut_tiles, shapes
+
+            rc, ref, _ = O.raster_job_dump(scene, ospec, ogeo, OneJob(), j, cap=1)
+            assert rc == 0
+            out = np.full(scene.width * scene.height, SENTINEL, np.uint32)
+            r.raster_job(shapes, cut_tiles, scene.background, scene.width, scene.height, 0,
+                         jobs.n_jobs - 1 - j, out)
+            frame = out.reshape(scene.height, scene.width)
+            wrote = frame != SENTINEL
+            np.testing.assert_array_equal(wrote, ref != 0, err_msg=f"job {j}: written pixels")
+            assert np.array_equal(frame[wrote], ref[wrote]), f"job {j}: {H.first_mismatch(np.where(wrote, frame, 0), ref)}"
+    finally:
+        r.close()
