@@ -29,9 +29,14 @@
  *                          stacked translucent rectangles give closed-form
  *                          Porter-Duff colours, the sorted queue equals
  *                          numpy's stable lexsort of the unsorted one.
- *   quadratic flattening . bounded against the exact Bezier area; the
- *                          bit-exact FP32 trajectory is parity unpinned
- *                          beyond that (no runnable reference here).
+ *   quadratic flattening . closed form: every border ordinate (searchTree +
+ *                          bifurcateCurve + yIntercept, via go_strand_border)
+ *                          equals the FP64 chord intercept between the
+ *                          Bernstein points P(k/2^n), P((k+1)/2^n), with n
+ *                          fixed by the curve's second difference (each
+ *                          halving divides |ctl - mid|_1 by exactly 4) and k
+ *                          by the border's position; mutants of the loop
+ *                          fail (tests/test_oracle_flatten.py).
  */
 #include "gudni_oracle.h"
 #include <stdlib.h>
@@ -716,9 +721,11 @@ static inline v2 mid_point(v2 a, v2 b)
     v2 r; r.x = 0.5f * a.x + 0.5f * b.x; r.y = 0.5f * a.y + 0.5f * b.y; return r;
 }
 
-/* bifurcateCurve (Kernels.cl:1233-1258) + intersectCurve (:1226-1230) */
-static float intersect_curve(trav_t t)
+/* bifurcateCurve (Kernels.cl:1233-1258) + intersectCurve (:1226-1230).
+ * *halvings (may be NULL) receives the number of subdivisions taken. */
+static float intersect_curve_n(trav_t t, int *halvings)
 {
+    int taken = 0;
     if (!(t.left.x == t.ctl.x && t.left.y == t.ctl.y)) {
         /* at most GO_MAX_BISECT halvings: a curve inside a 2^30-pixel canvas
          * is flat after ~35; beyond that FP32 can reach a fixed point where
@@ -731,13 +738,16 @@ static float intersect_curve(trav_t t)
             if (flat > O_FLATNESS) {
                 if (t.xpos < on.x) { t.ctl = lm; t.right = on; }
                 else               { t.left = on; t.ctl = rm; }
+                taken++;
             } else break;
         }
     }
+    if (halvings) *halvings = taken;
     if (t.left.x == t.right.x) return cl_min(t.left.y, t.right.y);
     /* yIntercept (Kernels.cl:818-820) */
     return (((t.right.y - t.left.y) / (t.right.x - t.left.x)) * (t.xpos - t.left.x)) + t.left.y;
 }
+static inline float intersect_curve(trav_t t) { return intersect_curve_n(t, NULL); }
 
 /* spawnThresholds (Kernels.cl:1264-1333) */
 static void spawn(col_t *c, uint32_t bit, trav_t *l, trav_t *r, int *added, int *enclosed)
@@ -782,6 +792,38 @@ static void search_tree(trav_t *tr, const uint8_t *tree, int tree_size, float dx
             tr->index = (tr->index << 1) + 2;
         }
     }
+}
+
+/* Test hook: the column-border ordinate one strand contributes, as
+ * buildThresholdArray computes it (traverseTree, Kernels.cl:1378-1408:
+ * translate by the column origin (dx, dy), checkInRange, searchTree at
+ * xpos = max(0, left.x) for the left border or min(1, right.x) for the
+ * right one; then spawnThresholds' choice, Kernels.cl:1264-1333: the curve's
+ * own end point when it lies inside the column, else intersectCurve).
+ * Returns 0 when the strand misses the column (no ordinate), else 1 with
+ * *y, *halvings (subdivisions taken; -1 for an end point) and *node (the
+ * Eytzinger index the search ended on). */
+int go_strand_border(const uint8_t *strand, float dx, float dy, int is_left,
+                     float *y, int *halvings, int *node)
+{
+    uint16_t size; memcpy(&size, strand, 2);
+    int tree_size = ((int)size - 4) / 2;
+    trav_t t;
+    t.right = ld2(strand + 8);  t.right.x -= dx; t.right.y -= dy;
+    t.left = ld2(strand + 16);  t.left.x -= dx;  t.left.y -= dy;
+    t.ctl = ld2(strand + 24);   t.ctl.x -= dx;   t.ctl.y -= dy;
+    t.index = 0;
+    if (!(t.left.x <= 1.0f && t.right.x > 0.0f)) return 0;
+    t.xpos = is_left ? cl_max(0.0f, t.left.x) : cl_min(1.0f, t.right.x);
+    search_tree(&t, strand + 32, tree_size, dx, dy, is_left);
+    *node = t.index;
+    if (is_left ? (t.left.x >= 0.0f) : (t.right.x <= 1.0f)) {
+        *y = is_left ? t.left.y : t.right.y;
+        *halvings = -1;
+    } else {
+        *y = intersect_curve_n(t, halvings);
+    }
+    return 1;
 }
 
 /* buildThresholdArray (Kernels.cl:1540-1595) */

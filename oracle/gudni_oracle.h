@@ -131,6 +131,13 @@ int go_render_scene(const go_spec *spec, const go_serialized *scene,
 
 void go_default_spec(go_spec *spec);
 
+/* Test hook: the ordinate of the column border (x = dx for the left border,
+ * dx + 1 for the right) that one strand contributes in buildThresholdArray
+ * (searchTree + bifurcateCurve + yIntercept, Kernels.cl:1226-1408), in
+ * column-local coordinates.  Returns 0 if the strand misses the column. */
+int go_strand_border(const uint8_t *strand, float dx, float dy, int is_left,
+                     float *y, int *halvings, int *node);
+
 /* FP64 restatement of the differentiable forward (gudni_diff.c): the loss
  * sum (pixel - target)^2 over the window [x0,x1) x [y0,y1) of a scene of
  * solid substances (rgba: 4 doubles per substance), the window's pixels

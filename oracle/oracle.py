@@ -84,6 +84,9 @@ def lib():
                                       C.POINTER(JobIn), C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_int]
         L.go_default_spec.argtypes = [C.POINTER(Spec)]
+        L.go_strand_border.restype = C.c_int
+        L.go_strand_border.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_int,
+                                       C.POINTER(C.c_float), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.go_diff_loss.restype = C.c_int
         L.go_diff_loss.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
@@ -108,6 +111,19 @@ def default_spec(**overrides) -> Spec:
 
 def _ptr(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def strand_border(strand: np.ndarray, dx: float, dy: float, is_left: bool, L=None):
+    """Border ordinate one strand gives a column (go_strand_border):
+    (y, halvings, node) in column-local coordinates, or None if the strand
+    misses the column.  `strand` is the strand's bytes (header first);
+    `L` selects another build of the oracle library (mutant tests)."""
+    L = L or lib()
+    b = np.ascontiguousarray(strand, np.uint8)
+    y, n, node = C.c_float(), C.c_int(), C.c_int()
+    ok = L.go_strand_border(_ptr(b), float(dx), float(dy), int(bool(is_left)),
+                            C.byref(y), C.byref(n), C.byref(node))
+    return (y.value, n.value, node.value) if ok else None
 
 
 def reorder_entry(size: int, i: int) -> int:
