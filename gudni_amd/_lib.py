@@ -3,6 +3,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgudni_hip.so")
@@ -158,9 +159,14 @@ def load():
             import torch  # noqa: F401
         except ImportError:
             pass
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"libgudni_hip.so is not built ({LIB_PATH}); run "
-                               "python -c 'import __graft_entry__; __graft_entry__.build()'")
+        from . import build as _build
+        if _build.stale():
+            # like the reference, which compiles its kernels at every setup
+            # (OpenCL/Setup.hs:124-134), never run a binary that does not
+            # match the sources: rebuild (there is no fallback if that fails)
+            print("[gudni] libgudni_hip.so is missing or older than its sources: building",
+                  file=sys.stderr, flush=True)
+            _build.build()
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

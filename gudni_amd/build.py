@@ -7,6 +7,8 @@ Flags that matter for results:
   -Xclang -target-feature -Xclang -packed-fp32-ops
                                              no packed FP32 instructions
 """
+import glob
+import hashlib
 import os
 import subprocess
 import sys
@@ -27,18 +29,51 @@ def hipcc() -> str:
     return "hipcc"
 
 
+STAMP = LIB + ".stamp"
+
+
+def _command(out: str):
+    return [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+            "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
+            "-fno-gpu-flush-denormals-to-zero",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-Wall", "-Wno-unused-command-line-argument",
+            "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc"),
+            "-o", out] + SOURCES
+
+
+def fingerprint() -> str:
+    """Hash of everything the library is built from: the compile command
+    (compiler, target, flags) and the bytes of every source and header."""
+    h = hashlib.sha256(" ".join(_command("lib")).encode())
+    files = sorted(set(SOURCES) | set(glob.glob(os.path.join(HERE, "csrc", "*")))
+                   | set(glob.glob(os.path.join(ROOT, "include", "*.h"))))
+    for f in files:
+        h.update(os.path.relpath(f, ROOT).encode())
+        with open(f, "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
+
+
+def stale() -> bool:
+    """True when the built library is missing or was built from other
+    sources or flags than the tree holds now."""
+    if not (os.path.exists(LIB) and os.path.exists(STAMP)):
+        return True
+    with open(STAMP) as fh:
+        return fh.read().strip() != fingerprint()
+
+
 def build(verbose: bool = False) -> str:
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
-           "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
-           "-fno-gpu-flush-denormals-to-zero",
-           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-Wall", "-Wno-unused-command-line-argument",
-           "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc"),
-           "-o", LIB + ".tmp"] + SOURCES
+    cmd = _command(LIB + ".tmp")
     if verbose:
         print(" ".join(cmd), flush=True)
+    fp = fingerprint()
     subprocess.run(cmd, check=True)
     os.replace(LIB + ".tmp", LIB)
+    with open(STAMP + ".tmp", "w") as fh:
+        fh.write(fp + "\n")
+    os.replace(STAMP + ".tmp", STAMP)
     return LIB
 
 

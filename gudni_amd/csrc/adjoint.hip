@@ -1286,17 +1286,18 @@ int device_update(gudni_ctx *ctx, gudni_adjoint_batch *a, int scene)
     } else {
         which.push_back(scene);
     }
+    // any failure from here on leaves the parameters on the device without
+    // a complete set of tables (a regrown buffer may be gone, counts stale):
+    // the batch refuses to run until it is uploaded again (or every scene
+    // is rebuilt by a successful update)
+    const bool was_broken = a->broken;
+    a->broken = true;
     rc = device_serialize_scenes(ctx, a, which);
     if (rc == GUDNI_OK) rc = device_bin_scenes(ctx, a, which);
-    if (rc) {
-        // the parameters on the device no longer match any complete set of
-        // tables: the batch refuses to run until it is uploaded again
-        a->broken = true;
-        return rc;
-    }
-    rc = assemble(ctx, a);
+    if (rc == GUDNI_OK) rc = assemble(ctx, a);
     if (rc) return rc;
     CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    a->broken = was_broken && which.size() < a->scenes.size();
     return GUDNI_OK;
 }
 

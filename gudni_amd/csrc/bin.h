@@ -54,7 +54,9 @@ struct Output {
 };
 
 // Enqueues the whole binning on `stream`; status[] holds the counts once the
-// stream reaches it (status[0] != 0: a capacity was exceeded, nothing valid).
+// stream reaches it (status[0] != 0: nothing valid; 1 = a capacity was
+// exceeded, status[6] bit 0 = nodes, bit 1 = pairs; 2 = the tree is deeper
+// than the level table).
 hipError_t enqueue(const Params &p, const Workspace &w, const ShapeRec *shapes,
                    const float4 *bbox, const float4 *shape_bounds, const Output &out,
                    hipStream_t stream);

@@ -208,7 +208,10 @@ __device__ void advance_body(const Dev &d, int level)
         if (nc > 0) d.status[0] = 2;
         return;
     }
-    if ((int64_t)nb + nc > d.cap_nodes || (int64_t)pb + pc > d.cap_pairs) {
+    const bool over_nodes = (int64_t)nb + nc > d.cap_nodes, over_pairs = (int64_t)pb + pc > d.cap_pairs;
+    if (over_nodes || over_pairs) {
+        // which capacity ran out: bit 0 nodes, bit 1 (node, shape) pairs
+        d.status[6] = (over_nodes ? 1u : 0u) | (over_pairs ? 2u : 0u);
         d.status[0] = 1;
         return;
     }

@@ -63,6 +63,9 @@ struct gudni_ctx {
     gudni::DevBuf cycles;    // stage cycle counters of the fast pass (timing only)
     gudni::DevBuf dump;      // dump staging
     int32_t retry_capacity = 0;
+    // bumped whenever the launch scratch (rings, control words) or the fast
+    // grid may have changed: a captured graph holds those pointers and sizes
+    uint64_t scratch_gen = 0;
     // uploaded frame (Level 2)
     int32_t frame_w = 0, frame_h = 0, frame_n_work = 0, frame_n_tiles = 0, frame_index = 0;
     int32_t frame_sort_bits = 0;  // sort radix of the spec the uploaded jobs were built with

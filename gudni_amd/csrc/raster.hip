@@ -187,6 +187,7 @@ void order_longest_first(int threads_per_tile, const TileRec *tiles, const uint3
 int ensure_scratch(gudni_ctx *ctx)
 {
     if (ctx->fast_blocks) return GUDNI_OK;
+    ++ctx->scratch_gen;
     const int waves_per_block = 4;
     int per_cu = 0;
     CHECK_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::raster_pass, 256, 0));
@@ -215,6 +216,7 @@ int ensure_control(gudni_ctx *ctx, int n_work)
 {
     const int want = std::max(1024, n_work);
     if (ctx->retry_capacity < want) {
+        ++ctx->scratch_gen;
         CHECK_HIP(ctx, ctx->control.ensure(sizeof(uint32_t) * (size_t)(dev::kCtlWords + want)));
         ctx->retry_capacity = want;
     }
@@ -475,12 +477,16 @@ int bin_scene(gudni_ctx *ctx, const BinIn &in, const BinOut &out, BinCounts &cnt
         if (status[0] == 0) break;
         if (status[0] != 1)
             return set_error(ctx, GUDNI_ERR_ARG, "scene binning: tile tree deeper than the canvas allows");
-        // capacities grow 4x up to 2^28 entries each (32-bit indices throughout)
+        // the capacity that ran out grows 4x, up to 2^28 entries (32-bit
+        // indices throughout); one already at the clamp fails at once
         constexpr int32_t kCapMax = 1 << 28;
-        if (attempt >= 8 || (cp >= kCapMax && cn >= kCapMax))
-            return set_error(ctx, GUDNI_ERR_ARG, "scene binning: tile tree needs more than 2^28 nodes or references");
-        ctx->bin_cap_pairs = (int32_t)std::min<int64_t>(4 * (int64_t)cp, kCapMax);
-        ctx->bin_cap_nodes = (int32_t)std::min<int64_t>(4 * (int64_t)cn, kCapMax);
+        const bool over_nodes = (status[6] & 1u) != 0, over_pairs = (status[6] & 2u) != 0;
+        if (attempt >= 8 || (over_nodes && cn >= kCapMax) || (over_pairs && cp >= kCapMax))
+            return set_error(ctx, GUDNI_ERR_ARG, over_nodes && cn >= kCapMax
+                                 ? "scene binning: tile tree needs more than 2^28 nodes"
+                                 : "scene binning: tile tree needs more than 2^28 (node, shape) references");
+        if (over_pairs) ctx->bin_cap_pairs = (int32_t)std::min<int64_t>(4 * (int64_t)cp, kCapMax);
+        if (over_nodes) ctx->bin_cap_nodes = (int32_t)std::min<int64_t>(4 * (int64_t)cn, kCapMax);
     }
     if (ctx->timing) CHECK_HIP(ctx, hipEventElapsedTime(&ctx->last_bin_ms, ctx->bin_ev[0], ctx->bin_ev[1]));
     cnt.n_tiles = (int32_t)status[1];
@@ -565,8 +571,13 @@ int bin_scenes(gudni_ctx *ctx, const BinIn *in, const BinOut *out, BinCounts *cn
                 cnt[s].n_refs = (int32_t)st[2];
                 cnt[s].n_work = (int32_t)st[3];
             } else if (st[0] == 1) {
-                cap_nodes[s] = (int32_t)std::min<int64_t>(4 * (int64_t)cap_nodes[s], max_nodes[s]);
-                cap_pairs[s] = std::max(cap_nodes[s], (int32_t)std::min<int64_t>(4 * (int64_t)cap_pairs[s], 1 << 28));
+                // grow only what ran out; fail once it is at its clamp
+                const bool over_nodes = (st[6] & 1u) != 0, over_pairs = (st[6] & 2u) != 0;
+                if ((over_nodes && cap_nodes[s] >= max_nodes[s]) || (over_pairs && cap_pairs[s] >= (1 << 28)))
+                    return set_error(ctx, GUDNI_ERR_ARG, "scene binning: tile tree exceeds the node or reference limit");
+                if (over_nodes) cap_nodes[s] = (int32_t)std::min<int64_t>(4 * (int64_t)cap_nodes[s], max_nodes[s]);
+                if (over_pairs) cap_pairs[s] = (int32_t)std::min<int64_t>(4 * (int64_t)cap_pairs[s], 1 << 28);
+                cap_pairs[s] = std::max(cap_nodes[s], cap_pairs[s]);
                 again.push_back(s);
             } else {
                 return set_error(ctx, GUDNI_ERR_ARG, "scene binning: tile tree deeper than the canvas allows");
@@ -757,6 +768,9 @@ static int run_job(gudni_ctx *ctx, const void *shapes, int32_t n_shapes, const v
                           (const TileRec *)tiles, n_tiles, w, h, xr);
     if (rc) return rc;
     CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    // a job's records go into the buffers an uploaded frame (or binned
+    // scene) renders from: that frame must be uploaded or binned again
+    ctx->frame_ready = false;
     rc = ensure_scratch(ctx);
     if (rc) return rc;
     std::vector<uint32_t> work;
@@ -1133,7 +1147,7 @@ struct gudni_batch {
     int32_t n_work = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    void *captured_control = nullptr;
+    uint64_t captured_gen = 0;     // ctx->scratch_gen the graph was captured with
     int32_t captured_format = 0;
     hipStream_t capture_stream = nullptr;   // capture needs a stream other than the null stream
 };
@@ -1521,7 +1535,9 @@ int gudni_batch_render(gudni_ctx *ctx, gudni_batch *b, int use_graph)
     if (rc) return rc;
     ctx->timed_render = false;
     if (!use_graph) return batch_launch(ctx, b);
-    if (b->exec && (b->captured_control != ctx->control.p || b->captured_format != ctx->out_format)) {
+    // the graph holds the scratch pointers, grid size and pixel format it
+    // was captured with: re-capture once any of them changed
+    if (b->exec && (b->captured_gen != ctx->scratch_gen || b->captured_format != ctx->out_format)) {
         (void)hipGraphExecDestroy(b->exec);
         (void)hipGraphDestroy(b->graph);
         b->exec = nullptr;
@@ -1551,7 +1567,7 @@ int gudni_batch_render(gudni_ctx *ctx, gudni_batch *b, int use_graph)
         CHECK_HIP(ctx, e);
         b->graph = g;
         CHECK_HIP(ctx, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-        b->captured_control = ctx->control.p;
+        b->captured_gen = ctx->scratch_gen;
         b->captured_format = ctx->out_format;
     }
     CHECK_HIP(ctx, hipGraphLaunch(b->exec, ctx->stream));

@@ -9,6 +9,7 @@ built here: they need the #define block the Haskell host generates
 line (a stray token the host's line replacement removes) and an OpenCL
 builtin runtime for the CPU, none of which exist in this image.
 """
+import hashlib
 import os
 import subprocess
 import sys
@@ -17,14 +18,44 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle_gudni.so")
 
 
+STAMP = LIB + ".stamp"
+SOURCES = [os.path.join(HERE, "gudni_oracle.c"), os.path.join(HERE, "gudni_diff.c"),
+           os.path.join(HERE, "gudni_oracle.h")]
+
+
+def _command(out: str):
+    return (["gcc", "-O2", "-std=c11", "-Wall", "-Wextra", "-ffp-contract=off",
+             "-fno-fast-math", "-fPIC", "-shared", "-o", out] + SOURCES[:2] + ["-lm", "-lpthread"])
+
+
+def fingerprint() -> str:
+    """Hash of the compile command and the bytes of every source."""
+    h = hashlib.sha256(" ".join(_command("lib")).encode())
+    for f in SOURCES:
+        with open(f, "rb") as fh:
+            h.update(os.path.basename(f).encode())
+            h.update(fh.read())
+    return h.hexdigest()
+
+
+def stale() -> bool:
+    """True when the library is missing or built from other sources/flags."""
+    if not (os.path.exists(LIB) and os.path.exists(STAMP)):
+        return True
+    with open(STAMP) as fh:
+        return fh.read().strip() != fingerprint()
+
+
 def build(verbose: bool = False) -> str:
-    srcs = [os.path.join(HERE, "gudni_oracle.c"), os.path.join(HERE, "gudni_diff.c")]
-    cmd = ["gcc", "-O2", "-std=c11", "-Wall", "-Wextra", "-ffp-contract=off",
-           "-fno-fast-math", "-fPIC", "-shared", "-o", LIB + ".tmp"] + srcs + ["-lm", "-lpthread"]
+    cmd = _command(LIB + ".tmp")
     if verbose:
         print(" ".join(cmd), flush=True)
+    fp = fingerprint()
     subprocess.run(cmd, check=True)
     os.replace(LIB + ".tmp", LIB)
+    with open(STAMP + ".tmp", "w") as fh:
+        fh.write(fp + "\n")
+    os.replace(STAMP + ".tmp", STAMP)
     return LIB
 
 

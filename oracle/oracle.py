@@ -61,8 +61,9 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError("oracle library not built: run oracle/build.py")
+        from . import build as _build
+        if _build.stale():      # never check against a stale build of the oracle
+            _build.build()
         L = C.CDLL(LIB_PATH)
         L.go_reorder_entry.restype = C.c_int
         L.go_reorder_entry.argtypes = [C.c_int, C.c_int]

@@ -13,12 +13,14 @@ def pytest_configure(config):
 
 
 def _ensure_built():
-    """Load what is built; build only when nothing is."""
+    """Rebuild a library that is missing or older than its sources (a
+    fingerprint of the sources and flags is stamped beside each build), so a
+    test never runs a stale binary."""
     from oracle import build as obuild
     from gudni_amd import build as pbuild
-    if not os.path.exists(obuild.LIB):
+    if obuild.stale():
         obuild.build()
-    if not os.path.exists(pbuild.LIB):
+    if pbuild.stale():
         pbuild.build()
 
 

@@ -133,3 +133,23 @@ def test_work_order_longest_first():
     deep[:, 4] = 40                                         # hdepth beyond any tile width
     assert L.gudni_order_work(deep.ctypes.data, n_tiles, work.ctypes.data, work.size, 256, 8,
                               out.ctypes.data) != 0
+
+
+def test_build_fingerprint_tracks_sources(tmp_path, monkeypatch):
+    """A library whose stamp does not match the sources and flags of the
+    tree is stale, so tests and loaders rebuild it instead of running it."""
+    from gudni_amd import build as pbuild
+    from oracle import build as obuild
+    assert not pbuild.stale() and not obuild.stale()
+    fp = pbuild.fingerprint()
+    assert fp == pbuild.fingerprint()
+    src = tmp_path / "extra.hip"
+    src.write_text("// a\n")
+    monkeypatch.setattr(pbuild, "SOURCES", pbuild.SOURCES + [str(src)])
+    fa = pbuild.fingerprint()
+    assert fa != fp and pbuild.stale()
+    src.write_text("// b\n")
+    assert pbuild.fingerprint() != fa
+    monkeypatch.setattr(pbuild, "SOURCES", pbuild.SOURCES[:-1])
+    monkeypatch.setattr(pbuild, "ARCH", "gfx942")
+    assert pbuild.fingerprint() != fp and pbuild.stale()

@@ -74,6 +74,24 @@ def test_batch_graph_replays(batch_and_refs):
         b.check()
 
 
+def test_batch_graph_recaptured_after_scratch_change(batch_and_refs):
+    """A captured graph holds the ring scratch and the fast grid: after
+    gudni_set_blocks_per_cu reallocates them, the next replay must be
+    re-captured against the new buffers and still give the oracle's frames."""
+    r, b, scenes, refs = batch_and_refs
+    try:
+        for bpc in (2, 6, 0):
+            say(f"graph after set_blocks_per_cu({bpc})")
+            b.render(use_graph=True)
+            b.check()
+            r.set_blocks_per_cu(bpc)
+            b.render(use_graph=True)
+            b.check()
+            check_all(b, scenes, refs, f"replay after blocks_per_cu={bpc}")
+    finally:
+        r.set_blocks_per_cu(0)
+
+
 def test_batch_sequential_scenes(batch_and_refs):
     r, b, scenes, refs = batch_and_refs
     say("sequential per-scene launches")

@@ -387,3 +387,30 @@ def test_level1_job_writes_below_a_short_tile_box(rasterizer):
     frame = out.reshape(scene.height, scene.width)
     assert np.array_equal(frame, ref), H.first_mismatch(frame, ref)
     assert not np.any(frame[100:] == np.uint32(0xDEADBEEF)), "rows below the box did not reach the host"
+
+
+def test_level1_job_invalidates_the_uploaded_frame():
+    """A Level-1 job writes its records into the buffers an uploaded frame
+    renders from, so a frame render after it must refuse (until the frame is
+    uploaded again) rather than render the job's records; after a fresh
+    upload the frame equals the oracle's again."""
+    import torch
+    r = G.Rasterizer()
+    try:
+        scene = S.uniform_512(300, 256, 5)
+        _, geo, jobs = G.prepare(scene)
+        r.upload_common(geo.geo, scene)
+        r.frame_upload(jobs, scene)
+        tiles, shapes = jobs.job(0)
+        out = np.zeros(scene.width * scene.height, np.uint32)
+        say("frame_upload, then a level-1 job, then frame_render")
+        r.raster_job(shapes, tiles, scene.background, scene.width, scene.height, 0, jobs.n_jobs - 1, out)
+        dev = torch.zeros(scene.width * scene.height, dtype=torch.int32, device="cuda")
+        with pytest.raises(G.GudniError) as ei:
+            r.frame_render(dev.data_ptr())
+        assert ei.value.code == -1
+        got, _ = H.gpu_render_level2(r, scene)
+        ref = H.oracle_render(scene, want_f64=False)["out"]
+        assert H.first_mismatch(got, ref) is None
+    finally:
+        r.close()
