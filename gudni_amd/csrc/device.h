@@ -42,7 +42,8 @@ constexpr uint64_t kTagSolid = 0x8000000000000000ull;
 constexpr uint64_t kTagCompoundMask = 0x3000000000000000ull;
 constexpr uint64_t kTagContinue = 0x1000000000000000ull;
 constexpr uint64_t kTagAdd = 0x2000000000000000ull;
-constexpr uint64_t kTagIdMask = 0x0FFFFFFFFFFFFFFFull;
+constexpr uint64_t kTagIdMask = 0x07FFFFFFFFFFFFFFull;   // bit 59 is the fill rule
+constexpr uint64_t kTagNonzero = GUDNI_TAG_NONZERO;
 // Both type bits set: a gradient substance (an extension; the reference
 // knows solids and pictures only).
 constexpr uint64_t kTagGradient = 0xC000000000000000ull;

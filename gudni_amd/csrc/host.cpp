@@ -103,7 +103,10 @@ static inline int x_direction(const Quad &q)
 // Append one oriented strand of m <= 16 curves (Strand.hs:180-206):
 // header {u16 size in 8-byte units, u16 0, u32 0}, rightmost point, leftmost
 // point + its control, then the interior (point, control) pairs in tree order.
-static void emit_strand(const Quad *qs, int m, std::vector<uint8_t> &heap, uint32_t curve_base = 0)
+// flags: the header's second u16 (GUDNI_STRAND_REVERSED for a strand of a
+// nonzero-fill shape that runs against its outline; 0 otherwise).
+static void emit_strand(const Quad *qs, int m, std::vector<uint8_t> &heap, uint32_t curve_base = 0,
+                        uint16_t flags = 0)
 {
     const size_t base = heap.size();
     const size_t bytes = 16u * (size_t)(m + 1);
@@ -112,6 +115,7 @@ static void emit_strand(const Quad *qs, int m, std::vector<uint8_t> &heap, uint3
     const uint16_t size = (uint16_t)(bytes / 8);
     std::memset(w, 0, 8);
     std::memcpy(w, &size, 2);
+    std::memcpy(w + 2, &flags, 2);
     std::memcpy(w + 4, &curve_base, 4);   // zero unless curves are tracked
     float *f = reinterpret_cast<float *>(w + 8);
     f[0] = qs[m - 1].p1.x; f[1] = qs[m - 1].p1.y;
@@ -130,7 +134,7 @@ static void emit_strand(const Quad *qs, int m, std::vector<uint8_t> &heap, uint3
 // global index of the outline's first pair.
 static int outline_strands(const float *pairs, int k, std::vector<Quad> &work,
                            std::vector<uint8_t> &heap, std::vector<CurveSrc> *curves = nullptr,
-                           int pair_base = 0)
+                           int pair_base = 0, bool mark_reversed = false)
 {
     if (k < 2) return 0;
     work.clear();
@@ -176,15 +180,16 @@ static int outline_strands(const float *pairs, int k, std::vector<Quad> &work,
             } else {
                 std::copy(src, src + m, oriented);
             }
+            const uint16_t flags = (mark_reversed && reversed) ? GUDNI_STRAND_REVERSED : 0;
             if (curves) {
-                emit_strand(oriented, m, heap, (uint32_t)curves->size());
+                emit_strand(oriented, m, heap, (uint32_t)curves->size(), flags);
                 for (int q = 0; q < m; ++q) {
                     CurveSrc c = wsrc[(size_t)s + (size_t)(reversed ? m - 1 - q : q)];
                     if (reversed) c.flags |= kCurveReversed;
                     curves->push_back(c);
                 }
             } else {
-                emit_strand(oriented, m, heap);
+                emit_strand(oriented, m, heap, 0, flags);
             }
             ++strands;
         }
@@ -220,7 +225,8 @@ int build_geometry(const gudni_outlines *sc, int32_t cw, int32_t ch, Geometry &g
         for (int o = o0; o < o1; ++o) {
             const int a = sc->outline_pair_offsets[o], e = sc->outline_pair_offsets[o + 1];
             n += (uint32_t)outline_strands(sc->pairs + 4 * (size_t)a, e - a, work, g.heap,
-                                           track_curves ? &g.curves : nullptr, a);
+                                           track_curves ? &g.curves : nullptr, a,
+                                           (sc->tags[i] & GUDNI_TAG_NONZERO) != 0);
         }
         ShapeRec rec;
         rec.tag = sc->tags[i];

@@ -173,10 +173,12 @@ __device__ Runs count_runs(const float *p, int k)
 }
 
 // One strand of m curves (Strand.hs:180-206): header {u16 size in 8-byte
-// units, u16 0, u32 0} and the right end point, the left end point and its
+// units, u16 flags (GUDNI_STRAND_REVERSED, nonzero-fill shapes only), u32 0
+// or the first curve's index} and the right end point, the left end point and its
 // control, then the interior (point, control) pairs in tree order.  A chunk
 // that runs right to left is reversed first (reverseIfBackwards).
-__device__ void write_strand(const Quad *buf, int m, uint8_t *dst, const Orders &ord, uint32_t curve_base)
+__device__ void write_strand(const Quad *buf, int m, uint8_t *dst, const Orders &ord, uint32_t curve_base,
+                             bool mark_reversed)
 {
     const bool rev = buf[0].p0.x > buf[m - 1].p1.x;
     auto at = [&](int q) -> Quad {
@@ -186,7 +188,7 @@ __device__ void write_strand(const Quad *buf, int m, uint8_t *dst, const Orders 
     };
     const Quad first = at(0), last = at(m - 1);
     uint4 h;
-    h.x = (uint32_t)(2 * (m + 1));
+    h.x = (uint32_t)(2 * (m + 1)) | ((mark_reversed && rev) ? (uint32_t)GUDNI_STRAND_REVERSED << 16 : 0u);
     h.y = curve_base;   // 0 without tracking
     h.z = __float_as_uint(last.p1.x);
     h.w = __float_as_uint(last.p1.y);
@@ -202,7 +204,8 @@ __device__ void write_strand(const Quad *buf, int m, uint8_t *dst, const Orders 
 // their curves' records from curve `cbase` on (pair_base: the outline's
 // first pair in the scene).
 __device__ void emit_outline(const float *p, int k, const Runs &r, uint8_t *heap, uint64_t base,
-                             const Orders &ord, CurveSrc *curves, uint64_t cbase, int32_t pair_base)
+                             const Orders &ord, CurveSrc *curves, uint64_t cbase, int32_t pair_base,
+                             bool mark_reversed)
 {
     if (k < 2) return;
     uint64_t pos_last = base, pos_rest = base + r.last_units;
@@ -216,7 +219,7 @@ __device__ void emit_outline(const float *p, int k, const Runs &r, uint8_t *heap
         if (!nb) return;
         uint64_t &pos = in_last ? pos_last : pos_rest;
         uint64_t &cpos = in_last ? cpos_last : cpos_rest;
-        write_strand(buf, nb, heap + 16 * pos, ord, curves ? (uint32_t)cpos : 0u);
+        write_strand(buf, nb, heap + 16 * pos, ord, curves ? (uint32_t)cpos : 0u, mark_reversed);
         if (curves) {
             const bool rev = buf[0].p0.x > buf[nb - 1].p1.x;
             for (int q = 0; q < nb; ++q) {
@@ -333,7 +336,8 @@ __device__ void emit_body(const Args &a, const Orders &ord, int t0, int ts)
             const int e0 = a.ooff[o], e1 = a.ooff[o + 1];
             const float *p = a.pairs + 4 * (size_t)e0;
             const Runs rr = count_runs(p, e1 - e0);
-            emit_outline(p, e1 - e0, rr, a.heap, base, ord, a.curves, cbase, e0);
+            emit_outline(p, e1 - e0, rr, a.heap, base, ord, a.curves, cbase, e0,
+                         (a.tags[s] & dev::kTagNonzero) != 0);
             base += rr.units;
             cbase += rr.units - rr.strands;
         }

@@ -25,7 +25,10 @@ TAG_PICTURE = 0x4000000000000000
 TAG_GRADIENT = 0xC000000000000000
 TAG_ADD = 0x2000000000000000
 TAG_SUBTRACT = 0x3000000000000000
-TAG_ID_MASK = 0x0FFFFFFFFFFFFFFF
+TAG_ID_MASK = 0x07FFFFFFFFFFFFFF
+# fill rule bit (an extension; the reference fills by crossing parity only):
+# the shape is filled where its winding number is non-zero
+TAG_NONZERO = 0x0800000000000000
 
 # SplitMix64 seeds of the benchmark scenes ("gud1".."gud5" as integers)
 SEEDS = {"uniform_512": 0x67756431, "blobs_4096": 0x67756432, "batch_256": 0x67756433,
@@ -224,19 +227,29 @@ def random_colors(rng: SplitMix64, n: int) -> np.ndarray:
     return (b / np.float32(255.0)).astype(np.float32)
 
 
-def uniform_512(n_shapes: int = 1000, size: int = 512, seed: int = SEEDS["uniform_512"]) -> Scene:
+def nonzero(tag: int) -> int:
+    """The same shape tag with the nonzero fill rule."""
+    return int(tag) | TAG_NONZERO
+
+
+def uniform_512(n_shapes: int = 1000, size: int = 512, seed: int = SEEDS["uniform_512"],
+                fill: str = "half") -> Scene:
     """512x512, 1000 closed shapes of 3-12 quadratics whose on- and off-curve
-    points are uniform over the canvas; solid random RGBA8 colours.  The
-    reference fills even-odd only (Kernels.cl:298-304), so every shape here is
-    even-odd."""
+    points are uniform over the canvas; solid random RGBA8 colours.
+    fill="half": shapes with even index fill even-odd (the reference's only
+    rule, Kernels.cl:298-304), odd ones nonzero; fill="evenodd": all even-odd
+    (what the reference itself can render)."""
+    if fill not in ("half", "evenodd"):
+        raise ValueError("fill must be 'half' or 'evenodd'")
     rng = SplitMix64(seed)
     k = rng.integers(3, 12, n_shapes)
     total = int(k.sum())
     pts = (rng.uniform(total * 4) * size).astype(np.float32).reshape(total, 4)
     colors = random_colors(rng, n_shapes)
-    tags = [solid_tag(i) for i in range(n_shapes)]
+    tags = [solid_tag(i) if (fill == "evenodd" or i % 2 == 0) else nonzero(solid_tag(i))
+            for i in range(n_shapes)]
     return _assemble(size, size, tags, k, pts, colors, np.zeros(4, np.uint8),
-                     np.zeros((0, 24), np.uint8), "uniform_512")
+                     np.zeros((0, 24), np.uint8), "uniform_512" if fill == "half" else "uniform_512_evenodd")
 
 
 def noise_texture(rng: SplitMix64, side: int = 1024) -> np.ndarray:

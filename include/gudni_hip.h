@@ -63,6 +63,17 @@ typedef struct gudni_spec {
                                      pass.                                    */
 } gudni_spec;
 
+/* Fill rule (an extension; the reference fills every shape by crossing
+ * parity, Kernels.cl:298-304, 1515-1519).  A shape whose tag has this bit
+ * set is filled where its winding number is non-zero; substance ids are the
+ * tag's bits 58..0.  For such shapes the serialisers record, in bit 0 of a
+ * strand header's second u16 (zero in the reference's layout,
+ * Strand.hs:196-206), that the strand runs against its outline
+ * (reverseIfBackwards, Strand.hs:137-143 reversed it); strands of even-odd
+ * shapes keep the reference's bytes. */
+#define GUDNI_TAG_NONZERO      0x0800000000000000ull
+#define GUDNI_STRAND_REVERSED  0x0001u
+
 /* Picture sampling at the pixel's integer position p (Kernels.cl:1426):
  * s = p / scale - translate.  NEAREST takes the texel at rtz(s); BILINEAR
  * blends the four texels around s - 0.5 (texels outside the picture are

@@ -46,7 +46,7 @@
 #define D_FLAT 0.25
 #define D_MAX_BISECT 64
 #define D_QCAP (1 << 16)
-#define D_TAG_ID 0x0FFFFFFFFFFFFFFFull
+#define D_TAG_ID 0x07FFFFFFFFFFFFFFull
 #define D_POS 0x80000000u
 #define D_PERSIST 0x40000000u
 #define D_PERSIST_AND_SLOPE 0xC0000000u

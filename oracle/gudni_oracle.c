@@ -55,6 +55,10 @@
 #define O_PERSIST_BOTTOM   0x40000000u
 #define O_UNPERSIST        0xBFFFFFFFu
 #define O_SHAPEBIT_MASK    0x0FFFFFFFu
+/* fill rule extension: a record of a nonzero-fill shape carries O_NONZERO,
+ * and O_REVERSED when its strand runs against the outline */
+#define O_NONZERO          0x20000000u
+#define O_REVERSED         0x10000000u
 #define O_MINCROP          0.2f
 #define O_FLATNESS         0.25f
 #define O_STACK_SECTIONS   8
@@ -68,7 +72,10 @@
 #define TAG_COMPOUND_MASK      0x3000000000000000ull
 #define TAG_CONTINUE           0x1000000000000000ull
 #define TAG_ADD                0x2000000000000000ull
-#define TAG_SUBSTANCEID_MASK   0x0FFFFFFFFFFFFFFFull
+#define TAG_SUBSTANCEID_MASK   0x07FFFFFFFFFFFFFFull
+/* fill rule (extension): winding number non-zero instead of crossing parity */
+#define TAG_NONZERO            0x0800000000000000ull
+#define GO_STRAND_REVERSED     0x0001u
 /* Extension with no counterpart in the reference: a third substance type
  * (both type bits set) whose colour is a linear or radial gradient. */
 #define TAG_GRADIENT           0xC000000000000000ull
@@ -211,7 +218,7 @@ static int hcompare(const bez_t *z)
 }
 
 /* write one strand of m curves (already oriented) at out; returns bytes */
-static long write_strand(const bez_t *zs, int m, uint8_t *out, long cap)
+static long write_strand(const bez_t *zs, int m, uint8_t *out, long cap, uint16_t flags)
 {
     int npts = 2 * m + 1;
     long bytes = 8 + 8L * npts;
@@ -221,8 +228,8 @@ static long write_strand(const bez_t *zs, int m, uint8_t *out, long cap)
     pts[2*m] = zs[m-1].b;
     {
         uint16_t size = (uint16_t)(bytes / 8);
-        uint16_t z16 = 0; uint32_t z32 = 0;
-        memcpy(out, &size, 2); memcpy(out + 2, &z16, 2); memcpy(out + 4, &z32, 4);
+        uint32_t z32 = 0;
+        memcpy(out, &size, 2); memcpy(out + 2, &flags, 2); memcpy(out + 4, &z32, 4);
     }
     for (int i = 0; i < npts; i++) {
         v2 p = pts[go_reorder_entry(npts, i)];
@@ -233,7 +240,10 @@ static long write_strand(const bez_t *zs, int m, uint8_t *out, long cap)
 }
 
 /* splitShape for one outline (Strand.hs:153-178). */
-static long outline_to_strands(const float *pairs, int k, uint8_t *out, long cap, int32_t *nstr)
+/* nonzero != 0: a nonzero-fill shape, whose reversed strands are marked
+ * (GO_STRAND_REVERSED in the header's second u16) */
+static long outline_to_strands(const float *pairs, int k, uint8_t *out, long cap, int32_t *nstr,
+                               int nonzero)
 {
     long used = 0;
     if (k < 2) return 0;
@@ -263,14 +273,16 @@ static long outline_to_strands(const float *pairs, int k, uint8_t *out, long cap
             int m = imin(16, e - c);
             bez_t zs[16];
             memcpy(zs, beziers + c, sizeof(bez_t) * (size_t)m);
+            uint16_t flags = 0;
             if (zs[0].a.x > zs[m-1].b.x) {         /* reverseIfBackwards */
                 bez_t tmp[16];
                 for (int q = 0; q < m; q++) {
                     tmp[q].a = zs[m-1-q].b; tmp[q].c = zs[m-1-q].c; tmp[q].b = zs[m-1-q].a;
                 }
                 memcpy(zs, tmp, sizeof(bez_t) * (size_t)m);
+                if (nonzero) flags = GO_STRAND_REVERSED;
             }
-            long w = write_strand(zs, m, out + used, cap - used);
+            long w = write_strand(zs, m, out + used, cap - used, flags);
             if (w < 0) { free(beziers); free(run_start); return -1; }
             used += w; (*nstr)++;
         }
@@ -279,18 +291,24 @@ static long outline_to_strands(const float *pairs, int k, uint8_t *out, long cap
     return used;
 }
 
-long go_serialize_outlines(const float *pairs, const int32_t *ooff, int32_t n_outlines,
-                           uint8_t *out, long cap, int32_t *n_strands)
+static long serialize_outlines(const float *pairs, const int32_t *ooff, int32_t n_outlines,
+                               uint8_t *out, long cap, int32_t *n_strands, int nonzero)
 {
     long used = 0;
     *n_strands = 0;
     for (int o = 0; o < n_outlines; o++) {
         long w = outline_to_strands(pairs + 4L * ooff[o], ooff[o+1] - ooff[o],
-                                    out + used, cap - used, n_strands);
+                                    out + used, cap - used, n_strands, nonzero);
         if (w < 0) return -1;
         used += w;
     }
     return used;
+}
+
+long go_serialize_outlines(const float *pairs, const int32_t *ooff, int32_t n_outlines,
+                           uint8_t *out, long cap, int32_t *n_strands)
+{
+    return serialize_outlines(pairs, ooff, n_outlines, out, cap, n_strands, 0);
 }
 
 int go_serialize_scene(int32_t n_shapes, const uint64_t *tags, const int32_t *soff,
@@ -324,8 +342,9 @@ int go_serialize_scene(int32_t n_shapes, const uint64_t *tags, const int32_t *so
         /* excludeBox (Serialize.hs:97-104) */
         if (l >= cw || t >= ch || r <= 0.0f || b <= 0.0f) continue;
         int32_t ns = 0;
-        long w = go_serialize_outlines(pairs, ooff + soff[i], soff[i+1] - soff[i],
-                                       s->geo + used, cap - used, &ns);
+        long w = serialize_outlines(pairs, ooff + soff[i], soff[i+1] - soff[i],
+                                    s->geo + used, cap - used, &ns,
+                                    (tags[i] & TAG_NONZERO) != 0);
         if (w < 0) return -1;
         uint32_t start = (uint32_t)(used / 16);  /* GeoRef in 16-byte units */
         memcpy(s->shapes + 16 * kept, &tags[i], 8);
@@ -617,6 +636,9 @@ typedef struct {
     long guard;                     /* remaining section iterations */
     /* ShapeState */
     uint32_t shape_bits; int32_t *shape_indices; uint64_t stack[O_STACK_SECTIONS];
+    /* winding number per shape bit (nonzero-fill shapes only); a shape's
+     * stack bit is its winding number != 0 */
+    int32_t wind[128];
     /* accurate mode on a tile of <= maxShapes shapes, all added, with
      * distinct substances: colours by groups of 32 tile-local shapes */
     int grouped;
@@ -836,9 +858,15 @@ static void build_threshold_array(col_t *c, const uint8_t *geo, const uint8_t *s
         uint32_t geo_start, num_strands;
         memcpy(&geo_start, srec + 8, 4); memcpy(&num_strands, srec + 12, 4);
         const uint8_t *strand = geo + 16L * geo_start;
+        uint64_t tag; memcpy(&tag, srec, 8);
+        const int nonzero = (tag & TAG_NONZERO) != 0;
         int added = 0, enclosed_shape = 0;
+        int32_t w0 = 0;     /* winding number of the column's top-left corner */
         for (uint32_t s = 0; s < num_strands; s++) {
             uint16_t size; memcpy(&size, strand, 2);
+            uint16_t sflags; memcpy(&sflags, strand + 2, 2);
+            const int reversed = nonzero && (sflags & GO_STRAND_REVERSED);
+            const uint32_t rule = nonzero ? (O_NONZERO | (reversed ? O_REVERSED : 0u)) : 0u;
             int enclosed_strand = 0;
             trav_t l, r;
             /* traverseTree (Kernels.cl:1378-1408) */
@@ -853,11 +881,16 @@ static void build_threshold_array(col_t *c, const uint8_t *geo, const uint8_t *s
                 r.xpos = cl_min(1.0f, l.right.x);
                 search_tree(&l, strand + 32, tree_size, dx, dy, 1);
                 search_tree(&r, strand + 32, tree_size, dx, dy, 0);
-                spawn(c, c->shape_bits, &l, &r, &added, &enclosed_strand);
+                spawn(c, c->shape_bits | rule, &l, &r, &added, &enclosed_strand);
             }
             strand += 8L * size;
             enclosed_shape = enclosed_shape != enclosed_strand;
+            /* the strand crosses the column's left border above its top:
+             * coming down that border it adds the sign of its original x
+             * direction to the winding number */
+            if (enclosed_strand) w0 += reversed ? -1 : 1;
         }
+        if (nonzero) enclosed_shape = w0 != 0;
         if (enclosed_shape) {
             uint32_t b = c->shape_bits;
             c->stack[b >> 6] ^= (1ull << (b & 63));
@@ -865,6 +898,7 @@ static void build_threshold_array(col_t *c, const uint8_t *geo, const uint8_t *s
         if (added || enclosed_shape) {
             if (c->shape_bits < (uint32_t)c->spec->max_shapes)
                 c->shape_indices[c->shape_bits] = shape_index;
+            if (c->shape_bits < 128u) c->wind[c->shape_bits] = w0;
             c->shape_bits += 1;
         }
     }
@@ -907,10 +941,33 @@ typedef struct {
     float bg[4]; int absx, absy;
 } colorctx_t;
 
-static void pass_header(col_t *c, uint32_t h)
+/* Crossing record h: even-odd shapes flip their bit (passHeader,
+ * Kernels.cl:298-304); a nonzero-fill shape adds the crossing's signed
+ * winding change and its bit becomes winding != 0.  For an edge of original
+ * direction e crossed by a move m the change is sign(e.x m.y - e.y m.x):
+ *   CROSS_RIGHT  moving right through the record (horizontalAdvance):
+ *                -sign(e.y) = -1 if exactly one of reversed, positive slope
+ *   CROSS_LEFT   moving back left (the start of verticalAdvance): +that
+ *   CROSS_DOWN   moving down the column's left border through the record's
+ *                end there (persistent records): sign(e.x) = reversed ? -1 : 1 */
+enum { CROSS_RIGHT, CROSS_LEFT, CROSS_DOWN };
+static void pass_header(col_t *c, uint32_t h, int how)
 {
     uint32_t b = h & O_SHAPEBIT_MASK;
-    c->stack[b >> 6] ^= (1ull << (b & 63));
+    if (!(h & O_NONZERO)) {
+        c->stack[b >> 6] ^= (1ull << (b & 63));
+        return;
+    }
+    const int rev = (h & O_REVERSED) != 0, pos = (h & O_POSITIVE_SLOPE) != 0;
+    int d;
+    if (how == CROSS_DOWN) d = rev ? -1 : 1;
+    else {
+        d = (rev != pos) ? -1 : 1;
+        if (how == CROSS_LEFT) d = -d;
+    }
+    const int32_t w = c->wind[b & 127] += d;
+    if (w != 0) c->stack[b >> 6] |= (1ull << (b & 63));
+    else        c->stack[b >> 6] &= ~(1ull << (b & 63));
 }
 
 /* findTop (Kernels.cl:281-292) */
@@ -1200,13 +1257,13 @@ static float split_next(col_t *c, parse_t *p)
 static void vertical_advance(col_t *c, parse_t *p)
 {
     if (p->sx1 != 1.0f) return;
-    for (int i = 0; i < p->num_active; i++) pass_header(c, qh_get(c, i));
+    for (int i = 0; i < p->num_active; i++) pass_header(c, qh_get(c, i), CROSS_LEFT);
     float next_break = cl_min(c->float_height, p->pixel_y);
     float active_bottom = c->qlen > 0 ? q_get(c, 0).bottom : FLT_MAX;
     if (active_bottom == p->sy1) {
         while (p->num_active > 0) {
             uint32_t h = qh_get(c, 0);
-            if (hdr_persist_bot(h)) pass_header(c, h);
+            if (hdr_persist_bot(h)) pass_header(c, h, CROSS_DOWN);
             q_pop(c); p->num_active--;
         }
     }
@@ -1221,13 +1278,13 @@ static void vertical_advance(col_t *c, parse_t *p)
             next_bottom = cl_min(next_break, split_next(c, p));
             while (p->num_active > 0 && q_get(c, 0).top == q_get(c, 0).bottom) {
                 uint32_t h = qh_get(c, 0);
-                if (hdr_persist_top(h)) pass_header(c, h);
+                if (hdr_persist_top(h)) pass_header(c, h, CROSS_DOWN);
                 q_pop(c); p->num_active--;
             }
             for (int i = 0; i < p->num_active; i++) {
                 if (q_get(c, i).top > 0.0f) {
                     uint32_t h = qh_get(c, i);
-                    if (hdr_persist_top(h)) pass_header(c, h);
+                    if (hdr_persist_top(h)) pass_header(c, h, CROSS_DOWN);
                 }
             }
         }
@@ -1314,7 +1371,7 @@ static void render_column(col_t *c, const colorctx_t *cs0, uint32_t *out, float 
             for (int i = 0; i < 4; i++) { p.acc[i] += col[i] * adj; p.acc[4 + i] += adj; }
             for (int i = 0; i < 4; i++) p.dacc[i] += dcol[i] * dadj;
             p.dacc[4] += dadj;
-            if (p.current < p.num_active) pass_header(c, qh_get(c, p.current));
+            if (p.current < p.num_active) pass_header(c, qh_get(c, p.current), CROSS_RIGHT);
             p.current++;
         }
         float color[4];

@@ -97,7 +97,7 @@ def fd_samples(scene, target, n=64, seed=7, h_point=1e-2, h_color=1e-3, max_trie
     owner = pair_shape(scene)
     pairs = scene.pairs.reshape(-1, 4).astype(np.float64)
     rgba = np.asarray(scene.substances, np.float64)
-    sid = (np.asarray(scene.tags, np.uint64) & np.uint64(0x0FFFFFFFFFFFFFFF)).astype(np.int64)
+    sid = (np.asarray(scene.tags, np.uint64) & np.uint64(0x07FFFFFFFFFFFFFF)).astype(np.int64)
     out, tries = [], 0
     while len(out) < n and tries < max_tries:
         tries += 1
