@@ -827,7 +827,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) a
     float4 *q_thr = f.q_thr + (size_t)slot * cap * kWave + lane;
     uint32_t *q_hdr = f.q_hdr + (size_t)slot * cap * kWave + lane;
     float4 *orig = a.q_orig + (size_t)slot * 2 * cap * kWave + lane;
-    int32_t *sidx = f.shape_idx + (size_t)slot * f.max_shapes * kWave + lane;
+    int32_t *sidx = f.shape_idx + (size_t)slot * kSidxWords * kWave + lane;
     TileTable *tt = &tables[wave];
     float4 *dcol = dcols[wave];
     float4 *straight = straights[wave];
@@ -962,6 +962,10 @@ int check_scene(gudni_ctx *ctx, const gudni_adjoint_batch::Scene &sc, int s)
         if ((tag & dev::kTagCompoundMask) != dev::kTagAdd)
             return set_error(ctx, GUDNI_ERR_ARG, "adjoint scene " + std::to_string(s) +
                                                      ": every shape must be an added shape");
+        if (tag & dev::kTagNonzero)
+            return set_error(ctx, GUDNI_ERR_ARG, "adjoint scene " + std::to_string(s) +
+                                                     ": the adjoint fills by crossing parity only "
+                                                     "(no GUDNI_TAG_NONZERO shapes)");
         ids.push_back(tag & dev::kTagIdMask);
     }
     std::sort(ids.begin(), ids.end());

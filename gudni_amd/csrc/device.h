@@ -32,6 +32,16 @@ constexpr uint32_t kShapeBitMask = 0x0FFFFFFFu;
 // of the shape-bit field carry them; the adjoint keeps a record's origin in
 // the bits above.
 constexpr uint32_t kLabelMask = 0x7Fu;
+// Fill rule (extension): records of a nonzero-fill shape carry kNonzero, and
+// kReversed when their strand runs against the outline (bits 29 and 28,
+// which the reference leaves clear: its bit 29 "enable" is unused on this
+// path and its shape bits never exceed 127).
+constexpr uint32_t kNonzero = 0x20000000u;
+constexpr uint32_t kReversed = 0x10000000u;
+// Per-lane words of the shape scratch: [0, 128) shape indices of a general
+// tile's bits, [128, 256) winding numbers of nonzero-fill shapes by label.
+constexpr int kSidxWords = 256;
+constexpr int kWindOffset = 128;
 constexpr int kOriginShift = 7;
 constexpr uint32_t kOriginMask = 0x3FFFFFu;   // 22 bits: bits 7..28
 constexpr float kMinCrop = 0.2f;          // MINCROP (Kernels.cl:47)
@@ -416,12 +426,17 @@ __device__ void generate(const Frame &f, const SceneDesc &s, Queue &q, const Col
             reinterpret_cast<const GUDNI_CONST uint32_t *>(ro(s.shapes) + shape_index);
         uint64_t offset = 16ull * __builtin_amdgcn_readfirstlane(srec[2]);
         const uint32_t num_strands = __builtin_amdgcn_readfirstlane(srec[3]);
+        const bool nonzero = (__builtin_amdgcn_readfirstlane(srec[1]) & (uint32_t)(kTagNonzero >> 32)) != 0u;
         const uint32_t label = kCompact ? (uint32_t)n : shape_bits;
         bool added = false, enclosed_shape = false;
+        int32_t w0 = 0;   // winding number of the segment's top-left corner (nonzero fill)
         for (uint32_t si = 0; si < num_strands; ++si) {
             const GUDNI_CONST uint8_t *strand = ro(s.geo) + offset;
-            // u16 size in 8-byte units, then u16 0: read as one dword
-            const uint32_t size = *reinterpret_cast<const GUDNI_CONST uint32_t *>(strand) & 0xFFFFu;
+            // u16 size in 8-byte units, then u16 flags: read as one dword
+            const uint32_t head = *reinterpret_cast<const GUDNI_CONST uint32_t *>(strand);
+            const uint32_t size = head & 0xFFFFu;
+            const bool reversed = nonzero && ((head >> 16) & GUDNI_STRAND_REVERSED) != 0u;
+            const uint32_t rule = nonzero ? (kNonzero | (reversed ? kReversed : 0u)) : 0u;
             bool enclosed_strand = false;
             // traverseTree (Kernels.cl:1378-1408)
             const float2 right = ld2(strand + 8);
@@ -439,16 +454,21 @@ __device__ void generate(const Frame &f, const SceneDesc &s, Queue &q, const Col
                 const int tree_size = ((int)size - 4) / 2;
                 search_tree(l, strand + 32, tree_size, dx, dy, true);
                 search_tree(r, strand + 32, tree_size, dx, dy, false);
-                spawn(q, c, label, l, r, added, enclosed_strand);
+                spawn(q, c, label | rule, l, r, added, enclosed_strand);
             }
             offset += 8u * size;
             enclosed_shape = enclosed_shape != enclosed_strand;
+            // a strand crossing the left border above the segment adds the
+            // sign of its original x direction (coming down that border)
+            if (enclosed_strand) w0 += reversed ? -1 : 1;
         }
+        if (nonzero) enclosed_shape = w0 != 0;
         if (enclosed_shape) {
             if (label < 64u) stack0 ^= 1ull << label;
             else stack1 ^= 1ull << (label - 64u);
         }
         if (added || enclosed_shape) {
+            if (nonzero) sidx[(kWindOffset + (int)label) * kWave] = w0;
             if (kCompact) {
                 if (label < 64u) contrib0 |= 1ull << label;
                 else contrib1 |= 1ull << (label - 64u);
@@ -557,6 +577,12 @@ struct Sweep {
     uint32_t rnd;
 };
 
+// How a crossing of a record moves through it (the fill rule's sign):
+// kCrossRight moving right (horizontalAdvance), kCrossLeft moving back to
+// the column's left border (the start of verticalAdvance), kCrossDown moving
+// down that border through a persistent record's end there.
+enum Cross { kCrossRight, kCrossLeft, kCrossDown };
+
 struct Stack {
     uint64_t s0, s1;
     uint32_t dirty;   // groups of 32 bits touched since their composite was cached
@@ -565,6 +591,35 @@ struct Stack {
         const uint32_t b = h & kLabelMask;
         if (b < 64u) s0 ^= 1ull << b;
         else s1 ^= 1ull << (b - 64u);
+        dirty |= 1u << (b >> 5);
+    }
+    // Crossing record h: an even-odd shape flips its bit (passHeader,
+    // Kernels.cl:298-304).  A nonzero-fill shape adds the crossing's winding
+    // change, sign(e.x m.y - e.y m.x) for an edge of original direction e
+    // crossed by a move m, to its counter (wind[label * 64], this lane's) and
+    // its bit becomes counter != 0.
+    template <Cross kHow>
+    __device__ __forceinline__ void cross(uint32_t h, int32_t *wind)
+    {
+        if (!(h & kNonzero)) {
+            flip(h);
+            return;
+        }
+        const bool rev = (h & kReversed) != 0u;
+        int d;
+        if (kHow == kCrossDown) {
+            d = rev ? -1 : 1;
+        } else {
+            d = (rev != ((h & kPositiveSlope) != 0u)) ? -1 : 1;
+            if (kHow == kCrossLeft) d = -d;
+        }
+        const uint32_t b = h & kLabelMask;
+        int32_t *w = wind + (int)b * kWave;
+        const int32_t v = *w + d;
+        *w = v;
+        const uint64_t m = 1ull << (b & 63u);
+        uint64_t &word = b < 64u ? s0 : s1;
+        word = v != 0 ? (word | m) : (word & ~m);
         dirty |= 1u << (b >> 5);
     }
 };
@@ -615,17 +670,18 @@ __device__ float split_next(Queue &q, Sweep &p)
     return slice;
 }
 
-// verticalAdvance (Kernels.cl:1744-1824)
-__device__ void vertical_advance(Queue &q, const Column &c, Sweep &p, Stack &st)
+// verticalAdvance (Kernels.cl:1744-1824).  wind: this lane's winding
+// counters (nonzero-fill shapes; may be null when no record has kNonzero).
+__device__ void vertical_advance(Queue &q, const Column &c, Sweep &p, Stack &st, int32_t *wind = nullptr)
 {
     if (p.sx1 != 1.0f) return;
-    for (int i = 0; i < p.num_active; ++i) st.flip(q.get_h(i));
+    for (int i = 0; i < p.num_active; ++i) st.cross<kCrossLeft>(q.get_h(i), wind);
     const float next_break = cl_min(c.float_height, p.pixel_y);
     const float active_bottom = q.len > 0 ? q.get(0).bottom : FLT_MAX;
     if (active_bottom == p.sy1) {
         while (p.num_active > 0) {
             const uint32_t h = q.get_h(0);
-            if (hdr_persist_bottom(h)) st.flip(h);
+            if (hdr_persist_bottom(h)) st.cross<kCrossDown>(h, wind);
             q.pop();
             --p.num_active;
         }
@@ -644,14 +700,14 @@ __device__ void vertical_advance(Queue &q, const Column &c, Sweep &p, Stack &st)
                 const Thr t0 = q.get(0);
                 if (!(t0.top == t0.bottom)) break;
                 const uint32_t h = q.get_h(0);
-                if (hdr_persist_top(h)) st.flip(h);
+                if (hdr_persist_top(h)) st.cross<kCrossDown>(h, wind);
                 q.pop();
                 --p.num_active;
             }
             for (int i = 0; i < p.num_active; ++i) {
                 if (q.get(i).top > 0.0f) {
                     const uint32_t h = q.get_h(i);
-                    if (hdr_persist_top(h)) st.flip(h);
+                    if (hdr_persist_top(h)) st.cross<kCrossDown>(h, wind);
                 }
             }
         }
@@ -965,8 +1021,9 @@ __device__ __forceinline__ uint32_t pixel_word(float4 c, bool rgba)
 // (Kernels.cl:1724-1742, 1826-1851, 1881-1916, 1978-2028)
 template <bool kCompact>
 __device__ uint32_t render(const Frame &f, const SceneDesc &s, Queue &q, const Column &c, Stack st,
-                           const int32_t *sidx, TileTable *tt)
+                           int32_t *sidx, TileTable *tt)
 {
+    int32_t *const wind = sidx + kWindOffset * kWave;
     Sweep p;
     p.current = 0; p.num_active = 0;
     p.sx0 = 0.0f; p.sy0 = 0.0f; p.sx1 = 1.0f; p.sy1 = 0.0f;
@@ -997,7 +1054,7 @@ __device__ uint32_t render(const Frame &f, const SceneDesc &s, Queue &q, const C
         }
         while ((p.sx1 < 1.0f) || (p.sy1 < p.pixel_y)) {
             if (--guard < 0) return kStatusGuard;
-            vertical_advance(q, c, p, st);
+            vertical_advance(q, c, p, st, wind);
             if (q.spilled || q.overflow) return 0;
             // horizontalAdvance with thresholdMidXLow (Kernels.cl:916-926)
             float nx;
@@ -1040,7 +1097,7 @@ __device__ uint32_t render(const Frame &f, const SceneDesc &s, Queue &q, const C
             acc.z += col.z * adj;
             acc.w += col.w * adj;
             area_sum += adj;
-            if (p.current < p.num_active) st.flip(q.get_h(p.current));
+            if (p.current < p.num_active) st.cross<kCrossRight>(q.get_h(p.current), wind);
             ++p.current;
         }
         const float4 color = make_float4(acc.x / area_sum, acc.y / area_sum, acc.z / area_sum,

@@ -46,7 +46,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) r
     const int cap = 1 << f.cap_log2;
     float4 *q_thr = f.q_thr + (size_t)slot * cap * kWave + lane;
     uint32_t *q_hdr = f.q_hdr + (size_t)slot * cap * kWave + lane;
-    int32_t *sidx = f.shape_idx + (size_t)slot * f.max_shapes * kWave + lane;
+    int32_t *sidx = f.shape_idx + (size_t)slot * kSidxWords * kWave + lane;
     TileTable *tt = &tables[wave];
     const int n_work = from_retry ? (int)min((uint32_t)f.retry_capacity, f.retry[0]) : f.n_work;
     Claimer claim;
@@ -202,13 +202,12 @@ int ensure_scratch(gudni_ctx *ctx)
     int slog = 0;
     while ((1 << slog) < ctx->spec.max_thresholds) ++slog;
     const size_t scap = (size_t)1 << slog;
-    const size_t ms = (size_t)ctx->spec.max_shapes;
     CHECK_HIP(ctx, ctx->fast_thr.ensure(fast_slots * fcap * 64 * 16));
     CHECK_HIP(ctx, ctx->fast_hdr.ensure(fast_slots * fcap * 64 * 4));
-    CHECK_HIP(ctx, ctx->fast_sidx.ensure(fast_slots * ms * 64 * 4));
+    CHECK_HIP(ctx, ctx->fast_sidx.ensure(fast_slots * dev::kSidxWords * 64 * 4));
     CHECK_HIP(ctx, ctx->slow_thr.ensure(slow_slots * scap * 64 * 16));
     CHECK_HIP(ctx, ctx->slow_hdr.ensure(slow_slots * scap * 64 * 4));
-    CHECK_HIP(ctx, ctx->slow_sidx.ensure(slow_slots * ms * 64 * 4));
+    CHECK_HIP(ctx, ctx->slow_sidx.ensure(slow_slots * dev::kSidxWords * 64 * 4));
     return GUDNI_OK;
 }
 
