@@ -47,6 +47,8 @@
 #define D_MAX_BISECT 64
 #define D_QCAP (1 << 16)
 #define D_TAG_ID 0x07FFFFFFFFFFFFFFull
+#define D_TAG_TYPE 0xC000000000000000ull
+#define D_TAG_PICTURE 0x4000000000000000ull
 #define D_POS 0x80000000u
 #define D_PERSIST 0x40000000u
 #define D_PERSIST_AND_SLOPE 0xC0000000u
@@ -76,6 +78,9 @@ typedef struct {
     dq *curves;
     dsrc *src;            /* per curve */
     uint64_t sid;
+    uint64_t tag;
+    double xmin, xmax;    /* strands' left-most and right-most end points */
+    double ymin, ymax;    /* over every curve point (end points and controls) */
 } dshape;
 
 static d2 lerp2(double t, d2 p, d2 q) { d2 r = { p.x * (1 - t) + q.x * t, p.y * (1 - t) + q.y * t }; return r; }
@@ -153,6 +158,20 @@ static void build_shape(dshape *s, const double *pairs, const int32_t *ooff, int
         free(cuts);
     }
     s->strand_start[s->n_strands] = nc;
+    s->xmin = DBL_MAX; s->xmax = -DBL_MAX;
+    s->ymin = DBL_MAX; s->ymax = -DBL_MAX;
+    for (int i = 0; i < nc; i++) {
+        const double ys[3] = { s->curves[i].a.y, s->curves[i].c.y, s->curves[i].b.y };
+        for (int k = 0; k < 3; k++) {
+            if (ys[k] < s->ymin) s->ymin = ys[k];
+            if (ys[k] > s->ymax) s->ymax = ys[k];
+        }
+    }
+    for (int k = 0; k < s->n_strands; k++) {
+        const double lx = s->curves[s->strand_start[k]].a.x, rx = s->curves[s->strand_start[k + 1] - 1].b.x;
+        if (lx < s->xmin) s->xmin = lx;
+        if (rx > s->xmax) s->xmax = rx;
+    }
     free(work);
     free(wsrc);
 }
@@ -299,7 +318,9 @@ static void insert_rec(dcol *c, drec t)
     c->q[c->qstart + cur] = t;
 }
 
-static uint64_t *g_split_sig;   /* go_diff_loss folds split decisions into its signature */
+/* go_diff_loss folds split decisions into its signature; one per thread, so
+ * losses may be evaluated in parallel threads */
+static _Thread_local uint64_t *g_split_sig;
 
 static double split_next(dcol *c, dsweep *p)
 {
@@ -370,8 +391,59 @@ static void vertical_advance(dcol *c, dsweep *p)
     p->current = 0;
 }
 
+/* Picture substances (readColor / getPicturePixel, Kernels.cl:856-860,
+ * 1411-1442, and the build's bilinear filter): the texel colour at the
+ * pixel's integer position (ax, ay), the sampling position computed in FP32
+ * as the raster path computes it, the blend in FP64.  Texels outside the
+ * picture are transparent. */
+typedef struct {
+    const float *subs;          /* raw substance records (4 floats each)   */
+    const uint8_t *pict;        /* RGBA8 pile                              */
+    const uint8_t *uses;        /* 24-byte PictureUse records              */
+    int32_t n_uses;
+    int32_t filter;             /* 0 nearest, 1 bilinear                   */
+} dpict;
+
+static void texel64(const dpict *pc, const uint8_t *use, long long x, long long y, double *out)
+{
+    int32_t w, h, off; memcpy(&w, use + 8, 4); memcpy(&h, use + 12, 4); memcpy(&off, use + 16, 4);
+    if (x < 0 || y < 0 || x >= w || y >= h) { out[0] = out[1] = out[2] = out[3] = 0.0; return; }
+    const uint8_t *px = pc->pict + off + 4 * (y * (long long)w + x);
+    for (int i = 0; i < 4; i++) out[i] = (double)px[i] / 255.0;
+}
+
+static int f32_to_int(float v, int floor_mode)
+{
+    if (!(v == v)) return 0;
+    if (v >= 2147483647.0f) return 2147483647;
+    if (v <= -2147483648.0f) return (-2147483647 - 1);
+    return floor_mode ? (int)floorf(v) : (int)v;
+}
+
+static void picture_color(const dpict *pc, uint64_t sid, int ax, int ay, double *out)
+{
+    uint32_t id; memcpy(&id, pc->subs + 4 * sid, 4);
+    if ((int32_t)id < 0 || (int32_t)id >= pc->n_uses) { out[0] = out[1] = out[2] = out[3] = 0.0; return; }
+    const uint8_t *use = pc->uses + 24L * id;
+    float tx, ty, scale; memcpy(&tx, use, 4); memcpy(&ty, use + 4, 4); memcpy(&scale, use + 20, 4);
+    scale = scale < 0.0000001f ? 0.0000001f : scale;
+    const float sx = ((float)ax / scale) - tx, sy = ((float)ay / scale) - ty;
+    if (pc->filter == 1) {
+        const float fx = sx - 0.5f, fy = sy - 0.5f;
+        const int ix = f32_to_int(fx, 1), iy = f32_to_int(fy, 1);
+        const double wx = (double)(fx - (float)ix), wy = (double)(fy - (float)iy);
+        double t0[4], t1[4], t2[4], t3[4];
+        texel64(pc, use, ix, iy, t0); texel64(pc, use, (long long)ix + 1, iy, t1);
+        texel64(pc, use, ix, (long long)iy + 1, t2); texel64(pc, use, (long long)ix + 1, (long long)iy + 1, t3);
+        for (int i = 0; i < 4; i++)
+            out[i] = ((t0[i] * (1 - wx)) + (t1[i] * wx)) * (1 - wy) + ((t2[i] * (1 - wx)) + (t3[i] * wx)) * wy;
+        return;
+    }
+    texel64(pc, use, f32_to_int(sx, 0), f32_to_int(sy, 0), out);
+}
+
 static void section_color(const dcol *c, int n_shapes, const dshape *shapes, const double *rgba,
-                          const double *bg, double *out)
+                          const double *bg, double *out, const dpict *pc, int ax, int ay)
 {
     double P[3] = { 0, 0, 0 }, A = 0.0;
     int done = 0;
@@ -382,6 +454,11 @@ static void section_color(const dcol *c, int n_shapes, const dshape *shapes, con
             int s = 64 * w + b;
             if (s >= n_shapes) break;
             const double *col = rgba + 4 * shapes[s].sid;
+            double pcol[4];
+            if (pc && (shapes[s].tag & D_TAG_TYPE) == D_TAG_PICTURE) {
+                picture_color(pc, shapes[s].sid, ax, ay, pcol);
+                col = pcol;
+            }
             double t = 1.0 - A;
             for (int i = 0; i < 3; i++) P[i] += col[i] * col[3] * t;
             A += col[3] * t;
@@ -398,22 +475,43 @@ static void section_color(const dcol *c, int n_shapes, const dshape *shapes, con
     out[3] = A;
 }
 
-int go_diff_loss(int32_t n_shapes, const uint64_t *tags, const int32_t *soff, const int32_t *ooff,
-                 const double *pairs, const double *rgba, const double *bg, int32_t W, int32_t H,
-                 const float *target, int32_t x0, int32_t y0, int32_t x1, int32_t y1, int32_t seg,
-                 double *loss_out, double *image_out, uint64_t *signature)
+static int diff_loss(int32_t n_shapes, const uint64_t *tags, const int32_t *soff, const int32_t *ooff,
+                     const double *pairs, const double *rgba, const double *bg, int32_t W, int32_t H,
+                     const float *target, int32_t x0, int32_t y0, int32_t x1, int32_t y1, int32_t seg,
+                     const int32_t *seg_map, const dpict *pc,
+                     double *loss_out, double *image_out, uint64_t *signature)
 {
-    /* seg > 0: columns cut into segments of seg rows at multiples of seg
-     * (the window must start on one and end on one or at the canvas edge);
-     * seg == 0: one segment over the window's rows */
-    if (seg < 0 || (seg > 0 && (y0 % seg != 0 || (y1 % seg != 0 && y1 != H)))) return -3;
+    /* seg_map (W*H, may be NULL): the height of the column segment each
+     * pixel belongs to (a power of two; the tile tree's 2^dd), segments
+     * starting at multiples of it: each column runs from the segment that
+     * holds row y0 through the one that holds row y1 - 1 (the loss counts
+     * every row of those segments, the image only the window's).
+     * Without a map: seg > 0 cuts the columns into segments of seg rows at
+     * multiples of seg (the window must start on one and end on one or at
+     * the canvas edge); seg == 0: one segment over the window's rows */
+    if (!seg_map && (seg < 0 || (seg > 0 && (y0 % seg != 0 || (y1 % seg != 0 && y1 != H))))) return -3;
+    if (seg_map) {
+        for (int x = x0; x < x1; x++)
+            for (int y = y0; y < y1; y++) {
+                const int32_t h = seg_map[(long)y * W + x];
+                if (h <= 0 || (h & (h - 1))) return -3;
+            }
+    }
     const int step = seg > 0 ? seg : (y1 - y0 > 0 ? y1 - y0 : 1);
     uint64_t sig = 0xcbf29ce484222325ull;
     dshape *shapes = calloc((size_t)(n_shapes > 0 ? n_shapes : 1), sizeof(dshape));
     for (int i = 0; i < n_shapes; i++) {
         build_shape(&shapes[i], pairs, ooff, soff[i], soff[i + 1], W, H, &sig);
         shapes[i].sid = tags[i] & D_TAG_ID;
+        shapes[i].tag = tags[i];
     }
+    for (int i = 0; i < n_shapes; i++)   /* gradients and nonzero fill are not modelled */
+        if ((tags[i] & D_TAG_TYPE) == D_TAG_TYPE || (tags[i] & 0x0800000000000000ull) ||
+            ((tags[i] & D_TAG_TYPE) == D_TAG_PICTURE && !pc)) {
+            for (int k = 0; k < n_shapes; k++) { free(shapes[k].curves); free(shapes[k].strand_start); free(shapes[k].src); }
+            free(shapes);
+            return -4;
+        }
     int nwords = (n_shapes + 63) / 64 + 1;
     dcol c;
     c.q = malloc(sizeof(drec) * D_QCAP);
@@ -421,17 +519,40 @@ int go_diff_loss(int32_t n_shapes, const uint64_t *tags, const int32_t *soff, co
     c.nwords = nwords;
     double loss = 0.0;
     int rc = 0;
-    for (int x = x0; x < x1 && rc == 0; x++)
-    for (int sy = y0; sy < y1 && rc == 0; sy += step) {
-        const int sy1 = sy + step < y1 ? sy + step : y1;
+    int *cand = malloc(sizeof(int) * (size_t)(n_shapes > 0 ? n_shapes : 1));
+    for (int x = x0; x < x1 && rc == 0; x++) {
+    /* the shapes some strand of which can reach column x (checkInRange),
+     * newest first, as the tile lists hold them */
+    int n_cand = 0;
+    for (int si = n_shapes - 1; si >= 0; si--) {
+        const dshape *s = &shapes[si];
+        if (s->kept && !(s->xmin - (double)x > 1.0 || !(s->xmax - (double)x > 0.0))) cand[n_cand++] = si;
+    }
+    for (int sy = seg_map ? y0 - y0 % seg_map[(long)y0 * W + x] : y0, sy1 = 0; sy < y1 && rc == 0; sy = sy1) {
+        if (seg_map) {
+            const int h = seg_map[(long)sy * W + x];
+            sy1 = sy + h < H ? sy + h : H;
+        } else {
+            sy1 = sy + step < y1 ? sy + step : y1;
+        }
         c.height_i = sy1 - sy; c.fh = (double)(sy1 - sy);
         const double dx = x, dy = sy;
         c.qstart = D_QCAP; c.qlen = 0; c.overflow = 0;
         memset(c.stack, 0, sizeof(uint64_t) * nwords);
-        /* shapes newest first, as the tile lists hold them */
-        for (int si = n_shapes - 1; si >= 0; si--) {
+        uint64_t skipped_y = 0;
+        /* shapes newest first; a shape no strand of which reaches the column
+         * (checkInRange for each) adds nothing and encloses nothing */
+        for (int ci = 0; ci < n_cand; ci++) {
+            const int si = cand[ci];
             dshape *s = &shapes[si];
-            if (!s->kept) continue;
+            /* every curve point lies below the segment: every record starts
+             * at or below its bottom (not added) and none crosses the left
+             * border above its top.  Every point lies above the segment: no
+             * record reaches into it, and the closed outlines cross the left
+             * border above the top an even number of times, so their
+             * enclosure flips cancel.  Either way the shape changes nothing;
+             * how many shapes are skipped is a decision of the signature. */
+            if (s->ymin - dy >= c.fh || s->ymax - dy < 0.0) { skipped_y++; continue; }
             int added = 0, enclosed_shape = 0;
             for (int k = 0; k < s->n_strands; k++) {
                 const dq *cs = s->curves + s->strand_start[k];
@@ -451,6 +572,7 @@ int go_diff_loss(int32_t n_shapes, const uint64_t *tags, const int32_t *soff, co
             }
             if (enclosed_shape) flip(&c, si);
         }
+        sig = fnv(sig, skipped_y);
         if (c.overflow) { rc = -1; break; }
         /* stable insertion sort of the queue (front first) */
         for (int i = 1; i < c.qlen; i++) {
@@ -484,7 +606,7 @@ int go_diff_loss(int32_t n_shapes, const uint64_t *tags, const int32_t *soff, co
                 }
                 p.sx0 = p.sx1; p.sx1 = nx;
                 double col[4];
-                section_color(&c, n_shapes, shapes, rgba, bg, col);
+                section_color(&c, n_shapes, shapes, rgba, bg, col, pc, x, sy + yi);
                 double area = (p.sx1 - p.sx0) * (p.sy1 - p.sy0);
                 for (int i = 0; i < 4; i++) acc[i] += col[i] * area;
                 area_sum += area;
@@ -494,21 +616,45 @@ int go_diff_loss(int32_t n_shapes, const uint64_t *tags, const int32_t *soff, co
             }
             sig = fnv(sig, (uint64_t)sections);
             long pix = (long)(sy + yi) * W + x;
+            const int in_window = sy + yi >= y0 && sy + yi < y1;
             for (int i = 0; i < 4; i++) {
                 double v = acc[i] / area_sum;
                 double e = v - (double)target[4 * pix + i];
                 loss += e * e;
-                if (image_out) image_out[4 * ((long)(sy - y0 + yi) * (x1 - x0) + (x - x0)) + i] = v;
+                if (image_out && in_window) image_out[4 * ((long)(sy - y0 + yi) * (x1 - x0) + (x - x0)) + i] = v;
             }
             p.sx0 = 0.0; p.sy0 = p.pixel_y;
         }
     }
+    }
+    free(cand);
     g_split_sig = NULL;
     for (int i = 0; i < n_shapes; i++) { free(shapes[i].curves); free(shapes[i].strand_start); free(shapes[i].src); }
     free(shapes); free(c.q); free(c.stack);
     *loss_out = loss;
     if (signature) *signature = sig;
     return rc;
+}
+
+int go_diff_loss(int32_t n_shapes, const uint64_t *tags, const int32_t *soff, const int32_t *ooff,
+                 const double *pairs, const double *rgba, const double *bg, int32_t W, int32_t H,
+                 const float *target, int32_t x0, int32_t y0, int32_t x1, int32_t y1, int32_t seg,
+                 double *loss_out, double *image_out, uint64_t *signature)
+{
+    return diff_loss(n_shapes, tags, soff, ooff, pairs, rgba, bg, W, H, target, x0, y0, x1, y1, seg,
+                     NULL, NULL, loss_out, image_out, signature);
+}
+
+int go_diff_loss_ex(int32_t n_shapes, const uint64_t *tags, const int32_t *soff, const int32_t *ooff,
+                    const double *pairs, const double *rgba, const double *bg, int32_t W, int32_t H,
+                    const float *target, int32_t x0, int32_t y0, int32_t x1, int32_t y1, int32_t seg,
+                    const int32_t *seg_map, const float *substances, const uint8_t *pict_data,
+                    const uint8_t *pict_uses, int32_t n_pict_uses, int32_t filter,
+                    double *loss_out, double *image_out, uint64_t *signature)
+{
+    dpict pc = { substances, pict_data, pict_uses, n_pict_uses, filter };
+    return diff_loss(n_shapes, tags, soff, ooff, pairs, rgba, bg, W, H, target, x0, y0, x1, y1, seg,
+                     seg_map, substances ? &pc : NULL, loss_out, image_out, signature);
 }
 
 
@@ -852,7 +998,7 @@ int go_diff_grad(int32_t n_shapes, const uint64_t *tags, const int32_t *soff, co
                     }
                     p.sx0 = p.sx1; p.sx1 = nx;
                     double col[4];
-                    section_color(&c, n_shapes, shapes, rgba, bg, col);
+                    section_color(&c, n_shapes, shapes, rgba, bg, col, NULL, 0, 0);
                     double ddy = p.sy1 - p.sy0;
                     double area = (p.sx1 - p.sx0) * ddy;
                     if (pass == 1) {

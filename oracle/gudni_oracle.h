@@ -149,6 +149,23 @@ int go_diff_loss(int32_t n_shapes, const uint64_t *tags, const int32_t *shape_ou
                  int32_t x0, int32_t y0, int32_t x1, int32_t y1, int32_t seg, double *loss,
                  double *image, uint64_t *signature);
 
+/* go_diff_loss with (a) seg_map (may be NULL): W*H int32, the height of the
+ * column segment each pixel belongs to (powers of two, segments at
+ * multiples of it, as a tile tree of mixed tile sizes cuts the columns):
+ * each column of the window runs over the whole segments holding its rows;
+ * and (b) picture substances: substances (raw 16-byte records; a picture's
+ * first word is its PictureUse index), the picture pile and PictureUse
+ * records, sampled nearest (filter 0) or bilinear (1) at the pixel's
+ * integer position.  substances NULL: solid colours only.  Returns -4 for
+ * gradient substances or nonzero-fill shapes (not modelled). */
+int go_diff_loss_ex(int32_t n_shapes, const uint64_t *tags, const int32_t *shape_outline_offsets,
+                    const int32_t *outline_pair_offsets, const double *pairs, const double *rgba,
+                    const double *background, int32_t width, int32_t height, const float *target,
+                    int32_t x0, int32_t y0, int32_t x1, int32_t y1, int32_t seg,
+                    const int32_t *seg_map, const float *substances, const uint8_t *pict_data,
+                    const uint8_t *pict_uses, int32_t n_pict_uses, int32_t filter,
+                    double *loss, double *image, uint64_t *signature);
+
 /* The adjoint's algorithm restated in FP64 over the whole canvas, columns
  * cut into segments of `seg` rows: loss, dL/d(outline points) (4 doubles per
  * pair) and dL/d(substance RGBA). */

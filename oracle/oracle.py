@@ -93,6 +93,13 @@ def lib():
                                    C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
                                    C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                    C.POINTER(C.c_double), C.c_void_p, C.POINTER(C.c_uint64)]
+        L.go_diff_loss_ex.restype = C.c_int
+        L.go_diff_loss_ex.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
+                                      C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                      C.c_int32, C.POINTER(C.c_double), C.c_void_p,
+                                      C.POINTER(C.c_uint64)]
         L.go_diff_grad.restype = C.c_int
         L.go_diff_grad.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
@@ -316,6 +323,38 @@ def diff_loss(scene, pairs, rgba, target, window, want_image=False, seg=0):
                             x0, y0, x1, y1, int(seg), C.byref(loss), _ptr(img), C.byref(sig))
     if rc != 0:
         raise RuntimeError(f"go_diff_loss failed ({rc})")
+    return loss.value, img, sig.value
+
+
+def diff_loss_ex(scene, pairs, rgba, target, window, seg_map=None, filter=0, want_image=False, seg=0):
+    """diff_loss with a per-pixel segment-height map (H, W) int32 (columns
+    cut as a tile tree of mixed tile sizes cuts them; each column runs over
+    the whole segments holding the window's rows) and picture substances
+    (the scene's pictures, sampled nearest (filter 0) or bilinear (1))."""
+    x0, y0, x1, y1 = [int(v) for v in window]
+    tags = np.ascontiguousarray(scene.tags, np.uint64)
+    soff = np.ascontiguousarray(scene.shape_outline_offsets, np.int32)
+    ooff = np.ascontiguousarray(scene.outline_pair_offsets, np.int32)
+    pairs = np.ascontiguousarray(pairs, np.float64)
+    rgba = np.ascontiguousarray(rgba, np.float64)
+    bg = np.ascontiguousarray(scene.background, np.float64)
+    target = np.ascontiguousarray(target, np.float32)
+    smap = None if seg_map is None else np.ascontiguousarray(seg_map, np.int32)
+    if smap is not None and smap.shape != (scene.height, scene.width):
+        raise ValueError("seg_map must be (height, width)")
+    subs = np.ascontiguousarray(scene.substances, np.float32)
+    pict = np.ascontiguousarray(scene.pict_data, np.uint8)
+    uses = np.ascontiguousarray(scene.pict_uses, np.uint8)
+    img = np.zeros((y1 - y0, x1 - x0, 4), np.float64) if want_image else None
+    loss = C.c_double()
+    sig = C.c_uint64()
+    rc = lib().go_diff_loss_ex(scene.n_shapes, _ptr(tags), _ptr(soff), _ptr(ooff), _ptr(pairs),
+                               _ptr(rgba), _ptr(bg), scene.width, scene.height, _ptr(target),
+                               x0, y0, x1, y1, int(seg), _ptr(smap), _ptr(subs),
+                               _ptr(pict) if pict.size else None, _ptr(uses) if uses.size else None,
+                               uses.size // 24, int(filter), C.byref(loss), _ptr(img), C.byref(sig))
+    if rc != 0:
+        raise RuntimeError(f"go_diff_loss_ex failed ({rc})")
     return loss.value, img, sig.value
 
 

@@ -181,3 +181,73 @@ def test_fp64_adjoint_closed_forms():
     right = [i for i in range(len(pts)) if abs(pts[i, 0] - 12.5) < 1e-6]
     got = sum(dp[i, 0] for i in right) + sum(dp[i, 2] for i in range(len(pts)) if abs(pts[i, 2] - 12.5) < 1e-6)
     assert abs(got - expect) < 1e-9, (got, expect)
+
+
+def fp64_image_ex(scene, seg_map=None, filt=0):
+    target = np.zeros((scene.height, scene.width, 4), np.float32)
+    _, img, _ = O.diff_loss_ex(scene, scene.pairs.reshape(-1, 4), np.asarray(scene.substances, np.float64),
+                               target, (0, 0, scene.width, scene.height), seg_map=seg_map, filter=filt,
+                               want_image=True)
+    return img
+
+
+def test_segment_map_of_one_height_equals_uniform_segments():
+    sc = polygon_scene(60, 128, 9)
+    target = np.zeros((sc.height, sc.width, 4), np.float32)
+    pairs, rgba = sc.pairs.reshape(-1, 4), np.asarray(sc.substances, np.float64)
+    a = O.diff_loss(sc, pairs, rgba, target, (0, 0, 128, 128), want_image=True, seg=16)
+    b = O.diff_loss_ex(sc, pairs, rgba, target, (0, 0, 128, 128), want_image=True,
+                       seg_map=np.full((128, 128), 16, np.int32))
+    assert a[0] == b[0] and a[2] == b[2]
+    np.testing.assert_array_equal(a[1], b[1])
+
+
+def test_segment_map_follows_a_mixed_tile_tree():
+    """A tile tree of mixed tile sizes cuts columns into segments of
+    different heights; the FP64 forward cut by the tree's segment map gives
+    the oracle's pixels, which are rendered through those very tiles."""
+    import gudni_amd as G
+    rng = np.random.default_rng(17)
+    shapes, subs = [], []
+    for i in range(700):   # crowded in one corner, sparse elsewhere
+        k = int(rng.integers(3, 9))
+        crowded = i < 600
+        c = rng.uniform(0, 120 if crowded else 512, 2)
+        r = rng.uniform(2, 10) if crowded else rng.uniform(5, 40)
+        ang = np.sort(rng.uniform(0, 2 * np.pi, k))
+        shapes.append((S.solid_tag(i), [S.straight_pairs([(float(c[0] + r * np.cos(a)), float(c[1] + r * np.sin(a)))
+                                                           for a in ang])]))
+        subs.append([float(v) for v in rng.uniform(0, 1, 4)])
+    sc = S.build_scene(512, 512, shapes, subs, name="crowded corner")
+    spec = G.default_spec()
+    m = A.segment_map(sc, spec)
+    assert len(np.unique(m)) >= 2, "the tree should mix segment heights"
+    ref = H.oracle_render(sc, spec, want_f64=True)
+    d = np.abs(fp64_image_ex(sc, m) - ref["f64"].reshape(sc.height, sc.width, 4)).max(axis=2)
+    assert np.median(d) < 1e-7
+    assert (d > 1e-5).mean() < 1e-3, f"{(d > 1e-5).sum()} pixels differ by more than 1e-5"
+
+
+def test_pictures_match_oracle():
+    """Picture substances sampled nearest (the reference's getPicturePixel)
+    and bilinear, at the pixel's integer position, as the oracle samples
+    them."""
+    import gudni_amd as G
+    sc = S.batch_scene(192, 150, 23, gradient_share=0, picture_share=4, texture_side=64)
+    kinds = np.asarray(sc.tags, np.uint64) & np.uint64(0xC000000000000000)
+    assert (kinds == np.uint64(S.TAG_PICTURE)).sum() >= 20
+    for filt in (0, 1):
+        ref = H.oracle_render(sc, G.default_spec(texture_filter=filt), want_f64=True)
+        d = np.abs(fp64_image_ex(sc, None, filt) - ref["f64"].reshape(sc.height, sc.width, 4)).max(axis=2)
+        assert np.median(d) < 1e-7, filt
+        assert (d > 1e-5).mean() < 2e-2 and d.max() < 2e-2, filt
+    # the two filters give different images
+    a, b = fp64_image_ex(sc, None, 0), fp64_image_ex(sc, None, 1)
+    assert np.abs(a - b).max() > 0.05
+
+
+def test_fd_threads_give_the_same_samples():
+    sc, target = A.case(96, 60, 21)
+    a = A.fd_samples(sc, target, n=10, seed=2, threads=1)
+    b = A.fd_samples(sc, target, n=10, seed=2, threads=8)
+    assert a == b and len(a) == 10
