@@ -235,6 +235,12 @@ class Rasterizer:
         self._check(self.L.gudni_frame_render(self.ctx, C.c_void_p(out_dev_ptr),
                                               C.c_void_p(out_f32_dev_ptr) if out_f32_dev_ptr else None))
 
+    def frame_render_static(self, out_dev_ptr: int, out_f32_dev_ptr: int | None = None):
+        """gudni_frame_render_static: the uploaded frame rendered with the
+        library's static scratch (no device allocation)."""
+        self._check(self.L.gudni_frame_render_static(self.ctx, C.c_void_p(out_dev_ptr),
+                                                     C.c_void_p(out_f32_dev_ptr) if out_f32_dev_ptr else None))
+
     def frame_check(self):
         self._check(self.L.gudni_frame_check(self.ctx))
 

@@ -95,6 +95,7 @@ SIGNATURES = {
     "gudni_frame_upload": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
                                      C.c_int32]),
     "gudni_frame_render": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gudni_frame_render_static": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "gudni_frame_check": (C.c_int, [C.c_void_p]),
     "gudni_enable_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "gudni_set_blocks_per_cu": (C.c_int, [C.c_void_p, C.c_int32]),

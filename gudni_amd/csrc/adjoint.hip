@@ -1373,6 +1373,7 @@ int launch(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t work_begin, int32_t n
     af.f.work = (const uint32_t *)a->work.p + work_begin;
     af.f.n_work = n_work;
     af.f.status = ctl + dev::kCtlStatus;
+    ctx->status_word = af.f.status;
     af.f.retry = ctl + dev::kCtlRetry;
     af.f.retry_capacity = ctx->retry_capacity;
     af.scenes = (const adj::AdjDesc *)a->descs.p;

@@ -66,6 +66,9 @@ struct gudni_ctx {
     // bumped whenever the launch scratch (rings, control words) or the fast
     // grid may have changed: a captured graph holds those pointers and sizes
     uint64_t scratch_gen = 0;
+    // status word of the last launch (the context's control words, or the
+    // library's static scratch after gudni_frame_render_static)
+    uint32_t *status_word = nullptr;
     // uploaded frame (Level 2)
     int32_t frame_w = 0, frame_h = 0, frame_n_work = 0, frame_n_tiles = 0, frame_index = 0;
     int32_t frame_sort_bits = 0;  // sort radix of the spec the uploaded jobs were built with

@@ -19,6 +19,8 @@
 //
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include <cstring>
 #include <string>
 #include <vector>
@@ -93,6 +95,12 @@ __global__ void reset_counters(uint32_t *counters, int n)
     if (i < n) counters[i] = 0u;
 }
 
+// Writes a scene descriptor passed by value (no host-to-device copy).
+__global__ void store_scene(SceneDesc *dst, SceneDesc sd)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *dst = sd;
+}
+
 // Start of a batch: clears the per-scene completion times and stamps the
 // start time in done[0].
 __global__ void batch_begin(unsigned long long *done, int n_scenes)
@@ -104,6 +112,73 @@ __global__ void batch_begin(unsigned long long *done, int n_scenes)
 
 } // namespace dev
 } // namespace gudni
+
+// ===========================================================================
+// The static scratch of gudni_frame_render_static
+// ===========================================================================
+// Callers that cannot allocate device memory (code under graph capture,
+// serving loops) render an uploaded frame with rings, shape scratch, control
+// words and scene descriptor carved from this one array, which every GPU the
+// library runs on holds until the process ends.  Layout (bytes):
+//   fast pass, 32 blocks = 128 wave slots: threshold rings of 128 records
+//     (16 MiB of float4, 4 MiB of headers) and shape scratch (8 MiB)
+//   retry pass, one wave slot: a ring of 4096 records (4 MiB + 1 MiB) and
+//     shape scratch (64 KiB)
+//   control words and a retry list of up to 2^17 work units (512 KiB)
+//   one scene descriptor
+namespace {
+constexpr int kStaticFastBlocks = 32;
+constexpr size_t kStaticFastSlots = (size_t)kStaticFastBlocks * 4;
+constexpr int kStaticSlowLog2 = 12;
+constexpr int kStaticMaxWork = 1 << 17;
+constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t kStaticFastRecs = (size_t)1 << gudni::dev::kFastCapLog2;
+constexpr size_t kStaticSlowRecs = (size_t)1 << kStaticSlowLog2;
+constexpr size_t kOffFastThr = 0;
+constexpr size_t kOffFastHdr = kOffFastThr + kStaticFastSlots * kStaticFastRecs * 64 * 16;
+constexpr size_t kOffFastSidx = kOffFastHdr + kStaticFastSlots * kStaticFastRecs * 64 * 4;
+constexpr size_t kOffSlowThr = kOffFastSidx + kStaticFastSlots * gudni::dev::kSidxWords * 64 * 4;
+constexpr size_t kOffSlowHdr = kOffSlowThr + kStaticSlowRecs * 64 * 16;
+constexpr size_t kOffSlowSidx = kOffSlowHdr + kStaticSlowRecs * 64 * 4;
+constexpr size_t kOffCtl = kOffSlowSidx + (size_t)gudni::dev::kSidxWords * 64 * 4;
+constexpr size_t kOffScene = align256(kOffCtl + 4 * ((size_t)gudni::dev::kCtlWords + kStaticMaxWork));
+constexpr size_t kStaticBytes = align256(kOffScene + sizeof(gudni::dev::SceneDesc));
+static_assert(kStaticBytes <= ((size_t)34 << 20), "the static scratch is stated as at most 34 MiB");
+constexpr int kStaticMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) uint8_t scratch[kStaticBytes];
+
+namespace {
+// Device address of `scratch` on each device (each GPU holds its own copy),
+// looked up once per device by gudni_create with that device current.
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kStaticMaxDevices];
+bool g_scratch_looked_up[kStaticMaxDevices];
+hipError_t g_scratch_err[kStaticMaxDevices];
+
+void lookup_static_scratch(int device)
+{
+    if (device < 0 || device >= kStaticMaxDevices) return;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_looked_up[device]) return;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_err[device] = e;
+    g_scratch_ptr[device] = e == hipSuccess ? p : nullptr;
+    g_scratch_looked_up[device] = true;
+}
+
+uint8_t *static_scratch(int device, hipError_t &err)
+{
+    err = hipErrorInvalidDevice;
+    if (device < 0 || device >= kStaticMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_looked_up[device]) return nullptr;
+    err = g_scratch_err[device];
+    return (uint8_t *)g_scratch_ptr[device];
+}
+}  // namespace
 
 // ===========================================================================
 // Host side of the device API
@@ -279,22 +354,47 @@ static dev::SceneDesc single_desc(gudni_ctx *ctx, const float bg[4], int32_t w, 
 
 // Both passes, stream-ordered.  The retry pass runs unconditionally; with no
 // abandoned units its waves exit on their first claim.
-static int launch_passes(gudni_ctx *ctx, dev::Frame f)
+// Where the two passes keep their control words, rings and shape scratch,
+// and their launch shapes.
+struct PassBuffers {
+    uint32_t *ctl;               // counters, status word, retry list
+    int32_t retry_capacity;
+    float4 *fast_thr;
+    uint32_t *fast_hdr;
+    int32_t *fast_sidx;
+    int fast_blocks;             // blocks of 256 threads
+    float4 *slow_thr;
+    uint32_t *slow_hdr;
+    int32_t *slow_sidx;
+    int slow_blocks, slow_threads;
+};
+
+static PassBuffers ctx_buffers(gudni_ctx *ctx)
 {
-    uint32_t *ctl = (uint32_t *)ctx->control.p;
+    return PassBuffers{(uint32_t *)ctx->control.p, ctx->retry_capacity,
+                       (float4 *)ctx->fast_thr.p, (uint32_t *)ctx->fast_hdr.p, (int32_t *)ctx->fast_sidx.p,
+                       ctx->fast_blocks,
+                       (float4 *)ctx->slow_thr.p, (uint32_t *)ctx->slow_hdr.p, (int32_t *)ctx->slow_sidx.p,
+                       ctx->slow_blocks, 256};
+}
+
+static int launch_passes(gudni_ctx *ctx, dev::Frame f, const PassBuffers &b)
+{
+    uint32_t *ctl = b.ctl;
     hipLaunchKernelGGL(dev::reset_counters, dim3(1), dim3(64), 0, ctx->stream, ctl, dev::kCtlWords);
     CHECK_HIP(ctx, hipGetLastError());
     f.status = ctl + dev::kCtlStatus;
+    ctx->status_word = f.status;
     f.retry = ctl + dev::kCtlRetry;
-    f.retry_capacity = ctx->retry_capacity;
+    f.retry_capacity = b.retry_capacity;
     // fast pass: small rings, as many waves as fit
     f.counter = ctl + dev::kCtlFast;
     f.n_parts = dev::kParts;
-    f.q_thr = (float4 *)ctx->fast_thr.p;
-    f.q_hdr = (uint32_t *)ctx->fast_hdr.p;
-    f.shape_idx = (int32_t *)ctx->fast_sidx.p;
+    f.q_thr = b.fast_thr;
+    f.q_hdr = b.fast_hdr;
+    f.shape_idx = b.fast_sidx;
     f.cap_log2 = dev::kFastCapLog2;
-    const int fast_grid = std::max(1, std::min(ctx->fast_blocks, (f.n_work + 3) / 4));
+    const int fast_grid = std::max(1, std::min(b.fast_blocks, (f.n_work + 3) / 4));
     hipLaunchKernelGGL(dev::raster_pass, dim3(fast_grid), dim3(256), 0, ctx->stream, f, 0);
     CHECK_HIP(ctx, hipGetLastError());
     if (ctx->timed_render) CHECK_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
@@ -303,20 +403,22 @@ static int launch_passes(gudni_ctx *ctx, dev::Frame f)
     while ((1 << slog) < ctx->spec.max_thresholds) ++slog;
     f.counter = ctl + dev::kCtlSlow;
     f.n_parts = 1;
-    f.q_thr = (float4 *)ctx->slow_thr.p;
-    f.q_hdr = (uint32_t *)ctx->slow_hdr.p;
-    f.shape_idx = (int32_t *)ctx->slow_sidx.p;
+    f.q_thr = b.slow_thr;
+    f.q_hdr = b.slow_hdr;
+    f.shape_idx = b.slow_sidx;
     f.cap_log2 = slog;
-    hipLaunchKernelGGL(dev::raster_pass, dim3(ctx->slow_blocks), dim3(256), 0, ctx->stream, f, 1);
+    hipLaunchKernelGGL(dev::raster_pass, dim3(b.slow_blocks), dim3(b.slow_threads), 0, ctx->stream, f, 1);
     CHECK_HIP(ctx, hipGetLastError());
     return GUDNI_OK;
 }
 
+static int launch_passes(gudni_ctx *ctx, dev::Frame f) { return launch_passes(ctx, f, ctx_buffers(ctx)); }
+
 int read_status(gudni_ctx *ctx)
 {
     uint32_t st = 0;
-    CHECK_HIP(ctx, hipMemcpyAsync(&st, (uint32_t *)ctx->control.p + dev::kCtlStatus, 4, hipMemcpyDeviceToHost,
-                                  ctx->stream));
+    const uint32_t *word = ctx->status_word ? ctx->status_word : (uint32_t *)ctx->control.p + dev::kCtlStatus;
+    CHECK_HIP(ctx, hipMemcpyAsync(&st, word, 4, hipMemcpyDeviceToHost, ctx->stream));
     CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (st & dev::kStatusOverflow)
         return set_error(ctx, GUDNI_ERR_OVERFLOW, "a column's threshold queue exceeded maxThresholds");
@@ -670,6 +772,9 @@ int gudni_create(const gudni_spec *spec, int device, gudni_ctx **out)
         return GUDNI_ERR_HIP;
     }
     ctx->cus = prop.multiProcessorCount;
+    // the static scratch's address on this device (for
+    // gudni_frame_render_static; a failure surfaces only there)
+    lookup_static_scratch(device);
     *out = ctx;
     return GUDNI_OK;
 }
@@ -965,6 +1070,50 @@ int gudni_frame_render(gudni_ctx *ctx, uint32_t *out_dev, float *out_f32_dev)
     if (rc) return rc;
     if (ctx->timing) CHECK_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     return GUDNI_OK;
+}
+
+int gudni_frame_render_static(gudni_ctx *ctx, uint32_t *out_dev, float *out_f32_dev)
+{
+    if (!ctx || !out_dev) return GUDNI_ERR_ARG;
+    if (!ctx->frame_ready) return set_error(ctx, GUDNI_ERR_ARG, "gudni_frame_upload was not called");
+    if (ctx->frame_n_work > kStaticMaxWork)
+        return set_error(ctx, GUDNI_ERR_ARG, "gudni_frame_render_static: the frame has " +
+                                                 std::to_string(ctx->frame_n_work) +
+                                                 " work units; the static scratch holds at most " +
+                                                 std::to_string(kStaticMaxWork));
+    if (ctx->spec.max_thresholds > (1 << kStaticSlowLog2))
+        return set_error(ctx, GUDNI_ERR_ARG, "gudni_frame_render_static: max_thresholds above " +
+                                                 std::to_string(1 << kStaticSlowLog2) +
+                                                 " (the static retry ring's capacity)");
+    hipError_t err;
+    uint8_t *base = static_scratch(ctx->device, err);
+    if (!base)
+        return set_error(ctx, GUDNI_ERR_HIP, std::string("gudni_frame_render_static: no address for the "
+                                                         "static scratch on this device: ") +
+                                                 hipGetErrorString(err));
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    dev::Frame f = base_frame(ctx);
+    f.tiles = (const TileRec *)ctx->tiles.p;
+    f.tile_job = (const int32_t *)ctx->tile_job.p;
+    f.work = (const uint32_t *)ctx->work.p;
+    f.n_work = ctx->frame_n_work;
+    dev::SceneDesc sd = single_desc(ctx, ctx->background, ctx->frame_w, ctx->frame_h, out_dev, out_f32_dev);
+    sd.sort_bits = ctx->frame_sort_bits;
+    dev::SceneDesc *sdev = reinterpret_cast<dev::SceneDesc *>(base + kOffScene);
+    hipLaunchKernelGGL(dev::store_scene, dim3(1), dim3(64), 0, ctx->stream, sdev, sd);
+    CHECK_HIP(ctx, hipGetLastError());
+    f.scenes = sdev;
+    f.tile_scene = nullptr;
+    f.scene_done = nullptr;
+    ctx->timed_render = false;   // timing would need events and a cycle buffer
+    const PassBuffers b{reinterpret_cast<uint32_t *>(base + kOffCtl), kStaticMaxWork,
+                        reinterpret_cast<float4 *>(base + kOffFastThr),
+                        reinterpret_cast<uint32_t *>(base + kOffFastHdr),
+                        reinterpret_cast<int32_t *>(base + kOffFastSidx), kStaticFastBlocks,
+                        reinterpret_cast<float4 *>(base + kOffSlowThr),
+                        reinterpret_cast<uint32_t *>(base + kOffSlowHdr),
+                        reinterpret_cast<int32_t *>(base + kOffSlowSidx), 1, 64};
+    return launch_passes(ctx, f, b);
 }
 
 int gudni_frame_check(gudni_ctx *ctx)
@@ -1569,6 +1718,8 @@ int gudni_batch_render(gudni_ctx *ctx, gudni_batch *b, int use_graph)
         b->captured_gen = ctx->scratch_gen;
         b->captured_format = ctx->out_format;
     }
+    // the replay writes the status word of the context's control words
+    ctx->status_word = (uint32_t *)ctx->control.p + dev::kCtlStatus;
     CHECK_HIP(ctx, hipGraphLaunch(b->exec, ctx->stream));
     return GUDNI_OK;
 }

@@ -229,6 +229,21 @@ int gudni_frame_upload(gudni_ctx *ctx, const gudni_jobs *jobs,
  * without waiting).  out_dev: width*height BGRA words; out_f32_dev (may be
  * NULL): width*height*4 floats, the pre-quantisation RGBA of each pixel. */
 int gudni_frame_render(gudni_ctx *ctx, uint32_t *out_dev, float *out_f32_dev);
+/* gudni_frame_render without any device allocation, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops): no
+ * counterpart in the reference.  The kernels take their threshold rings,
+ * shape scratch, control words and scene descriptor from one static device
+ * array of the library (at most 34 MiB, held by every GPU the library runs
+ * on until the process ends) instead of the context's buffers; the frame
+ * must have been uploaded (gudni_frame_upload / gudni_scene_bin, which do
+ * allocate).  Limits: at most 131072 work units and max_thresholds at most
+ * 4096 (GUDNI_ERR_ARG beyond them); the fast pass runs 32 blocks, so it is
+ * slower than gudni_frame_render on a large frame.  The results are those
+ * of gudni_frame_render.  Every call on one GPU uses that GPU's array, whose
+ * control words are cleared by each call: calls that overlap on one GPU
+ * (from any context or stream) race.  Timing is not recorded.  Errors of the
+ * kernels surface at gudni_frame_check. */
+int gudni_frame_render_static(gudni_ctx *ctx, uint32_t *out_dev, float *out_f32_dev);
 /* Waits for the stream and reports a queue overflow or guard hit seen by the
  * last render. */
 int gudni_frame_check(gudni_ctx *ctx);
