@@ -49,16 +49,39 @@ def algorithmic_bytes(scene, geo, jobs):
             + 4 * scene.width * scene.height)
 
 
+def cpu_info():
+    """CPU model, logical CPUs of the machine and the CPUs this process may use
+    (its affinity, capped by OMP_NUM_THREADS where the host sets one: the
+    share of the host that goes with one GPU)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = usable
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        share = max(1, min(usable, int(os.environ["OMP_NUM_THREADS"])))
+    return {"model": model, "machine_logical_cpus": os.cpu_count(), "affinity_cpus": usable,
+            "threads_used": share}
+
+
 def cpu_baseline(scene, seconds_target=15.0, texture_filter=1):
-    """The oracle (CPU restatement, all host threads) on a bounded sample of
-    the same frame: whole raster jobs, taken in order until ~seconds_target."""
+    """The oracle (CPU restatement) on the host CPUs this process is given, on
+    a bounded sample of the same frame: whole raster jobs, taken in order
+    until ~seconds_target (the whole 4096^2 frame fits in that on 16 cores)."""
     from oracle import oracle as O
     spec = O.default_spec()
     spec.coverage_mode = 1   # the reference's own arithmetic, as its CPU path would run
     spec.texture_filter = texture_filter
     og = O.Geometry(scene)
     oj = O.JobList(spec, og, scene.width, scene.height)
-    threads = min(16, os.cpu_count() or 1)
+    info = cpu_info()
+    threads = info["threads_used"]
     rnd = O.random_field(spec.random_field_size)
     out = np.zeros(scene.width * scene.height, np.uint32)
     pixels = 0
@@ -72,14 +95,18 @@ def cpu_baseline(scene, seconds_target=15.0, texture_filter=1):
     dt = time.perf_counter() - t0
     return {"value": pixels / dt / 1e9, "unit": "Gpixels/s", "cores": threads, "kind": "port",
             "sample": f"{jobs_done} of {oj.n_jobs} raster jobs ({pixels} pixels) of the same "
-                      f"4096x4096 frame, oracle C restatement in the reference's arithmetic, "
-                      f"{dt:.1f} s"}
+                      f"4096x4096 frame in REFERENCE coverage mode, oracle C restatement in the "
+                      f"reference's arithmetic, {dt:.1f} s on {threads} threads",
+            "whole_frame": jobs_done == oj.n_jobs, "cpu": info}
 
 
 def measured_traffic(kernel_ms):
     """HBM bytes per fast-pass launch from the committed rocprofv3 counter
     summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the
-    counter reads for a plain copy of known size)."""
+    counter reads for a plain copy of known size), with the source
+    fingerprint and commit of the library that was profiled and whether it
+    is the library this run uses."""
+    from gudni_amd import build as pbuild
     path = os.path.join(ROOT, "profiles", "raster_pass_blobs_4096_traffic.json")
     if not os.path.exists(path):
         return None, None
@@ -90,7 +117,10 @@ def measured_traffic(kernel_ms):
         "fetch_bytes": t["fetch_kib"] * 1024 * t["fetch_factor"],
         "write_bytes": t["write_kib"] * 1024 * t["write_factor"],
         "profiled_kernel_ms": t["fast_pass_avg_ms"], "this_run_kernel_ms": kernel_ms,
-        "lds_bank_conflict_rate": t["lds_bank_conflict_rate"]}
+        "lds_bank_conflict_rate": t["lds_bank_conflict_rate"],
+        "profiled_commit": t.get("commit"), "profiled_fingerprint": t.get("fingerprint"),
+        "profiled_library_is_this_run": t.get("fingerprint") == pbuild.fingerprint(),
+        "valu_utilization": t.get("valu_utilization")}
 
 
 def device_binning_leg(r, scene, geo, stream, steps):
@@ -631,6 +661,38 @@ def main():
     t_read = time.perf_counter() - t
     del host_frame
 
+    # the same frame in REFERENCE coverage mode, and the pinned spec
+    # (256-pixel tiles, Setup.hs:81-83) in both modes
+    def spec_frame_ms(sp, n=5):
+        rr = G.Rasterizer(sp)
+        rr.set_stream(stream.cuda_stream)
+        rr.upload_common(geo.geo, scene)
+        rr.frame_upload(G.RasterJobs(sp, geo, scene.width, scene.height), scene)
+        rr.frame_render(out.data_ptr())
+        rr.frame_check()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(n):
+            rr.frame_render(out.data_ptr())
+        e1.record(stream)
+        rr.frame_check()
+        rr.close()
+        return e0.elapsed_time(e1) / n
+
+    log("reference coverage mode and the pinned spec")
+    other_frames = {
+        "tuned_spec_reference_mode_ms": spec_frame_ms(G.default_spec(
+            texture_filter=args.texture_filter, sort_radix_bits=radix, max_tile_size=tile, coverage_mode=1)),
+        "pinned_spec_accurate_ms": spec_frame_ms(G.default_spec(texture_filter=args.texture_filter)),
+        "pinned_spec_reference_mode_ms": spec_frame_ms(G.default_spec(texture_filter=args.texture_filter,
+                                                                      coverage_mode=1)),
+        "pinned_spec": "256-px tiles, 256 threads per tile, maxThresholds 4096, maxShapes 127, radix 0"}
+    for k in list(other_frames):
+        if k.endswith("_ms"):
+            other_frames[k.replace("_ms", "_gpixels_per_s")] = npix / (other_frames[k] * 1e-3) / 1e9
+    log(f"other frames: {json.dumps(other_frames)}")
+
     b_alg = algorithmic_bytes(scene, geo, jobs)
     achieved = b_alg / (fast_ms * 1e-3) / 1e9
     # the same figure with the pinned 256-pixel tiles' (smaller) byte count:
@@ -701,6 +763,8 @@ def main():
                      "frac_with_256_px_tile_bytes": b_alg_pinned / (fast_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "peak_measured_copy_gbs": copy_gbs, "frac_of_measured_copy": achieved / copy_gbs,
                      "stage_cycle_share": stage_share},
+        "other_frames": other_frames,
+        "bin_plus_render_ms": binning["bin_plus_render_ms"],
         "level1_host_buffers": level1,
         "device_binning": binning,
         "device_pipeline": pipeline,
