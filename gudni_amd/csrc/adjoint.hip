@@ -420,13 +420,46 @@ __device__ __forceinline__ float4 label_color(const SceneDesc &s, const int32_t 
     return read_color(s, tag, ax, ay);
 }
 
+// determineColor's merge of consecutive shapes with one substance id
+// (Kernels.cl:1447-1513), as a walk top-down: visit(tag) says whether the
+// layer is composited.  A new id composites if its shape adds (or
+// continues); a repeated id never composites again, and only the deprecated
+// continue compound toggles whether the merged run is set.
+struct MergeWalk {
+    uint64_t last_id = ~0ull;
+    bool last_is_continue = true, last_is_set = false;
+    __device__ __forceinline__ bool visit(uint64_t tag)
+    {
+        const uint64_t sid = tag & kTagIdMask;
+        const bool is_cont = (tag & kTagCompoundMask) == kTagContinue;
+        const bool is_add = (tag & kTagCompoundMask) == kTagAdd;
+        if (sid == last_id) {
+            if (last_is_continue) {
+                if (!is_cont) { last_is_set = is_add; last_is_continue = false; }
+                else last_is_set = !last_is_set;
+            }
+            return false;
+        }
+        last_id = sid;
+        last_is_set = is_add || is_cont;
+        return last_is_set;
+    }
+};
+
+__device__ __forceinline__ uint64_t label_tag(const SceneDesc &s, const int32_t *sidx, const TileTable *tt,
+                                              bool compact, int b)
+{
+    return compact ? tt->tag[b] : s.shapes[sidx[b * kWave]].tag;
+}
+
 // Colour gradient of one section: gc = dL/dC for C = (P/A, A), S = (P, A)
 // the section's premultiplied composite.  Walks the stack top-down as the
 // forward does (stopping where its alpha reached 1).  For layer i with
 // transmittance T and straight colour (c, a): dS/dc = T a, dS/da = T (c, 1) -
 // V with V = T * (composite below i) = (S - F_{i+1}) / (1 - a), F_{i+1} the
 // composite down to and including i; an opaque layer takes V from a walk of
-// the layers below it.
+// the layers below it.  Layers the merge walk does not composite (a repeated
+// substance id, a subtract compound) add no term.
 template <bool kCompact>
 __device__ void color_grad(const SceneDesc &s, const Stack &st, const int32_t *sidx, const TileTable *tt,
                            const float4 *straight, float4 *dcol, float *d_rgba, int ax, int ay, float4 S,
@@ -437,11 +470,13 @@ __device__ void color_grad(const SceneDesc &s, const Stack &st, const int32_t *s
     const float gA = gc.w - (gc.x * C.x + gc.y * C.y + gc.z * C.z) / S.w;
     uint64_t w0 = st.s0, w1 = st.s1;
     float4 F = make_float4(0.f, 0.f, 0.f, 0.f);
+    MergeWalk mw;
     for (;;) {
         int b;
         if (w1) { b = 127 - __clzll((long long)w1); w1 &= ~(1ull << (b - 64)); }
         else if (w0) { b = 63 - __clzll((long long)w0); w0 &= ~(1ull << b); }
         else break;
+        if (!mw.visit(label_tag(s, sidx, tt, kCompact, b))) continue;
         uint64_t tag;
         const float4 col = label_color<kCompact>(s, sidx, tt, straight, b, ax, ay, tag);
         const float T = 1.0f - F.w;
@@ -457,14 +492,17 @@ __device__ void color_grad(const SceneDesc &s, const Stack &st, const int32_t *s
                 const float inv = 1.0f / (1.0f - col.w);
                 V = make_float4((S.x - Fn.x) * inv, (S.y - Fn.y) * inv, (S.z - Fn.z) * inv, (S.w - Fn.w) * inv);
             } else {
-                // composite of the layers below, scaled by T
+                // composite of the layers below, scaled by T (the merge
+                // walk goes on from this layer)
                 uint64_t u0 = w0, u1 = w1;
                 float4 U = make_float4(0.f, 0.f, 0.f, 0.f);
                 bool reached_one = false;
+                MergeWalk mu = mw;
                 while (u0 | u1) {
                     int bb;
                     if (u1) { bb = 127 - __clzll((long long)u1); u1 &= ~(1ull << (bb - 64)); }
                     else { bb = 63 - __clzll((long long)u0); u0 &= ~(1ull << bb); }
+                    if (!mu.visit(label_tag(s, sidx, tt, kCompact, bb))) continue;
                     uint64_t tag2;
                     const float4 c2 = label_color<kCompact>(s, sidx, tt, straight, bb, ax, ay, tag2);
                     const float t2 = 1.0f - U.w;
@@ -733,7 +771,7 @@ __device__ void adj_unit(const Frame &f, const AdjDesc &d, int lane, uint32_t un
     const int tile = (int)(unit >> 2), quarter = (int)(unit & 3u);
     const TileRec t = f.tiles[tile];
     const int job = f.tile_job ? f.tile_job[tile] : 0;
-    const int column = quarter * kWave + lane;
+    const int column = unit_column(t.hdepth, f.compute_depth, quarter, lane);
     Column c;
     const bool active = init_column(f, s, t, job, column, c);
     if (kCompact) {
@@ -749,10 +787,8 @@ __device__ void adj_unit(const Frame &f, const AdjDesc &d, int lane, uint32_t un
         }
         wave_sync();
     }
-    const int hd = t.hdepth;
-    const int w = 1 << hd;
-    const int x0 = t.box[0] + (hd >= 6 ? ((quarter * kWave) & (w - 1)) : 0);
-    const int x1 = x0 + (hd >= 6 ? kWave : w);
+    const int x0 = t.box[0] + unit_x0(t.hdepth, f.compute_depth, quarter);
+    const int x1 = x0 + unit_width(t.hdepth, f.compute_depth);
     const float span0 = (float)x0 - 1.0f, span1 = (float)x1 + 1.0f;
     // optional phase clocks (wave-uniform points): [0] A generate [1] A sort
     // [2] A sweep [3] B generate [4] B sort [5] B sweep [6] C [7] units
@@ -957,21 +993,12 @@ namespace {
 // continue compounds) and has a substance of its own.
 int check_scene(gudni_ctx *ctx, const gudni_adjoint_batch::Scene &sc, int s)
 {
-    std::vector<uint64_t> ids;
     for (uint64_t tag : sc.tags) {
-        if ((tag & dev::kTagCompoundMask) != dev::kTagAdd)
-            return set_error(ctx, GUDNI_ERR_ARG, "adjoint scene " + std::to_string(s) +
-                                                     ": every shape must be an added shape");
         if (tag & dev::kTagNonzero)
             return set_error(ctx, GUDNI_ERR_ARG, "adjoint scene " + std::to_string(s) +
                                                      ": the adjoint fills by crossing parity only "
                                                      "(no GUDNI_TAG_NONZERO shapes)");
-        ids.push_back(tag & dev::kTagIdMask);
     }
-    std::sort(ids.begin(), ids.end());
-    if (std::adjacent_find(ids.begin(), ids.end()) != ids.end())
-        return set_error(ctx, GUDNI_ERR_ARG, "adjoint scene " + std::to_string(s) +
-                                                 ": shapes must not share a substance");
     return GUDNI_OK;
 }
 

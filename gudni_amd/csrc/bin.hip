@@ -81,13 +81,13 @@ struct PairFlags {
     }
 };
 
-// isActiveThread (Kernels.cl:1720-1722) over the columns of a tile quarter
+// isActiveThread (Kernels.cl:1720-1722) over the columns of work unit qd
 __device__ bool quarter_active(const TileRec &r, int qd, int cd, int width, int height)
 {
     const int hd = r.hdepth, vd = r.vdepth;
     const int dd = max(0, vd - (cd - hd));
     for (int l = 0; l < 64; ++l) {
-        const int col = qd * 64 + l;
+        const int col = unit_column(hd, cd, qd, l);
         const int ix = ((1 << hd) - 1) & col, iy = (col >> hd) << dd;
         if (iy < (r.box[3] - r.box[1]) && ix + r.box[0] < width && iy + r.box[1] < height)
             return true;

@@ -384,7 +384,7 @@ template <bool kCompact>
 __device__ void generate(const Frame &f, const SceneDesc &s, Queue &q, const Column &c, bool active, int lane,
                          float span0, float span1, int32_t *sidx,
                          uint32_t &shape_bits, uint64_t &stack0, uint64_t &stack1,
-                         uint64_t &contrib0, uint64_t &contrib1)
+                         uint64_t &contrib0, uint64_t &contrib1, bool &any_nonzero)
 {
     const float dx = (float)c.dx, dy = (float)c.dy;
     const uint32_t max_shapes = (uint32_t)f.max_shapes;
@@ -427,6 +427,7 @@ __device__ void generate(const Frame &f, const SceneDesc &s, Queue &q, const Col
         uint64_t offset = 16ull * __builtin_amdgcn_readfirstlane(srec[2]);
         const uint32_t num_strands = __builtin_amdgcn_readfirstlane(srec[3]);
         const bool nonzero = (__builtin_amdgcn_readfirstlane(srec[1]) & (uint32_t)(kTagNonzero >> 32)) != 0u;
+        any_nonzero = any_nonzero || nonzero;   // wave-uniform
         const uint32_t label = kCompact ? (uint32_t)n : shape_bits;
         bool added = false, enclosed_shape = false;
         int32_t w0 = 0;   // winding number of the segment's top-left corner (nonzero fill)
@@ -598,10 +599,11 @@ struct Stack {
     // change, sign(e.x m.y - e.y m.x) for an edge of original direction e
     // crossed by a move m, to its counter (wind[label * 64], this lane's) and
     // its bit becomes counter != 0.
+    // any: some record may carry kNonzero (wave-uniform; else this is flip).
     template <Cross kHow>
-    __device__ __forceinline__ void cross(uint32_t h, int32_t *wind)
+    __device__ __forceinline__ void cross(uint32_t h, int32_t *wind, bool any)
     {
-        if (!(h & kNonzero)) {
+        if (!any || !(h & kNonzero)) {
             flip(h);
             return;
         }
@@ -670,18 +672,19 @@ __device__ float split_next(Queue &q, Sweep &p)
     return slice;
 }
 
-// verticalAdvance (Kernels.cl:1744-1824).  wind: this lane's winding
-// counters (nonzero-fill shapes; may be null when no record has kNonzero).
-__device__ void vertical_advance(Queue &q, const Column &c, Sweep &p, Stack &st, int32_t *wind = nullptr)
+// verticalAdvance (Kernels.cl:1744-1824).  nz (wave-uniform): some record
+// may belong to a nonzero-fill shape; wind is this lane's winding counters.
+__device__ void vertical_advance(Queue &q, const Column &c, Sweep &p, Stack &st, int32_t *wind = nullptr,
+                                 bool nz = false)
 {
     if (p.sx1 != 1.0f) return;
-    for (int i = 0; i < p.num_active; ++i) st.cross<kCrossLeft>(q.get_h(i), wind);
+    for (int i = 0; i < p.num_active; ++i) st.cross<kCrossLeft>(q.get_h(i), wind, nz);
     const float next_break = cl_min(c.float_height, p.pixel_y);
     const float active_bottom = q.len > 0 ? q.get(0).bottom : FLT_MAX;
     if (active_bottom == p.sy1) {
         while (p.num_active > 0) {
             const uint32_t h = q.get_h(0);
-            if (hdr_persist_bottom(h)) st.cross<kCrossDown>(h, wind);
+            if (hdr_persist_bottom(h)) st.cross<kCrossDown>(h, wind, nz);
             q.pop();
             --p.num_active;
         }
@@ -700,14 +703,14 @@ __device__ void vertical_advance(Queue &q, const Column &c, Sweep &p, Stack &st,
                 const Thr t0 = q.get(0);
                 if (!(t0.top == t0.bottom)) break;
                 const uint32_t h = q.get_h(0);
-                if (hdr_persist_top(h)) st.cross<kCrossDown>(h, wind);
+                if (hdr_persist_top(h)) st.cross<kCrossDown>(h, wind, nz);
                 q.pop();
                 --p.num_active;
             }
             for (int i = 0; i < p.num_active; ++i) {
                 if (q.get(i).top > 0.0f) {
                     const uint32_t h = q.get_h(i);
-                    if (hdr_persist_top(h)) st.cross<kCrossDown>(h, wind);
+                    if (hdr_persist_top(h)) st.cross<kCrossDown>(h, wind, nz);
                 }
             }
         }
@@ -1021,7 +1024,7 @@ __device__ __forceinline__ uint32_t pixel_word(float4 c, bool rgba)
 // (Kernels.cl:1724-1742, 1826-1851, 1881-1916, 1978-2028)
 template <bool kCompact>
 __device__ uint32_t render(const Frame &f, const SceneDesc &s, Queue &q, const Column &c, Stack st,
-                           int32_t *sidx, TileTable *tt)
+                           int32_t *sidx, TileTable *tt, bool nz)
 {
     int32_t *const wind = sidx + kWindOffset * kWave;
     Sweep p;
@@ -1054,7 +1057,7 @@ __device__ uint32_t render(const Frame &f, const SceneDesc &s, Queue &q, const C
         }
         while ((p.sx1 < 1.0f) || (p.sy1 < p.pixel_y)) {
             if (--guard < 0) return kStatusGuard;
-            vertical_advance(q, c, p, st, wind);
+            vertical_advance(q, c, p, st, wind, nz);
             if (q.spilled || q.overflow) return 0;
             // horizontalAdvance with thresholdMidXLow (Kernels.cl:916-926)
             float nx;
@@ -1097,7 +1100,7 @@ __device__ uint32_t render(const Frame &f, const SceneDesc &s, Queue &q, const C
             acc.z += col.z * adj;
             acc.w += col.w * adj;
             area_sum += adj;
-            if (p.current < p.num_active) st.cross<kCrossRight>(q.get_h(p.current), wind);
+            if (p.current < p.num_active) st.cross<kCrossRight>(q.get_h(p.current), wind, nz);
             ++p.current;
         }
         const float4 color = make_float4(acc.x / area_sum, acc.y / area_sum, acc.z / area_sum,
@@ -1174,7 +1177,7 @@ __device__ void run_unit(const Frame &f, const SceneDesc &s, int lane, uint32_t 
     const int tile = (int)(unit >> 2), quarter = (int)(unit & 3u);
     const TileRec t = f.tiles[tile];
     const int job = f.tile_job ? f.tile_job[tile] : 0;
-    const int column = quarter * kWave + lane;
+    const int column = unit_column(t.hdepth, f.compute_depth, quarter, lane);
     Column c;
     const bool active = init_column(f, s, t, job, column, c);
     const int dslot = tile * f.threads_per_tile + column;
@@ -1224,14 +1227,13 @@ __device__ void run_unit(const Frame &f, const SceneDesc &s, int lane, uint32_t 
     }
     uint32_t bits = 0;
     uint64_t s0 = 0, s1 = 0, c0 = 0, c1 = 0;
+    bool any_nonzero = false;
     {
         // x extent of this unit's columns (isActiveThread's internalX)
-        const int hd = t.hdepth;
-        const int w = 1 << hd;
-        const int x0 = t.box[0] + (hd >= 6 ? ((quarter * kWave) & (w - 1)) : 0);
-        const int x1 = x0 + (hd >= 6 ? kWave : w);
+        const int x0 = t.box[0] + unit_x0(t.hdepth, f.compute_depth, quarter);
+        const int x1 = x0 + unit_width(t.hdepth, f.compute_depth);
         generate<kCompact>(f, s, q, c, active, lane, (float)x0 - 1.0f, (float)x1 + 1.0f, sidx, bits,
-                           s0, s1, c0, c1);
+                           s0, s1, c0, c1, any_nonzero);
     }
     const bool ok = active && !q.spilled && !q.overflow;
     const long long t1 = timed ? clock64() : 0;
@@ -1268,7 +1270,8 @@ __device__ void run_unit(const Frame &f, const SceneDesc &s, int lane, uint32_t 
         }
     }
     const long long t2 = timed ? clock64() : 0;
-    if (ok) status = render<kCompact>(f, s, q, c, Stack{s0, s1, 0u}, sidx, tt);
+    // only a wave that walked a nonzero-fill shape keeps winding counters
+    if (ok) status = render<kCompact>(f, s, q, c, Stack{s0, s1, 0u}, sidx, tt, any_nonzero);
     if (active && q.overflow) status |= kStatusOverflow;
     if (!active && f.dump_qlen) f.dump_qlen[dslot] = -1;
     if (timed) {

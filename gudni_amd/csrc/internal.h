@@ -39,6 +39,41 @@ static_assert(sizeof(PictUse) == 24, "PictureUse is 24 bytes");
 
 struct BBox { float l, t, r, b; };
 
+// Work units.  A tile of width 2^hd is run by 2^cd threads (cd = 8), one per
+// (column, row segment) as initTileState numbers them (Kernels.cl:1692-1722:
+// thread `col` takes x = col & (2^hd - 1) and segment col >> hd), in four
+// units of 64 (one wave each).  A unit is a block of 2^cw columns by 2^sw
+// segments, as square as the tile allows (8 x 8 where the tile has 8 or more
+// segments): the x span of a unit is the filter of the shapes its wave walks,
+// so a narrow block walks fewer.  Every column is computed the same whichever
+// unit and lane runs it.  Unit q is the (q mod n_x)-th block across and the
+// (q div n_x)-th down.
+struct UnitShape {
+    int cw_bits, sw_bits, n_x;
+};
+constexpr UnitShape unit_shape(int hd, int cd)
+{
+    int sw = cd - hd < 3 ? cd - hd : 3;
+    if (sw < 0) sw = 0;
+    int cw = 6 - sw;
+    if (cw > hd) {
+        cw = hd;
+        sw = 6 - hd;
+    }
+    return UnitShape{cw, sw, 1 << (hd - cw)};
+}
+// The thread (reference column index) lane `lane` of unit q runs.
+constexpr int unit_column(int hd, int cd, int q, int lane)
+{
+    const UnitShape u = unit_shape(hd, cd);
+    const int x = ((q % u.n_x) << u.cw_bits) | (lane & ((1 << u.cw_bits) - 1));
+    const int seg = ((q / u.n_x) << u.sw_bits) | (lane >> u.cw_bits);
+    return (seg << hd) | x;
+}
+// First column (tile-relative) and width of unit q's block.
+constexpr int unit_x0(int hd, int cd, int q) { return (q % unit_shape(hd, cd).n_x) << unit_shape(hd, cd).cw_bits; }
+constexpr int unit_width(int hd, int cd) { return 1 << unit_shape(hd, cd).cw_bits; }
+
 // Where a strand curve comes from, for gradients with respect to the outline
 // points: the outline curve's on-curve start, control and on-curve end (point
 // indices: pair q's on-curve point is 2q, its control 2q+1), the knob-split

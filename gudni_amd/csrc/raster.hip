@@ -4,9 +4,10 @@
 // Kernels.cl:1692-1722) and runs threshold generation, the per-column sort
 // and the scanline coverage/compositing sweep back to back, so records never
 // leave the wave's scratch between stages.  Work is claimed by whole
-// wavefronts: a work unit is (tile, quarter) = 64 consecutive columns of a
-// tile, and quarters without an active column are never queued (an 8x8 tile
-// is a single unit).  The grid is persistent; every wave loops on an atomic
+// wavefronts: a work unit (tile << 2 | q) is a block of 64 of the tile's
+// 256 column segments, 8 columns x 8 segments where the tile allows
+// (unit_column), and units without an active column are never queued (an
+// 8x8 tile is a single unit).  The grid is persistent; every wave loops on an atomic
 // work counter until the list is exhausted, so every wave reaches the exit.
 //
 // Per-lane threshold queues live in a column-interleaved scratch ring
@@ -187,7 +188,7 @@ using namespace gudni;
 
 namespace gudni {
 
-// Which quarters of a tile have an active column (isActiveThread,
+// Which work units of a tile have an active column (isActiveThread,
 // Kernels.cl:1720-1722), evaluated for every column on the host.
 void build_work(const gudni_spec &spec, const TileRec *tiles, int n_tiles, int width,
                        int height, std::vector<uint32_t> &work)
@@ -201,7 +202,7 @@ void build_work(const gudni_spec &spec, const TileRec *tiles, int n_tiles, int w
         for (int qd = 0; qd < quarters; ++qd) {
             bool any = false;
             for (int l = 0; l < 64 && !any; ++l) {
-                const int col = qd * 64 + l;
+                const int col = unit_column(hd, cd, qd, l);
                 const int ix = ((1 << hd) - 1) & col, iy = (col >> hd) << dd;
                 any = iy < (r.box[3] - r.box[1]) && ix + r.box[0] < width && iy + r.box[1] < height;
             }

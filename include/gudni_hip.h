@@ -320,9 +320,12 @@ int   gudni_raster_scene(gudni_ctx *ctx, const gudni_scene *scene, uint32_t *hos
  * RGBA) for solid substances (zero for pictures).  Discrete decisions (knob
  * splits, strand runs, tiles, curve subdivision, record drops, section
  * structure) are held; the knob-split parameter follows the analytic vertical
- * tangent.  Requirements: every shape is an added shape with a substance of
- * its own.  One persistent launch runs the forward and, recomputing each
- * tile's records, the backward, for all scenes of the handle. */
+ * tangent.  Shapes may share substances and be subtract compounds: the
+ * colour walk merges them as determineColor does (Kernels.cl:1447-1513) and
+ * a substance's gradient sums over every layer it colours.  Nonzero-fill
+ * shapes (GUDNI_TAG_NONZERO) are refused.  One persistent launch runs the
+ * forward and, recomputing each tile's records, the backward, for all
+ * scenes of the handle. */
 typedef struct gudni_adjoint_scene {
     gudni_outlines outlines;            /* the geometry parameters           */
     const float *rgba;  int32_t n_substances;   /* 16 B substance records    */

@@ -48,6 +48,9 @@
 #define D_QCAP (1 << 16)
 #define D_TAG_ID 0x07FFFFFFFFFFFFFFull
 #define D_TAG_TYPE 0xC000000000000000ull
+#define D_TAG_COMPOUND 0x3000000000000000ull
+#define D_TAG_ADD 0x2000000000000000ull
+#define D_TAG_CONTINUE 0x1000000000000000ull
 #define D_TAG_PICTURE 0x4000000000000000ull
 #define D_POS 0x80000000u
 #define D_PERSIST 0x40000000u
@@ -447,16 +450,35 @@ static void section_color(const dcol *c, int n_shapes, const dshape *shapes, con
 {
     double P[3] = { 0, 0, 0 }, A = 0.0;
     int done = 0;
+    /* determineColor's merge of consecutive shapes with one substance id
+     * (Kernels.cl:1447-1513): a new id composites if its shape adds (or
+     * continues); a repeated id is never composited again, and only the
+     * deprecated continue compound toggles whether the merged run is set */
+    uint64_t last_id = ~0ull;
+    int last_is_continue = 1, last_is_set = 0;
     for (int w = 0; w < c->nwords && !done; w++) {
         uint64_t bits = c->stack[w];
         while (bits) {
             int b = __builtin_ctzll(bits); bits &= bits - 1;
             int s = 64 * w + b;
             if (s >= n_shapes) break;
-            const double *col = rgba + 4 * shapes[s].sid;
+            const uint64_t tag = shapes[s].tag, sid = shapes[s].sid;
+            const int is_cont = (tag & D_TAG_COMPOUND) == D_TAG_CONTINUE;
+            const int is_add = (tag & D_TAG_COMPOUND) == D_TAG_ADD;
+            if (sid == last_id) {
+                if (last_is_continue) {
+                    if (!is_cont) { last_is_set = is_add; last_is_continue = 0; }
+                    else last_is_set = !last_is_set;
+                }
+                continue;
+            }
+            last_id = sid;
+            last_is_set = is_add || is_cont;
+            if (!last_is_set) continue;
+            const double *col = rgba + 4 * sid;
             double pcol[4];
-            if (pc && (shapes[s].tag & D_TAG_TYPE) == D_TAG_PICTURE) {
-                picture_color(pc, shapes[s].sid, ax, ay, pcol);
+            if (pc && (tag & D_TAG_TYPE) == D_TAG_PICTURE) {
+                picture_color(pc, sid, ax, ay, pcol);
                 col = pcol;
             }
             double t = 1.0 - A;
@@ -833,10 +855,31 @@ static void section_grad(const dcol *c, int n_shapes, const dshape *shapes, cons
     double gP[3] = { gc[0] / S[3], gc[1] / S[3], gc[2] / S[3] };
     double gA = gc[3] - (gc[0] * C[0] + gc[1] * C[1] + gc[2] * C[2]) / S[3];
     double F[4] = { 0, 0, 0, 0 };
+    /* the composited layers, front first: determineColor's merge walk
+     * (Kernels.cl:1447-1513) as in section_color */
     int layers[4096], nl = 0;
+    uint64_t last_id = ~0ull;
+    int last_is_continue = 1, last_is_set = 0;
     for (int w = 0; w < c->nwords; w++) {
         uint64_t bits = c->stack[w];
-        while (bits && nl < 4096) { int b = __builtin_ctzll(bits); bits &= bits - 1; if (64 * w + b < n_shapes) layers[nl++] = 64 * w + b; }
+        while (bits && nl < 4096) {
+            int b = __builtin_ctzll(bits); bits &= bits - 1;
+            const int sh = 64 * w + b;
+            if (sh >= n_shapes) continue;
+            const uint64_t tag = shapes[sh].tag, sid = shapes[sh].sid;
+            const int is_cont = (tag & D_TAG_COMPOUND) == D_TAG_CONTINUE;
+            const int is_add = (tag & D_TAG_COMPOUND) == D_TAG_ADD;
+            if (sid == last_id) {
+                if (last_is_continue) {
+                    if (!is_cont) { last_is_set = is_add; last_is_continue = 0; }
+                    else last_is_set = !last_is_set;
+                }
+                continue;
+            }
+            last_id = sid;
+            last_is_set = is_add || is_cont;
+            if (last_is_set) layers[nl++] = sh;
+        }
     }
     for (int i = 0; i < nl; i++) {
         const double *col = rgba + 4 * shapes[layers[i]].sid;
@@ -881,6 +924,7 @@ int go_diff_grad(int32_t n_shapes, const uint64_t *tags, const int32_t *soff, co
     for (int i = 0; i < n_shapes; i++) {
         build_shape(&shapes[i], pairs, ooff, soff[i], soff[i + 1], W, H, &sig);
         shapes[i].sid = tags[i] & D_TAG_ID;
+        shapes[i].tag = tags[i];
     }
     memset(dpairs, 0, sizeof(double) * 4 * (size_t)n_pairs);
     memset(drgba, 0, sizeof(double) * 4 * (size_t)n_subs);

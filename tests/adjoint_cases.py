@@ -153,7 +153,12 @@ def fd_samples(scene, target, n=64, seed=7, h_point=1e-2, h_color=1e-3, max_trie
             kind, p, comp, h = "rgba", int(sid[s]), int(rng.integers(4)), h_color
             if not solid[s]:
                 continue   # a picture's colour is its texels, not a parameter
-        win = shape_window(scene, s)
+        if kind == "rgba":
+            # a substance colours every shape that uses it
+            ws = [shape_window(scene, t) for t in np.flatnonzero(sid == p)]
+            win = (min(w[0] for w in ws), min(w[1] for w in ws), max(w[2] for w in ws), max(w[3] for w in ws))
+        else:
+            win = shape_window(scene, s)
         if win[2] <= win[0] or win[3] <= win[1]:
             continue
         cands.append((kind, p, comp, h, win))

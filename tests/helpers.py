@@ -70,6 +70,20 @@ TILE_DT = np.dtype([("box", "<i4", 4), ("hd", "<i2"), ("vd", "<i2"), ("col", "<i
                     ("start", "<i4"), ("count", "<i4")])
 
 
+def unit_columns(hd, cd, q, lanes):
+    """Threads (reference column indices) of work unit q of a tile of width
+    2^hd: a block of 2^cw columns by 2^sw row segments, 8 x 8 where the tile
+    has 8 or more segments, blocks numbered across then down."""
+    sw = max(0, min(cd - hd, 3))
+    cw = 6 - sw
+    if cw > hd:
+        cw, sw = hd, 6 - hd
+    n_x = 1 << (hd - cw)
+    x = ((q % n_x) << cw) | (lanes & ((1 << cw) - 1))
+    seg = ((q // n_x) << sw) | (lanes >> cw)
+    return (seg << hd) | x
+
+
 def host_frame_layout(spec, jobs, width, height):
     """The host build laid out as a whole frame, the way gudni_frame_upload
     does it: shapeStart absolute into the concatenated references, reversed job
@@ -87,7 +101,7 @@ def host_frame_layout(spec, jobs, width, height):
         hd, vd = int(r["hd"]), int(r["vd"])
         dd = max(0, vd - (cd - hd))
         for q in range(spec.threads_per_tile // 64):
-            col = q * 64 + lanes
+            col = unit_columns(hd, cd, q, lanes)
             ix = ((1 << hd) - 1) & col
             iy = (col >> hd) << dd
             box = r["box"]

@@ -153,3 +153,19 @@ def test_build_fingerprint_tracks_sources(tmp_path, monkeypatch):
     monkeypatch.setattr(pbuild, "SOURCES", pbuild.SOURCES[:-1])
     monkeypatch.setattr(pbuild, "ARCH", "gfx942")
     assert pbuild.fingerprint() != fp and pbuild.stale()
+
+
+def test_work_units_cover_every_thread_once():
+    """The four work units of a tile run each of its 256 threads (reference
+    column indices) exactly once, in blocks as square as the tile allows."""
+    from tests.helpers import unit_columns
+    lanes = np.arange(64)
+    for hd in range(2, 9):
+        cols = np.concatenate([unit_columns(hd, 8, q, lanes) for q in range(4)])
+        assert np.array_equal(np.sort(cols), np.arange(256)), hd
+        for q in range(4):
+            c = unit_columns(hd, 8, q, lanes)
+            nx, ns = len(np.unique(c & ((1 << hd) - 1))), len(np.unique(c >> hd))
+            assert nx * ns == 64
+            if 3 <= hd <= 5:
+                assert nx == 8 and ns == 8, (hd, nx, ns)

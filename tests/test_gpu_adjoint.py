@@ -250,3 +250,46 @@ def test_whole_batch_update_regrows_binning():
             fresh.close()
     finally:
         r.close()
+
+
+def compound_case(size, n, seed, n_subs):
+    """A blob scene whose shapes share n_subs substances and a quarter of
+    which are subtract compounds (cSubtract, Figure/ShapeTree.hs:84-166):
+    (perturbed scene, target)."""
+    import copy
+    from gudni_amd import scenes as S
+    base = S.batch_scene(size, n, seed, gradient_share=0)
+    base = copy.deepcopy(base)
+    tags = []
+    for i in range(base.n_shapes):
+        tags.append(S.solid_tag(i % n_subs, subtract=(i % 4 == 1)))
+    base.tags = np.array(tags, np.uint64)
+    base.substances = np.ascontiguousarray(np.asarray(base.substances, np.float32)[:n_subs])
+    ref = H.oracle_render(base, want_f64=False)
+    target = ref["f32"].reshape(base.height, base.width, 4).astype(np.float32)
+    return A.perturbed(base, seed + 1000), target
+
+
+@pytest.mark.parametrize("tile", [256, 32])
+def test_adjoint_compounds_and_shared_substances(tile):
+    """determineColor's merge of shapes sharing a substance and its subtract
+    compounds (Kernels.cl:1447-1513) in the adjoint: forward image and loss
+    equal the oracle's, gradients match FP64 central differences."""
+    sc, target = compound_case(256, 150, 43, 40)
+    spec = G.default_spec(max_tile_size=tile)
+    r = G.Rasterizer(spec)
+    try:
+        say(f"adjoint with compounds and shared substances, tile {tile}")
+        loss, dp, dc, img = gpu_adjoint(r, sc, target)
+    finally:
+        r.close()
+    ref = H.oracle_render(sc, spec, want_f64=True)
+    err = np.abs(img.astype(np.float64) - ref["f64"].reshape(img.shape)).max()
+    assert err <= COVERAGE_TOL, f"forward image differs from the oracle by {err}"
+    ref_loss = float(((ref["f64"].reshape(img.shape) - target) ** 2).sum())
+    assert abs(loss - ref_loss) <= 1e-4 * ref_loss
+    samples = A.fd_samples(sc, target, n=64, seed=9, seg=A.segment_map(sc, spec), max_tries=1200)
+    assert len(samples) == 64
+    got, fd, e, bad = A.compare(samples, dp, dc)
+    say(f"max |adjoint - fd| {e.max():.3e}")
+    assert not bad.any(), [(s, g) for s, g, b in zip(samples, got, bad) if b][:5]

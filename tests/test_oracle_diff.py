@@ -251,3 +251,53 @@ def test_fd_threads_give_the_same_samples():
     a = A.fd_samples(sc, target, n=10, seed=2, threads=1)
     b = A.fd_samples(sc, target, n=10, seed=2, threads=8)
     assert a == b and len(a) == 10
+
+
+def compound_mix_scene(seed=4, size=128, n=60):
+    """Shapes sharing substances, subtract compounds (cSubtract, Figure/
+    ShapeTree.hs:84-166) and solid colours, on straight edges."""
+    rng = np.random.default_rng(seed)
+    shapes, subs = [], []
+    for i in range(n):
+        k = int(rng.integers(3, 8))
+        c = rng.uniform(0, size, 2)
+        r = rng.uniform(6, 30)
+        ang = np.sort(rng.uniform(0, 2 * np.pi, k))
+        pts = [(float(c[0] + r * np.cos(a)), float(c[1] + r * np.sin(a))) for a in ang]
+        sid = int(rng.integers(0, n // 3))          # about three shapes per substance
+        subtract = rng.uniform() < 0.3
+        shapes.append((S.solid_tag(sid, subtract=subtract), [S.straight_pairs(pts)]))
+    for _ in range(n // 3):
+        subs.append([float(v) for v in rng.uniform(0, 1, 4)])
+    return S.build_scene(size, size, shapes, subs, name="compound mix")
+
+
+def test_fp64_forward_merges_compounds_like_the_oracle():
+    """determineColor's merge of shapes with one substance id and its
+    subtract compounds (Kernels.cl:1447-1513) in the FP64 forward gives the
+    oracle's pixels."""
+    from tests.test_gpu_parity import compound_scene
+    for sc in (compound_scene(), compound_mix_scene(), compound_mix_scene(9, 96, 45)):
+        ref = H.oracle_render(sc, want_f64=True)
+        d = np.abs(fp64_image(sc) - ref["f64"].reshape(sc.height, sc.width, 4)).max(axis=2)
+        assert np.median(d) < 1e-7, sc.name
+        assert (d > 1e-5).mean() < 1e-3, (sc.name, (d > 1e-5).sum())
+
+
+def test_fp64_adjoint_with_compounds_matches_finite_differences():
+    """The FP64 adjoint with determineColor's merge walk (shapes sharing a
+    substance, subtract compounds) against central differences; a colour
+    sample's window covers every shape of its substance."""
+    import copy
+    base = copy.deepcopy(S.batch_scene(160, 90, 47, gradient_share=0))
+    n_subs = 25
+    base.tags = np.array([S.solid_tag(i % n_subs, subtract=(i % 4 == 1)) for i in range(base.n_shapes)],
+                         np.uint64)
+    base.substances = np.ascontiguousarray(np.asarray(base.substances, np.float32)[:n_subs])
+    target = H.oracle_render(base, want_f64=False)["f32"].reshape(160, 160, 4).astype(np.float32)
+    sc = A.perturbed(base, 1047)
+    _, dp, dc = O.diff_grad(sc, sc.pairs, sc.substances, target, seg=256)
+    samples = A.fd_samples(sc, target, n=64, seed=5, seg=256)
+    assert len(samples) == 64 and sum(k == "rgba" for k, *_ in samples) >= 8
+    got, fd, err, bad = A.compare(samples, dp, dc)
+    assert not bad.any(), [(s, g) for s, g, b in zip(samples, got, bad) if b]
