@@ -860,8 +860,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) a
     const int wave = threadIdx.x / kWave;
     const int slot = blockIdx.x * (blockDim.x / kWave) + wave;
     const int cap = 1 << f.cap_log2;
-    float4 *q_thr = f.q_thr + (size_t)slot * cap * kWave + lane;
-    uint32_t *q_hdr = f.q_hdr + (size_t)slot * cap * kWave + lane;
+    float4 *q_thr = f.q_thr + ((size_t)slot * kWave + lane) * cap;
+    uint32_t *q_hdr = f.q_hdr + ((size_t)slot * kWave + lane) * cap;
     float4 *orig = a.q_orig + (size_t)slot * 2 * cap * kWave + lane;
     int32_t *sidx = f.shape_idx + (size_t)slot * kSidxWords * kWave + lane;
     TileTable *tt = &tables[wave];

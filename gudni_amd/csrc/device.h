@@ -164,8 +164,8 @@ struct Frame {
     int32_t coverage_mode;
     int32_t out_rgba;             // pixel words in R,G,B,A byte order, else the reference's B,G,R,A
     // scratch
-    float4 *q_thr;                // [slots][cap][64]
-    uint32_t *q_hdr;              // [slots][cap][64]
+    float4 *q_thr;                // [slots][64][cap]
+    uint32_t *q_hdr;              // [slots][64][cap]
     int32_t *shape_idx;           // [slots][max_shapes][64]
     int32_t cap_log2;
     // control
@@ -184,16 +184,21 @@ struct Frame {
     unsigned long long *stage_cycles;
 };
 
-// A lane's threshold queue: a ring in the interleaved scratch.  Logical
-// element 0 is the front of the reference's ThresholdQueue (qSlice.sStart).
+// A lane's threshold queue: a ring of its own in the wave slot's scratch.
+// Logical element 0 is the front of the reference's ThresholdQueue
+// (qSlice.sStart).  A ring's records are contiguous, so the few records at
+// the front of a queue, which the sweep reads again and again, share cache
+// lines (with the rings of a wave interleaved record by record, as the
+// reference interleaves its threshold columns, the 4096^2 frame took 3.6%
+// longer).
 struct Queue {
-    float4 *thr;      // lane's column of the slot's ring (stride 64)
+    float4 *thr;      // the lane's ring (cap records)
     uint32_t *hdr;
     int head, len, mask, maxt;
     bool spilled;     // ring too small: abandon the unit
     bool overflow;    // live queue > maxThresholds (reference would corrupt)
 
-    __device__ __forceinline__ int row(int i) const { return ((head + i) & mask) * kWave; }
+    __device__ __forceinline__ int row(int i) const { return (head + i) & mask; }
     __device__ __forceinline__ Thr get(int i) const
     {
         const float4 v = thr[row(i)];
@@ -427,7 +432,7 @@ __device__ void generate(const Frame &f, const SceneDesc &s, Queue &q, const Col
         uint64_t offset = 16ull * __builtin_amdgcn_readfirstlane(srec[2]);
         const uint32_t num_strands = __builtin_amdgcn_readfirstlane(srec[3]);
         const bool nonzero = (__builtin_amdgcn_readfirstlane(srec[1]) & (uint32_t)(kTagNonzero >> 32)) != 0u;
-        any_nonzero = any_nonzero || nonzero;   // wave-uniform
+        any_nonzero = any_nonzero || nonzero;   // this lane walked a nonzero-fill shape
         const uint32_t label = kCompact ? (uint32_t)n : shape_bits;
         bool added = false, enclosed_shape = false;
         int32_t w0 = 0;   // winding number of the segment's top-left corner (nonzero fill)
@@ -599,7 +604,7 @@ struct Stack {
     // change, sign(e.x m.y - e.y m.x) for an edge of original direction e
     // crossed by a move m, to its counter (wind[label * 64], this lane's) and
     // its bit becomes counter != 0.
-    // any: some record may carry kNonzero (wave-uniform; else this is flip).
+    // any: some record may carry kNonzero (else this is flip).
     template <Cross kHow>
     __device__ __forceinline__ void cross(uint32_t h, int32_t *wind, bool any)
     {
@@ -1021,7 +1026,12 @@ __device__ __forceinline__ uint32_t pixel_word(float4 c, bool rgba)
 }
 
 // renderThresholdArray + calculatePixel + horizontalAdvance + sectionColor
-// (Kernels.cl:1724-1742, 1826-1851, 1881-1916, 1978-2028)
+// (Kernels.cl:1724-1742, 1826-1851, 1881-1916, 1978-2028).  The reference
+// loops over a column's pixels and, inside, over each pixel's sections; here
+// one loop steps either a section or the end of a pixel.  The sections and
+// their arithmetic are the same, but lanes of a wave whose pixels hold
+// different numbers of sections no longer wait for each other at every
+// pixel, only at the end of their columns (the 4096^2 frame: 57.4 -> 51.9 ms).
 template <bool kCompact>
 __device__ uint32_t render(const Frame &f, const SceneDesc &s, Queue &q, const Column &c, Stack st,
                            int32_t *sidx, TileTable *tt, bool nz)
@@ -1039,23 +1049,22 @@ __device__ uint32_t render(const Frame &f, const SceneDesc &s, Queue &q, const C
     }
     long guard = kSectionGuard;
     int ay = c.dy;
-
-
-    int yint = -1;
+    int yint = 0;
     const bool grouped = kCompact && tt->grouped;
     const int lane = threadIdx.x & (kWave - 1);
     st.dirty = 0xFu;
-    for (p.pixel_y = 1.0f; p.pixel_y <= c.float_height; p.pixel_y += 1.0f) {
-        ++yint;
-        float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        float area_sum = 0.0f;
-        if (grouped) {
-            // picture colours depend on the pixel: their groups are stale
-            const uint64_t p0 = st.s0 & tt->pic0, p1 = st.s1 & tt->pic1;
-            st.dirty |= ((uint32_t)p0 ? 1u : 0u) | ((p0 >> 32) ? 2u : 0u) |
-                        ((uint32_t)p1 ? 4u : 0u) | ((p1 >> 32) ? 8u : 0u);
-        }
-        while ((p.sx1 < 1.0f) || (p.sy1 < p.pixel_y)) {
+    p.pixel_y = 1.0f;
+    if (!(p.pixel_y <= c.float_height)) return 0;
+    float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float area_sum = 0.0f;
+    if (grouped) {
+        // picture colours depend on the pixel: their groups are stale
+        const uint64_t p0 = st.s0 & tt->pic0, p1 = st.s1 & tt->pic1;
+        st.dirty |= ((uint32_t)p0 ? 1u : 0u) | ((p0 >> 32) ? 2u : 0u) |
+                    ((uint32_t)p1 ? 4u : 0u) | ((p1 >> 32) ? 8u : 0u);
+    }
+    for (;;) {
+        if ((p.sx1 < 1.0f) || (p.sy1 < p.pixel_y)) {
             if (--guard < 0) return kStatusGuard;
             vertical_advance(q, c, p, st, wind, nz);
             if (q.spilled || q.overflow) return 0;
@@ -1102,15 +1111,28 @@ __device__ uint32_t render(const Frame &f, const SceneDesc &s, Queue &q, const C
             area_sum += adj;
             if (p.current < p.num_active) st.cross<kCrossRight>(q.get_h(p.current), wind, nz);
             ++p.current;
+        } else {
+            // the pixel's last section is done: its colour is the area-weighted
+            // mean (writePixelGlobal, Kernels.cl:1853-1862), then the next pixel
+            const float4 color = make_float4(acc.x / area_sum, acc.y / area_sum, acc.z / area_sum,
+                                             acc.w / area_sum);
+            const long long pix = (long long)(c.dy + yint) * s.width + c.dx;
+            s.out[pix] = pixel_word(color, f.out_rgba != 0);
+            if (s.out_f32) s.out_f32[pix] = color;
+            p.sx0 = 0.0f;
+            p.sy0 = p.pixel_y;
+            ++ay;
+            ++yint;
+            p.pixel_y += 1.0f;
+            if (!(p.pixel_y <= c.float_height)) break;
+            acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            area_sum = 0.0f;
+            if (grouped) {
+                const uint64_t p0 = st.s0 & tt->pic0, p1 = st.s1 & tt->pic1;
+                st.dirty |= ((uint32_t)p0 ? 1u : 0u) | ((p0 >> 32) ? 2u : 0u) |
+                            ((uint32_t)p1 ? 4u : 0u) | ((p1 >> 32) ? 8u : 0u);
+            }
         }
-        const float4 color = make_float4(acc.x / area_sum, acc.y / area_sum, acc.z / area_sum,
-                                         acc.w / area_sum);
-        const long long pix = (long long)(c.dy + yint) * s.width + c.dx;
-        s.out[pix] = pixel_word(color, f.out_rgba != 0);
-        if (s.out_f32) s.out_f32[pix] = color;
-        p.sx0 = 0.0f;
-        p.sy0 = p.pixel_y;
-        ++ay;
     }
     return 0;
 }
@@ -1270,7 +1292,7 @@ __device__ void run_unit(const Frame &f, const SceneDesc &s, int lane, uint32_t 
         }
     }
     const long long t2 = timed ? clock64() : 0;
-    // only a wave that walked a nonzero-fill shape keeps winding counters
+    // only a lane that walked a nonzero-fill shape keeps winding counters
     if (ok) status = render<kCompact>(f, s, q, c, Stack{s0, s1, 0u}, sidx, tt, any_nonzero);
     if (active && q.overflow) status |= kStatusOverflow;
     if (!active && f.dump_qlen) f.dump_qlen[dslot] = -1;

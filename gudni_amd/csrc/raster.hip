@@ -10,13 +10,11 @@
 // 8x8 tile is a single unit).  The grid is persistent; every wave loops on an atomic
 // work counter until the list is exhausted, so every wave reaches the exit.
 //
-// Per-lane threshold queues live in a column-interleaved scratch ring
-// (element i of lane l at row*64 + l): lanes touching the same row hit one
-// contiguous 1 KiB span.  The fast pass gives each lane a ring of kFastCap
-// records; a lane whose live queue outgrows it abandons its unit, which is
-// re-run by a second persistent pass whose rings hold maxThresholds records.
-// The result is identical to one pass at full capacity, with ~1/32 of the
-// scratch.
+// Each lane's threshold queue is a ring of its own in the wave slot's
+// scratch.  The fast pass gives each lane a ring of kFastCap records; a lane
+// whose live queue outgrows it abandons its unit, which is re-run by a second
+// persistent pass whose rings hold maxThresholds records.  The result is
+// identical to one pass at full capacity, with ~1/32 of the scratch.
 //
 #include <hip/hip_runtime.h>
 
@@ -47,8 +45,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) r
     const int wave = threadIdx.x / kWave;
     const int slot = blockIdx.x * (blockDim.x / kWave) + wave;
     const int cap = 1 << f.cap_log2;
-    float4 *q_thr = f.q_thr + (size_t)slot * cap * kWave + lane;
-    uint32_t *q_hdr = f.q_hdr + (size_t)slot * cap * kWave + lane;
+    float4 *q_thr = f.q_thr + ((size_t)slot * kWave + lane) * cap;
+    uint32_t *q_hdr = f.q_hdr + ((size_t)slot * kWave + lane) * cap;
     int32_t *sidx = f.shape_idx + (size_t)slot * kSidxWords * kWave + lane;
     TileTable *tt = &tables[wave];
     const int n_work = from_retry ? (int)min((uint32_t)f.retry_capacity, f.retry[0]) : f.n_work;
