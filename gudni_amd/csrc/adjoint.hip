@@ -596,12 +596,14 @@ __device__ uint32_t sweep_forward(const Frame &f, const AdjDesc &d, Queue &q, co
     p.sx0 = 0.0f; p.sy0 = 0.0f; p.sx1 = 1.0f; p.sy1 = 0.0f;
     p.rnd = 0;
     long guard = kSectionGuard;
-    int ay = c.dy, yint = -1;
-    for (p.pixel_y = 1.0f; p.pixel_y <= c.float_height; p.pixel_y += 1.0f) {
-        ++yint;
-        float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        float area_sum = 0.0f;
-        while ((p.sx1 < 1.0f) || (p.sy1 < p.pixel_y)) {
+    int ay = c.dy;
+    // one loop over sections and pixel ends, as render's
+    p.pixel_y = 1.0f;
+    if (!(p.pixel_y <= c.float_height)) return 0;
+    float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float area_sum = 0.0f;
+    for (;;) {
+        if ((p.sx1 < 1.0f) || (p.sy1 < p.pixel_y)) {
             if (--guard < 0) return kStatusGuard;
             vertical_advance(q, c, p, st);
             if (q.spilled || q.overflow) return 0;
@@ -622,19 +624,24 @@ __device__ uint32_t sweep_forward(const Frame &f, const AdjDesc &d, Queue &q, co
             area_sum += area;
             if (p.current < p.num_active) st.flip(q.get_h(p.current));
             ++p.current;
+        } else {
+            const float4 color = make_float4(acc.x / area_sum, acc.y / area_sum, acc.z / area_sum,
+                                             acc.w / area_sum);
+            const long long pix = (long long)ay * s.width + c.dx;
+            s.out_f32[pix] = color;
+            const float4 tg = d.target[pix];
+            const float4 e = make_float4(color.x - tg.x, color.y - tg.y, color.z - tg.z, color.w - tg.w);
+            loss += e.x * e.x + e.y * e.y + e.z * e.z + e.w * e.w;
+            const float k = 2.0f / area_sum;
+            d.gpix[pix] = make_float4(k * e.x, k * e.y, k * e.z, k * e.w);
+            p.sx0 = 0.0f;
+            p.sy0 = p.pixel_y;
+            ++ay;
+            p.pixel_y += 1.0f;
+            if (!(p.pixel_y <= c.float_height)) break;
+            acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            area_sum = 0.0f;
         }
-        const float4 color = make_float4(acc.x / area_sum, acc.y / area_sum, acc.z / area_sum,
-                                         acc.w / area_sum);
-        const long long pix = (long long)(c.dy + yint) * s.width + c.dx;
-        s.out_f32[pix] = color;
-        const float4 tg = d.target[pix];
-        const float4 e = make_float4(color.x - tg.x, color.y - tg.y, color.z - tg.z, color.w - tg.w);
-        loss += e.x * e.x + e.y * e.y + e.z * e.z + e.w * e.w;
-        const float k = 2.0f / area_sum;
-        d.gpix[pix] = make_float4(k * e.x, k * e.y, k * e.z, k * e.w);
-        p.sx0 = 0.0f;
-        p.sy0 = p.pixel_y;
-        ++ay;
     }
     return 0;
 }
@@ -651,22 +658,24 @@ __device__ uint32_t sweep_backward(const Frame &f, const AdjDesc &d, Queue &q, c
     p.sx0 = 0.0f; p.sy0 = 0.0f; p.sx1 = 1.0f; p.sy1 = 0.0f;
     p.rnd = 0;
     long guard = kSectionGuard;
-    int ay = c.dy, yint = -1;
-    for (p.pixel_y = 1.0f; p.pixel_y <= c.float_height; p.pixel_y += 1.0f) {
-        ++yint;
-        const long long pix = (long long)(c.dy + yint) * s.width + c.dx;
-        const float4 G = d.gpix[pix];
-        bool have_prev = false, prev_inside = false;
-        uint32_t prev_origin = 0;
-        float4 prev_col = make_float4(0.f, 0.f, 0.f, 0.f);
-        // span boundaries of this row: sum C (e_j - e_{j-1}) along the span's
-        // top and bottom edges; `pend` is the boundary at the current span's
-        // top with the span above's bottom sum and the end point it moves with
-        float4 top_sum = make_float4(0.f, 0.f, 0.f, 0.f), bot_sum = top_sum, pend_bot = top_sum;
-        float e_top = 0.0f, e_bot = 0.0f;
-        bool span_open = false, pend = false;
-        int pend_org = -1, pend_end = 0, cur_org = -1, cur_end = 0;
-        while ((p.sx1 < 1.0f) || (p.sy1 < p.pixel_y)) {
+    int ay = c.dy;
+    // one loop over sections and pixel ends, as render's; the span state
+    // below is reset at each pixel's start
+    p.pixel_y = 1.0f;
+    if (!(p.pixel_y <= c.float_height)) return 0;
+    float4 G = d.gpix[(long long)ay * s.width + c.dx];
+    bool have_prev = false, prev_inside = false;
+    uint32_t prev_origin = 0;
+    float4 prev_col = make_float4(0.f, 0.f, 0.f, 0.f);
+    // span boundaries of this row: sum C (e_j - e_{j-1}) along the span's
+    // top and bottom edges; `pend` is the boundary at the current span's
+    // top with the span above's bottom sum and the end point it moves with
+    float4 top_sum = make_float4(0.f, 0.f, 0.f, 0.f), bot_sum = top_sum, pend_bot = top_sum;
+    float e_top = 0.0f, e_bot = 0.0f;
+    bool span_open = false, pend = false;
+    int pend_org = -1, pend_end = 0, cur_org = -1, cur_end = 0;
+    for (;;) {
+        if ((p.sx1 < 1.0f) || (p.sy1 < p.pixel_y)) {
             if (--guard < 0) return kStatusGuard;
             vertical_advance(q, c, p, st);
             if (q.spilled || q.overflow) return 0;
@@ -746,11 +755,29 @@ __device__ uint32_t sweep_backward(const Frame &f, const AdjDesc &d, Queue &q, c
                 have_prev = false;
             }
             ++p.current;
+        } else {
+            if (span_open && pend && pend_org >= 0) span_term(orig, pend_org, pend_end, G, pend_bot, top_sum);
+            p.sx0 = 0.0f;
+            p.sy0 = p.pixel_y;
+            ++ay;
+            p.pixel_y += 1.0f;
+            if (!(p.pixel_y <= c.float_height)) break;
+            G = d.gpix[(long long)ay * s.width + c.dx];
+            have_prev = false;
+            prev_inside = false;
+            prev_origin = 0;
+            prev_col = make_float4(0.f, 0.f, 0.f, 0.f);
+            top_sum = make_float4(0.f, 0.f, 0.f, 0.f);
+            bot_sum = top_sum;
+            pend_bot = top_sum;
+            e_top = e_bot = 0.0f;
+            span_open = false;
+            pend = false;
+            pend_org = -1;
+            pend_end = 0;
+            cur_org = -1;
+            cur_end = 0;
         }
-        if (span_open && pend && pend_org >= 0) span_term(orig, pend_org, pend_end, G, pend_bot, top_sum);
-        p.sx0 = 0.0f;
-        p.sy0 = p.pixel_y;
-        ++ay;
     }
     return 0;
 }

@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 
 #include <mutex>
+#include <system_error>
+#include <thread>
 
 #include <cstring>
 #include <string>
@@ -464,6 +466,55 @@ int validate_uses(gudni_ctx *ctx, const void *pict_uses, int32_t n_pict_uses,
 // geometry heap with even 8-byte sizes >= 4 (so float4 reads stay aligned),
 // substance ids inside the substance table and picture ids inside the usage
 // table.
+// One shape reference of a job: its substance and strands must lie inside
+// the uploaded tables and heap.  Writes the reference's extent (the
+// generation prefilter's input) and returns null, or returns what is wrong.
+static const char *ref_extent(const CommonView &cv, const ShapeRec &r, float *extent)
+{
+    const uint64_t sid = r.tag & dev::kTagIdMask;
+    if (sid >= (uint64_t)cv.n_subs) return "shape substance id outside the substance table";
+    const uint64_t type = r.tag & dev::kTagTypeMask;
+    if (type != dev::kTagSolid) {
+        const uint32_t id = __builtin_bit_cast(uint32_t, cv.subs[sid].x);
+        if (type == dev::kTagGradient) {
+            if (id >= (uint32_t)cv.n_grads) return "gradient substance refers to a missing gradient";
+        } else if (id >= (uint32_t)cv.n_uses) {
+            return "picture substance refers to a missing usage";
+        }
+    }
+    const size_t geo_bytes = cv.geo_bytes;
+    size_t pos = 16u * (size_t)r.geo_start;
+    // Extent of the shape's strands: x from the end points, y over every
+    // point (curves stay inside their control hull).  NaN coordinates can
+    // never pass the kernel's range tests, so they are left out; a NaN y
+    // makes the y extent unknown, which disables the y test.
+    float xlo = INFINITY, xhi = -INFINITY, ylo = INFINITY, yhi = -INFINITY;
+    bool y_nan = false;
+    for (uint32_t k = 0; k < r.num_strands; ++k) {
+        if (pos + 8 > geo_bytes) return "strand header outside the geometry heap";
+        uint16_t size;
+        std::memcpy(&size, cv.geo + pos, 2);
+        if (size < 4 || (size & 1) || pos + 8u * size > geo_bytes) return "malformed strand in the geometry heap";
+        const float *pts = reinterpret_cast<const float *>(cv.geo + pos + 8);
+        const int npts = (int)size - 1;         // right, left, control, tree pairs
+        if (pts[2] < xlo) xlo = pts[2];
+        if (pts[0] > xhi) xhi = pts[0];
+        for (int p = 0; p < npts; ++p) {
+            const float y = pts[2 * p + 1];
+            if (y != y) y_nan = true;
+            if (y < ylo) ylo = y;
+            if (y > yhi) yhi = y;
+        }
+        pos += 8u * size;
+    }
+    if (y_nan) ylo = -INFINITY;
+    extent[0] = xlo;
+    extent[1] = ylo;
+    extent[2] = xhi;
+    extent[3] = yhi;
+    return nullptr;
+}
+
 int validate_job(gudni_ctx *ctx, const CommonView &cv, const ShapeRec *shapes,
                         int32_t n_shapes, const TileRec *tiles, int32_t n_tiles, int32_t w,
                         int32_t h, std::vector<float> &xrange)
@@ -479,54 +530,39 @@ int validate_job(gudni_ctx *ctx, const CommonView &cv, const ShapeRec *shapes,
         if (r.box[0] < 0 || r.box[1] < 0)
             return set_error(ctx, GUDNI_ERR_ARG, "tile box has a negative origin");
     }
-    const size_t geo_bytes = cv.geo_bytes;
-    for (int32_t s = 0; s < n_shapes; ++s) {
-        const ShapeRec &r = shapes[s];
-        const uint64_t sid = r.tag & dev::kTagIdMask;
-        if (sid >= (uint64_t)cv.n_subs)
-            return set_error(ctx, GUDNI_ERR_ARG, "shape substance id outside the substance table");
-        const uint64_t type = r.tag & dev::kTagTypeMask;
-        if (type != dev::kTagSolid) {
-            const uint32_t id = __builtin_bit_cast(uint32_t, cv.subs[sid].x);
-            if (type == dev::kTagGradient) {
-                if (id >= (uint32_t)cv.n_grads)
-                    return set_error(ctx, GUDNI_ERR_ARG, "gradient substance refers to a missing gradient");
-            } else if (id >= (uint32_t)cv.n_uses) {
-                return set_error(ctx, GUDNI_ERR_ARG, "picture substance refers to a missing usage");
+    // the shape references in parallel chunks; the first malformed reference
+    // (lowest index) is the one reported
+    const int64_t n = n_shapes;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const unsigned nt = n >= 65536 ? std::min(16u, hw) : 1u;
+    std::vector<int64_t> bad(nt, -1);
+    std::vector<const char *> why(nt, nullptr);
+    auto chunk = [&](unsigned t) {
+        const int64_t lo = n * t / nt, hi = n * (t + 1) / nt;
+        for (int64_t i = lo; i < hi; ++i) {
+            const char *m = ref_extent(cv, shapes[i], &xrange[4 * (size_t)i]);
+            if (m) {
+                bad[t] = i;
+                why[t] = m;
+                return;
             }
         }
-        size_t pos = 16u * (size_t)r.geo_start;
-        // Extent of the shape's strands: x from the end points, y over every
-        // point (curves stay inside their control hull).  NaN coordinates can
-        // never pass the kernel's range tests, so they are left out; a NaN y
-        // makes the y extent unknown, which disables the y test.
-        float xlo = INFINITY, xhi = -INFINITY, ylo = INFINITY, yhi = -INFINITY;
-        bool y_nan = false;
-        for (uint32_t k = 0; k < r.num_strands; ++k) {
-            if (pos + 8 > geo_bytes)
-                return set_error(ctx, GUDNI_ERR_ARG, "strand header outside the geometry heap");
-            uint16_t size;
-            std::memcpy(&size, cv.geo + pos, 2);
-            if (size < 4 || (size & 1) || pos + 8u * size > geo_bytes)
-                return set_error(ctx, GUDNI_ERR_ARG, "malformed strand in the geometry heap");
-            const float *pts = reinterpret_cast<const float *>(cv.geo + pos + 8);
-            const int npts = (int)size - 1;         // right, left, control, tree pairs
-            if (pts[2] < xlo) xlo = pts[2];
-            if (pts[0] > xhi) xhi = pts[0];
-            for (int p = 0; p < npts; ++p) {
-                const float y = pts[2 * p + 1];
-                if (y != y) y_nan = true;
-                if (y < ylo) ylo = y;
-                if (y > yhi) yhi = y;
-            }
-            pos += 8u * size;
+    };
+    std::vector<std::thread> pool;
+    unsigned started = 1;   // chunks 1 .. started-1 have a thread of their own
+    try {
+        for (unsigned t = 1; t < nt; ++t) {
+            pool.emplace_back(chunk, t);
+            ++started;
         }
-        if (y_nan) ylo = -INFINITY;
-        xrange[4 * (size_t)s + 0] = xlo;
-        xrange[4 * (size_t)s + 1] = ylo;
-        xrange[4 * (size_t)s + 2] = xhi;
-        xrange[4 * (size_t)s + 3] = yhi;
+    } catch (const std::system_error &) {
+        // no more threads: the calling thread takes the remaining chunks
     }
+    chunk(0);
+    for (unsigned t = started; t < nt; ++t) chunk(t);
+    for (std::thread &th : pool) th.join();
+    for (unsigned t = 0; t < nt; ++t)
+        if (bad[t] >= 0) return set_error(ctx, GUDNI_ERR_ARG, why[t]);
     return GUDNI_OK;
 }
 

@@ -240,6 +240,44 @@ def test_queue_overflow_is_an_error():
         r.close()
 
 
+def test_level1_rejects_malformed_references(rasterizer):
+    """A job's shape references are checked on the host before anything runs
+    (in parallel chunks for a large job): a substance id outside the table or
+    a strand outside the geometry heap is an error naming the first malformed
+    reference, and the same job with valid references renders."""
+    scene = S.uniform_512(300, 256, 5)
+    _, geo, jobs = G.prepare(scene)
+    rasterizer.upload_common(geo.geo, scene)
+    tiles, shapes = jobs.job(0)
+    ref_t = np.dtype([("tag", "<u8"), ("geo", "<u4"), ("n", "<u4")])
+    refs = np.frombuffer(np.asarray(shapes, np.uint8).tobytes(), ref_t)
+    big = np.resize(refs, 200_000)   # the tiles read the first len(refs) of them
+    heap_units = np.asarray(geo.geo, np.uint8).size // 16
+
+    def job_with(i_sid, i_geo):
+        bad = big.copy()
+        if i_sid is not None:
+            bad["tag"][i_sid] = (int(bad["tag"][i_sid]) & ~int(S.TAG_ID_MASK)) | (len(scene.substances) + 7)
+        if i_geo is not None:
+            bad["geo"][i_geo] = heap_units + 3
+            bad["n"][i_geo] = 1
+        return np.frombuffer(bad.tobytes(), np.uint8)
+
+    out = np.zeros(scene.width * scene.height, np.uint32)
+    for i_sid, i_geo, want in [(150_000, 190_000, "substance id"), (180_000, 120_000, "geometry heap"),
+                               (70_000, None, "substance id"), (None, 199_999, "geometry heap")]:
+        say(f"malformed references at {i_sid}, {i_geo}")
+        with pytest.raises(G.GudniError) as ei:
+            rasterizer.raster_job(job_with(i_sid, i_geo), tiles, scene.background, scene.width, scene.height,
+                                  0, jobs.n_jobs - 1, out)
+        assert ei.value.code == -1 and want in str(ei.value), str(ei.value)
+    good = rasterizer.raster_job(job_with(None, None), tiles, scene.background, scene.width, scene.height,
+                                 0, jobs.n_jobs - 1, out)
+    alone = rasterizer.raster_job(np.asarray(shapes, np.uint8), tiles, scene.background, scene.width,
+                                  scene.height, 0, jobs.n_jobs - 1, np.zeros_like(out))
+    assert np.array_equal(good, alone)
+
+
 def test_empty_and_offcanvas(rasterizer):
     """A scene whose only shape lies off the canvas is culled (excludeBox):
     every pixel is the background."""
