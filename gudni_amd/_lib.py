@@ -114,8 +114,8 @@ SIGNATURES = {
     "gudni_batch_check": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "gudni_batch_frame": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "gudni_batch_free": (None, [C.c_void_p]),
-    "gudni_order_work": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
-                                   C.c_void_p]),
+    "gudni_order_work": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
+                                   C.c_int32, C.c_void_p]),
     "gudni_scene_upload": (C.c_int, [C.c_void_p, C.POINTER(SceneDesc)]),
     "gudni_scene_bin": (C.c_int, [C.c_void_p]),
     "gudni_scene_counts": (C.c_int, [C.c_void_p, C.c_void_p]),

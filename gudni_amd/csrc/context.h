@@ -141,10 +141,13 @@ void build_work(const gudni_spec &spec, const TileRec *tiles, int n_tiles, int w
 // rows per lane x ((tile shapes + 1) / tile width)^2 within a factor 8 of the
 // longest) sorted longest first and dealt tile by tile, round robin, to the
 // starts of the `parts` contiguous ranges the claimers begin with, so none is
-// claimed last; every other unit follows in its original order, which keeps
-// a scene's tiles together on one XCD.
-void order_longest_first(int threads_per_tile, const TileRec *tiles, const uint32_t *work, int n, int parts,
-                         uint32_t *out);
+// claimed last.  With tile_scene (each tile's scene) the other units follow
+// scene by scene, the cheapest scene (sum of its units' costs) first, each
+// scene's units together and dealt round robin to the ranges, so scenes
+// finish in order of cost on every XCD; without it they keep their original
+// order, range after range.  Either way a scene's tiles stay on one XCD.
+void order_longest_first(int threads_per_tile, const TileRec *tiles, const int32_t *tile_scene,
+                         const uint32_t *work, int n, int parts, uint32_t *out);
 // Scratch rings of the raster passes, sized on first use.
 int ensure_scratch(gudni_ctx *ctx);
 // Pass counters, status word and retry list for n_work units.

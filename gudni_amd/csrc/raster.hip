@@ -21,6 +21,7 @@
 #include <mutex>
 #include <system_error>
 #include <thread>
+#include <unordered_map>
 
 #include <cstring>
 #include <string>
@@ -211,8 +212,8 @@ void build_work(const gudni_spec &spec, const TileRec *tiles, int n_tiles, int w
     }
 }
 
-void order_longest_first(int threads_per_tile, const TileRec *tiles, const uint32_t *work, int n, int parts,
-                         uint32_t *out)
+void order_longest_first(int threads_per_tile, const TileRec *tiles, const int32_t *tile_scene,
+                         const uint32_t *work, int n, int parts, uint32_t *out)
 {
     const int cd = ceil_log2(threads_per_tile);
     // a lane sweeps `rows` pixels whose cost grows with the tile's shape
@@ -251,12 +252,43 @@ void order_longest_first(int threads_per_tile, const TileRec *tiles, const uint3
         p = (p + 1) % parts;
         i = j;
     }
-    // then every other unit in its original order, range after range
-    p = 0;
+    if (!tile_scene) {
+        // then every other unit in its original order, range after range
+        p = 0;
+        for (int i = 0; i < n; ++i) {
+            if (placed[i]) continue;
+            while (fill[p] == size[p]) ++p;
+            out[begin[p] + fill[p]++] = work[i];
+        }
+        return;
+    }
+    // then the scenes' other units, cheapest scene first, a scene at a time
+    // dealt round robin to the ranges (its units together and in their
+    // order, spilling into the next range with room when one is full): each
+    // XCD finishes its small scenes early instead of every scene finishing
+    // near the end of the batch
+    std::vector<int> scenes;
+    std::unordered_map<int, std::vector<int>> units_of;
+    std::unordered_map<int, double> scene_cost;
     for (int i = 0; i < n; ++i) {
         if (placed[i]) continue;
-        while (fill[p] == size[p]) ++p;
-        out[begin[p] + fill[p]++] = work[i];
+        const int sc = tile_scene[work[i] >> 2];
+        auto it = units_of.find(sc);
+        if (it == units_of.end()) {
+            scenes.push_back(sc);
+            it = units_of.emplace(sc, std::vector<int>()).first;
+        }
+        it->second.push_back(i);
+        scene_cost[sc] += cost[i];
+    }
+    std::stable_sort(scenes.begin(), scenes.end(), [&](int a, int b) { return scene_cost[a] < scene_cost[b]; });
+    p = 0;
+    for (int sc : scenes) {
+        for (int i : units_of[sc]) {
+            while (fill[p] == size[p]) p = (p + 1) % parts;
+            out[begin[p] + fill[p]++] = work[i];
+        }
+        p = (p + 1) % parts;
     }
 }
 
@@ -1449,8 +1481,8 @@ int gudni_batch_upload(gudni_ctx *ctx, int32_t n_scenes, const gudni_batch_scene
     if (e == hipSuccess) e = up(b->work, work.data(), 4 * work.size());
     if (e == hipSuccess) {
         std::vector<uint32_t> all(work.size());
-        order_longest_first(ctx->spec.threads_per_tile, tiles.data(), work.data(), (int)work.size(), dev::kParts,
-                            all.data());
+        order_longest_first(ctx->spec.threads_per_tile, tiles.data(), tile_scene.data(), work.data(),
+                            (int)work.size(), dev::kParts, all.data());
         e = up(b->work_all, all.data(), 4 * all.size());
     }
     if (e == hipSuccess) e = up(b->random, random_field, 4 * (size_t)n_random);
@@ -1651,11 +1683,15 @@ int gudni_batch_upload_outlines(gudni_ctx *ctx, int32_t n_scenes, const gudni_ou
     b->n_work = (int32_t)(work_used / 4);
     {
         std::vector<TileRec> th(tiles_used / 32);
+        std::vector<int32_t> ts(tiles_used / 32);
         std::vector<uint32_t> wh((size_t)b->n_work), all((size_t)b->n_work);
         if (!th.empty()) e = hipMemcpy(th.data(), b->tiles.p, tiles_used, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && !ts.empty())
+            e = hipMemcpy(ts.data(), b->tile_scene.p, 4 * ts.size(), hipMemcpyDeviceToHost);
         if (e == hipSuccess && !wh.empty()) e = hipMemcpy(wh.data(), b->work.p, work_used, hipMemcpyDeviceToHost);
         if (e != hipSuccess) return hip_fail(e);
-        order_longest_first(ctx->spec.threads_per_tile, th.data(), wh.data(), b->n_work, dev::kParts, all.data());
+        order_longest_first(ctx->spec.threads_per_tile, th.data(), ts.data(), wh.data(), b->n_work, dev::kParts,
+                            all.data());
         e = upload(ctx, b->work_all, all.data(), 4 * all.size());
         if (e != hipSuccess) return hip_fail(e);
     }
@@ -1799,8 +1835,8 @@ int gudni_batch_frame(gudni_ctx *ctx, gudni_batch *b, int32_t scene, uint32_t *h
     return GUDNI_OK;
 }
 
-int gudni_order_work(const void *tiles, int32_t n_tiles, const uint32_t *work, int32_t n_work,
-                     int32_t threads_per_tile, int32_t parts, uint32_t *out)
+int gudni_order_work(const void *tiles, int32_t n_tiles, const int32_t *tile_scene, const uint32_t *work,
+                     int32_t n_work, int32_t threads_per_tile, int32_t parts, uint32_t *out)
 {
     if (n_tiles < 0 || n_work < 0 || parts < 1 || threads_per_tile < 1 || (n_work && (!tiles || !work || !out)))
         return GUDNI_ERR_ARG;
@@ -1811,7 +1847,7 @@ int gudni_order_work(const void *tiles, int32_t n_tiles, const uint32_t *work, i
         const TileRec &r = t[work[i] >> 2];
         if (r.hdepth < 0 || r.hdepth > 15) return GUDNI_ERR_ARG;
     }
-    order_longest_first(threads_per_tile, t, work, n_work, parts, out);
+    order_longest_first(threads_per_tile, t, tile_scene, work, n_work, parts, out);
     return GUDNI_OK;
 }
 

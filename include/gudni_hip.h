@@ -461,13 +461,17 @@ void gudni_batch_free(gudni_batch *b);
  * width)^2; the units within a factor 8 of the costliest go first, longest
  * first, dealt tile by tile, round robin, to the starts of the `parts`
  * contiguous ranges the launch's claimers begin with (range p = [n p / parts,
- * n (p+1) / parts)), so that none is claimed last; the others follow in
- * their original order.  Batches built by
- * gudni_batch_upload and gudni_batch_upload_outlines use it; per-scene
- * launches keep each scene's own order.  An extension: the reference launches
- * one job at a time (OpenCL/CallKernels.hs:244-255). */
-int  gudni_order_work(const void *tiles, int32_t n_tiles, const uint32_t *work, int32_t n_work,
-                      int32_t threads_per_tile, int32_t parts, uint32_t *out);
+ * n (p+1) / parts)), so that none is claimed last.  With `tile_scene` (the
+ * scene of each tile; may be null) the others follow scene by scene, the
+ * scene with the smallest summed cost first, each scene's units together and
+ * in their order, scenes dealt round robin to the ranges (a full range passes
+ * the rest on to the next), so small scenes finish early; without it they
+ * follow in their original order.  Batches built by gudni_batch_upload and
+ * gudni_batch_upload_outlines use it with their scenes; per-scene launches
+ * keep each scene's own order.  An extension: the reference launches one job
+ * at a time (OpenCL/CallKernels.hs:244-255). */
+int  gudni_order_work(const void *tiles, int32_t n_tiles, const int32_t *tile_scene, const uint32_t *work,
+                      int32_t n_work, int32_t threads_per_tile, int32_t parts, uint32_t *out);
 
 /* Test hook: per-column threshold records after generate (queue order) and
  * after sort, plus the initial shape state, for one RasterJob.  Slot =

@@ -99,7 +99,7 @@ def test_work_order_longest_first():
                     np.uint32)
     out = np.zeros_like(work)
     L = _lib.load()
-    assert L.gudni_order_work(tiles.ctypes.data, n_tiles, work.ctypes.data, work.size, 256, 8,
+    assert L.gudni_order_work(tiles.ctypes.data, n_tiles, None, work.ctypes.data, work.size, 256, 8,
                               out.ctypes.data) == 0
     assert sorted(out.tolist()) == sorted(work.tolist())
 
@@ -128,11 +128,66 @@ def test_work_order_longest_first():
         if t[i] == t[i - 1]:
             assert (out[i] & 3) == (out[i - 1] & 3) + 1
     bad = np.zeros(4, np.uint32)
-    assert L.gudni_order_work(tiles.ctypes.data, 1, work.ctypes.data, 4, 256, 8, bad.ctypes.data) != 0
+    assert L.gudni_order_work(tiles.ctypes.data, 1, None, work.ctypes.data, 4, 256, 8, bad.ctypes.data) != 0
     deep = tiles.copy()
     deep[:, 4] = 40                                         # hdepth beyond any tile width
-    assert L.gudni_order_work(deep.ctypes.data, n_tiles, work.ctypes.data, work.size, 256, 8,
+    assert L.gudni_order_work(deep.ctypes.data, n_tiles, None, work.ctypes.data, work.size, 256, 8,
                               out.ctypes.data) != 0
+
+
+def test_work_order_cheapest_scene_first():
+    """gudni_order_work with the tiles' scenes: the long units open the
+    ranges as without them; after them each range holds whole runs of scenes
+    in order of their summed cost, cheapest first, each scene's units in
+    their original order, and the scenes are dealt round robin."""
+    rng = np.random.default_rng(9)
+    n_scenes = 40
+    tiles, tile_scene = [], []
+    for sc in range(n_scenes):
+        k = int(rng.integers(1, 30))
+        for _ in range(k):
+            hd = int(rng.integers(3, 7))
+            tiles.append([0, 0, 1 << hd, 1 << hd, hd | (hd << 16), 0, 0, int(rng.integers(0, 120))])
+            tile_scene.append(sc)
+    tiles = np.array(tiles, np.int32)
+    tile_scene = np.array(tile_scene, np.int32)
+    hd = tiles[:, 4] & 0xFFFF
+    n_tiles = len(tiles)
+    work = np.array([(t << 2) | q for t in range(n_tiles) for q in range(4) if q < max(1, (1 << hd[t]) // 64)],
+                    np.uint32)
+    out = np.zeros_like(work)
+    L = _lib.load()
+    assert L.gudni_order_work(tiles.ctypes.data, n_tiles, tile_scene.ctypes.data, work.ctypes.data, work.size,
+                              256, 8, out.ctypes.data) == 0
+    assert sorted(out.tolist()) == sorted(work.tolist())
+    t = (work >> 2).astype(np.int64)
+    rows = np.maximum(1, tiles[t, 3] >> np.maximum(0, 8 - hd[t]))
+    cost = dict(zip(work.tolist(), (rows * ((tiles[t, 7] + 1.0) / (1 << hd[t])) ** 2).tolist()))
+    top = max(cost.values())
+    heavy = {u for u, c in cost.items() if c * 8 >= top}
+    scene_cost = {}
+    for u, c in cost.items():
+        if u not in heavy:
+            scene_cost[int(tile_scene[u >> 2])] = scene_cost.get(int(tile_scene[u >> 2]), 0.0) + c
+    pos = {int(u): i for i, u in enumerate(work)}
+    n = work.size
+    bounds = [n * p // 8 for p in range(9)]
+    first_scene = []
+    for p in range(8):
+        rng_units = [int(u) for u in out[bounds[p]:bounds[p + 1]]]
+        k = sum(u in heavy for u in rng_units)
+        assert all(u in heavy for u in rng_units[:k]), "long units open the range"
+        rest = rng_units[k:]
+        costs = [scene_cost[int(tile_scene[u >> 2])] for u in rest]
+        assert costs == sorted(costs), "cheapest scenes first"
+        for sc in {int(tile_scene[u >> 2]) for u in rest}:
+            mine = [pos[u] for u in rest if tile_scene[u >> 2] == sc]
+            assert mine == sorted(mine), "a scene's units keep their order"
+        if rest:
+            first_scene.append(int(tile_scene[rest[0] >> 2]))
+    # round robin: the eight cheapest scenes open the eight ranges' scene runs
+    cheapest = sorted(scene_cost, key=lambda sc: (scene_cost[sc], sc))[:8]
+    assert sorted(first_scene) == sorted(cheapest)
 
 
 def test_build_fingerprint_tracks_sources(tmp_path, monkeypatch):
