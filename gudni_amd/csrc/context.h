@@ -62,6 +62,9 @@ struct gudni_ctx {
     gudni::dev::SceneDesc single_scene;  // host copy of what scene_table holds
     gudni::DevBuf cycles;    // stage cycle counters of the fast pass (timing only)
     gudni::DevBuf dump;      // dump staging
+    // pinned host staging of the frame box a Level-1 job reads and writes
+    void *l1_stage = nullptr;
+    size_t l1_stage_bytes = 0;
     int32_t retry_capacity = 0;
     // bumped whenever the launch scratch (rings, control words) or the fast
     // grid may have changed: a captured graph holds those pointers and sizes

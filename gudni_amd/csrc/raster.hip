@@ -866,6 +866,7 @@ void gudni_destroy(gudni_ctx *ctx)
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t &e : ctx->ser_ev)
         if (e) (void)hipEventDestroy(e);
+    if (ctx->l1_stage) (void)hipHostFree(ctx->l1_stage);
     delete ctx;
 }
 
@@ -956,15 +957,16 @@ static int run_job(gudni_ctx *ctx, const void *shapes, int32_t n_shapes, const v
     CHECK_HIP(ctx, upload(ctx, ctx->work, work.data(), 4u * work.size()));
     const size_t npix = (size_t)w * (size_t)h;
     CHECK_HIP(ctx, ctx->out.ensure(4 * npix));
-    // the pixels a job can write lie in the rows its tiles' columns reach:
-    // only those rows of the host frame travel, both ways and in one piece
-    // each, so every other pixel stays as it was.  A column is active while
-    // its row origin lies inside the box and the canvas, and it writes 2^dd
-    // rows from there, clipped to the canvas but not to the box
-    // (initTileState / isActiveThread, Kernels.cl:1692-1722): a box shorter
-    // than its depths say is written below its bottom.  Origins are >= 0.
+    // the pixels a job can write lie in the box of its tiles' columns: only
+    // that box of the host frame travels, both ways, so every other pixel
+    // stays as it was.  A column is active while its row origin lies inside
+    // the box and the canvas, and it writes 2^dd rows from there, clipped to
+    // the canvas but not to the box (initTileState / isActiveThread,
+    // Kernels.cl:1692-1722): a box shorter than its depths say is written
+    // below its bottom.  Origins are >= 0.  The box goes through a pinned
+    // staging buffer, row by row on the host, in one transfer each way.
     const int cd = ceil_log2(ctx->spec.threads_per_tile);
-    int32_t by0 = h, by1 = 0;
+    int32_t by0 = h, by1 = 0, bx0 = w, bx1 = 0;
     for (int32_t t = 0; t < n_tiles; ++t) {
         const TileRec &r = ((const TileRec *)tiles)[t];
         const int64_t bh = (int64_t)r.box[3] - r.box[1];
@@ -974,12 +976,26 @@ static int run_job(gudni_ctx *ctx, const void *shapes, int32_t n_shapes, const v
         const int64_t rows = std::min<int64_t>((bh + seg - 1) / seg, ctx->spec.threads_per_tile >> r.hdepth);
         by0 = std::min(by0, r.box[1]);
         by1 = (int32_t)std::max<int64_t>(by1, std::min<int64_t>(h, r.box[1] + rows * seg));
+        bx0 = std::min(bx0, r.box[0]);
+        bx1 = (int32_t)std::max<int64_t>(bx1, std::min<int64_t>(w, (int64_t)r.box[0] + ((int64_t)1 << r.hdepth)));
     }
-    const bool window = host_out && by0 < by1;
-    const size_t win_off = 4 * (size_t)by0 * (size_t)w, win_bytes = 4 * (size_t)(by1 - by0) * (size_t)w;
-    if (window)
-        CHECK_HIP(ctx, hipMemcpyAsync((uint8_t *)ctx->out.p + win_off, (const uint8_t *)host_out + win_off, win_bytes,
-                                      hipMemcpyHostToDevice, ctx->stream));
+    const bool window = host_out && by0 < by1 && bx0 < bx1;
+    const size_t bw = window ? (size_t)(bx1 - bx0) : 0, bh = window ? (size_t)(by1 - by0) : 0;
+    uint32_t *box_dev = (uint32_t *)ctx->out.p + (size_t)by0 * (size_t)w + (size_t)bx0;
+    if (window) {
+        if (ctx->l1_stage_bytes < 4 * bw * bh) {
+            if (ctx->l1_stage) CHECK_HIP(ctx, hipHostFree(ctx->l1_stage));
+            ctx->l1_stage = nullptr;
+            ctx->l1_stage_bytes = 0;
+            CHECK_HIP(ctx, hipHostMalloc(&ctx->l1_stage, 4 * bw * bh, hipHostMallocDefault));
+            ctx->l1_stage_bytes = 4 * bw * bh;
+        }
+        uint32_t *stage = (uint32_t *)ctx->l1_stage;
+        for (size_t y = 0; y < bh; ++y)
+            std::memcpy(stage + y * bw, host_out + (size_t)(by0 + y) * w + bx0, 4 * bw);
+        CHECK_HIP(ctx, hipMemcpy2DAsync(box_dev, 4 * (size_t)w, stage, 4 * bw, 4 * bw, bh, hipMemcpyHostToDevice,
+                                        ctx->stream));
+    }
     dev::Frame f = base_frame(ctx);
     f.tiles = (const TileRec *)ctx->tiles.p;
     f.tile_job = (const int32_t *)ctx->tile_job.p;
@@ -1023,9 +1039,14 @@ static int run_job(gudni_ctx *ctx, const void *shapes, int32_t n_shapes, const v
     if (rc) return rc;
     rc = read_status(ctx);
     if (rc) return rc;
-    if (window)
-        CHECK_HIP(ctx, hipMemcpyAsync((uint8_t *)host_out + win_off, (const uint8_t *)ctx->out.p + win_off, win_bytes,
-                                      hipMemcpyDeviceToHost, ctx->stream));
+    if (window) {
+        uint32_t *stage = (uint32_t *)ctx->l1_stage;
+        CHECK_HIP(ctx, hipMemcpy2DAsync(stage, 4 * bw, box_dev, 4 * (size_t)w, 4 * bw, bh, hipMemcpyDeviceToHost,
+                                        ctx->stream));
+        CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        for (size_t y = 0; y < bh; ++y)
+            std::memcpy(host_out + (size_t)(by0 + y) * w + bx0, stage + y * bw, 4 * bw);
+    }
     if (dump) {
         std::vector<uint8_t> hb(total);
         CHECK_HIP(ctx, hipMemcpyAsync(hb.data(), ctx->dump.p, total, hipMemcpyDeviceToHost, ctx->stream));
