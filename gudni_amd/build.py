@@ -45,7 +45,11 @@ def _command(out: str):
 def fingerprint() -> str:
     """Hash of everything the library is built from: the compile command
     (compiler, target, flags) and the bytes of every source and header."""
-    h = hashlib.sha256(" ".join(_command("lib")).encode())
+    # the command with the tree's own location left out, so that a copy of
+    # the tree elsewhere (built here, run there) keeps its fingerprint
+    cmd = " ".join(os.path.relpath(a, ROOT) if os.path.isabs(a) and a.startswith(ROOT + os.sep) else a
+                   for a in _command("lib"))
+    h = hashlib.sha256(cmd.encode())
     files = sorted(set(SOURCES) | set(glob.glob(os.path.join(HERE, "csrc", "*")))
                    | set(glob.glob(os.path.join(ROOT, "include", "*.h"))))
     for f in files:

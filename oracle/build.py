@@ -30,7 +30,11 @@ def _command(out: str):
 
 def fingerprint() -> str:
     """Hash of the compile command and the bytes of every source."""
-    h = hashlib.sha256(" ".join(_command("lib")).encode())
+    # the command with the tree's own location left out, so that a copy of
+    # the tree elsewhere keeps its fingerprint
+    cmd = " ".join(os.path.relpath(a, HERE) if os.path.isabs(a) and a.startswith(HERE + os.sep) else a
+                   for a in _command("lib"))
+    h = hashlib.sha256(cmd.encode())
     for f in SOURCES:
         with open(f, "rb") as fh:
             h.update(os.path.basename(f).encode())

@@ -210,6 +210,25 @@ def test_build_fingerprint_tracks_sources(tmp_path, monkeypatch):
     assert pbuild.fingerprint() != fp and pbuild.stale()
 
 
+def test_build_fingerprint_moves_with_the_tree(tmp_path):
+    """The same sources and flags in a copy of the tree at another path have
+    the same fingerprint, so a tree built here is not rebuilt where it runs."""
+    import shutil
+    import subprocess
+    import sys
+    from gudni_amd import build as pbuild
+    from oracle import build as obuild
+    root = pbuild.ROOT
+    copy = tmp_path / "elsewhere"
+    for d in ("gudni_amd", "include", "oracle"):
+        shutil.copytree(os.path.join(root, d), copy / d,
+                        ignore=shutil.ignore_patterns("*.so", "*.stamp", "__pycache__"))
+    code = ("from gudni_amd import build as B; from oracle import build as O; "
+            "print(B.fingerprint()); print(O.fingerprint())")
+    out = subprocess.run([sys.executable, "-c", code], cwd=copy, capture_output=True, text=True, check=True)
+    assert out.stdout.split() == [pbuild.fingerprint(), obuild.fingerprint()]
+
+
 def test_work_units_cover_every_thread_once():
     """The four work units of a tile run each of its 256 threads (reference
     column indices) exactly once, in blocks as square as the tile allows."""
