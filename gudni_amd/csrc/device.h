@@ -754,12 +754,26 @@ __device__ __forceinline__ int f2i_floor(float v)
     return (int)floorf(v);
 }
 
-__device__ __forceinline__ float4 texel(const SceneDesc &s, const PictUse &u, long long x, long long y)
+// b / 255 for a byte b, correctly rounded: the product by 1/255 (rounded to
+// nearest) corrected by one FMA of its exact residual, which equals the IEEE
+// division for every one of the 256 bytes.
+__device__ __forceinline__ float unit_byte(uint32_t b)
 {
-    if (x < 0 || y < 0 || x >= u.w || y >= u.h) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const uchar4 px = *reinterpret_cast<const uchar4 *>(s.pict + u.mem_offset + 4ll * (y * u.w + x));
-    return make_float4((float)px.x / 255.0f, (float)px.y / 255.0f, (float)px.z / 255.0f,
-                       (float)px.w / 255.0f);
+    const float x = (float)b;
+    const float r = 0x1.010102p-8f;
+    const float q = x * r;
+    return fmaf(fmaf(-q, 255.0f, x), r, q);
+}
+
+// Texel (x, y) of a picture use, transparent black outside it.  The
+// coordinates are taken as unsigned, so a negative one is outside too; a
+// texel's byte offset fits in 31 bits (validate_uses).
+__device__ __forceinline__ float4 texel(const SceneDesc &s, const PictUse &u, uint32_t x, uint32_t y)
+{
+    if (x >= (uint32_t)u.w || y >= (uint32_t)u.h) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const uint32_t off = (uint32_t)u.mem_offset + 4u * (y * (uint32_t)u.w + x);
+    const uchar4 px = *reinterpret_cast<const uchar4 *>(s.pict + off);
+    return make_float4(unit_byte(px.x), unit_byte(px.y), unit_byte(px.z), unit_byte(px.w));
 }
 
 // Gradient substance at the pixel's integer position (where pictures are
@@ -772,15 +786,19 @@ __device__ float4 eval_gradient(const float4 *g, int ax, int ay)
     int n = (int)__float_as_uint(h0.y);
     const float x0 = h0.z, y0 = h0.w, x1 = h1.x, y1 = h1.y;
     const float px = (float)ax, py = (float)ay;
-    float t;
+    // radial: |p - c| / radius; linear: (p - a).(b - a) / |b - a|^2 -- one
+    // division for both, so a wave with both kinds divides once
+    float num, den;
     if (kind == 1) {
         const float ex = px - x0, ey = py - y0;
-        t = x1 > 0.0f ? sqrtf((ex * ex) + (ey * ey)) / x1 : 0.0f;
+        num = sqrtf((ex * ex) + (ey * ey));
+        den = x1;
     } else {
         const float dx = x1 - x0, dy = y1 - y0;
-        const float den = (dx * dx) + (dy * dy);
-        t = den > 0.0f ? (((px - x0) * dx) + ((py - y0) * dy)) / den : 0.0f;
+        num = ((px - x0) * dx) + ((py - y0) * dy);
+        den = (dx * dx) + (dy * dy);
     }
+    float t = den > 0.0f ? num / den : 0.0f;
     t = cl_min(cl_max(t, 0.0f), 1.0f);
     n = n < 1 ? 1 : (n > 4 ? 4 : n);
     const float o[4] = {off.x, off.y, off.z, off.w};
@@ -812,8 +830,9 @@ __device__ float4 read_color_sub(const SceneDesc &s, uint64_t tag, float4 sub, i
         const float fx = sx - 0.5f, fy = sy - 0.5f;
         const int ix = f2i_floor(fx), iy = f2i_floor(fy);
         const float wx = fx - (float)ix, wy = fy - (float)iy;
-        const float4 t0 = texel(s, u, ix, iy), t1 = texel(s, u, (long long)ix + 1, iy);
-        const float4 t2 = texel(s, u, ix, (long long)iy + 1), t3 = texel(s, u, (long long)ix + 1, (long long)iy + 1);
+        const uint32_t x0 = (uint32_t)ix, y0 = (uint32_t)iy;
+        const float4 t0 = texel(s, u, x0, y0), t1 = texel(s, u, x0 + 1u, y0);
+        const float4 t2 = texel(s, u, x0, y0 + 1u), t3 = texel(s, u, x0 + 1u, y0 + 1u);
         const float ax0 = 1.0f - wx, ay0 = 1.0f - wy;
         float4 r;
         r.x = (((t0.x * ax0) + (t1.x * wx)) * ay0) + (((t2.x * ax0) + (t3.x * wx)) * wy);
@@ -822,15 +841,7 @@ __device__ float4 read_color_sub(const SceneDesc &s, uint64_t tag, float4 sub, i
         r.w = (((t0.w * ax0) + (t1.w * wx)) * ay0) + (((t2.w * ax0) + (t3.w * wx)) * wy);
         return r;
     }
-    const int rx = f2i_rtz(sx);
-    const int ry = f2i_rtz(sy);
-    if (rx >= 0 && ry >= 0 && rx < u.w && ry < u.h) {
-        const uchar4 px = *reinterpret_cast<const uchar4 *>(
-            s.pict + u.mem_offset + 4ll * ((long long)ry * u.w + rx));
-        return make_float4((float)px.x / 255.0f, (float)px.y / 255.0f, (float)px.z / 255.0f,
-                           (float)px.w / 255.0f);
-    }
-    return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return texel(s, u, (uint32_t)f2i_rtz(sx), (uint32_t)f2i_rtz(sy));
 }
 
 __device__ __forceinline__ float4 read_color(const SceneDesc &s, uint64_t tag, int ax, int ay)

@@ -488,6 +488,10 @@ int validate_uses(gudni_ctx *ctx, const void *pict_uses, int32_t n_pict_uses,
         if (u.w < 0 || u.h < 0 || u.mem_offset < 0 || (u.mem_offset & 3) || need > (int64_t)pict_bytes)
             return set_error(ctx, GUDNI_ERR_ARG, "picture usage " + std::to_string(i) +
                                                      " outside the picture data");
+        // the kernels address texels with 32-bit byte offsets
+        if (need > (int64_t)INT32_MAX)
+            return set_error(ctx, GUDNI_ERR_ARG, "picture usage " + std::to_string(i) +
+                                                     " reaches beyond 2 GiB of picture data");
     }
     return GUDNI_OK;
 }

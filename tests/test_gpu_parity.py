@@ -240,6 +240,33 @@ def test_queue_overflow_is_an_error():
         r.close()
 
 
+@pytest.mark.parametrize("filt", [0, 1])   # nearest, bilinear
+def test_texel_bytes_are_exact(filt):
+    """Every byte value of a picture becomes byte / 255 correctly rounded (the
+    kernel multiplies by the reciprocal and corrects with one FMA; the oracle
+    divides): a 16x16 texture holding all 256 values in each channel (alpha
+    included), sampled by a canvas-filling rectangle over an opaque
+    background, gives pre-quantisation colours equal to the oracle's bit for
+    bit."""
+    i = np.arange(256, dtype=np.uint32)
+    tex = np.stack([i, 255 - i, (i * 7) & 255, (i * 13 + 5) & 255], axis=1).astype(np.uint8).reshape(-1)
+    shapes = [(S.picture_tag(0), [S.rectangle_pairs(0.0, 0.0, 24.0, 24.0)])]
+    scene = S.build_scene(24, 24, shapes, [S.picture_substance(0)], tex, [(0.0, 0.0, 16, 16, 0, 1.0)],
+                          background=(0.25, 0.5, 0.75, 1.0), name="texel bytes")
+    spec = G.default_spec(texture_filter=filt)
+    ref = H.oracle_render(scene, spec)
+    assert ref["rc"] == 0
+    r = G.Rasterizer(spec)
+    try:
+        u32, f32 = H.gpu_render_level2(r, scene, spec)
+    finally:
+        r.close()
+    want = np.asarray(ref["f32"], np.float32).reshape(f32.shape)
+    assert np.array_equal(f32.view(np.uint32), want.view(np.uint32)), \
+        np.argwhere(f32.view(np.uint32) != want.view(np.uint32))[:5]
+    assert H.first_mismatch(u32, ref["out"]) is None
+
+
 def test_level1_rejects_malformed_references(rasterizer):
     """A job's shape references are checked on the host before anything runs
     (in parallel chunks for a large job): a substance id outside the table or
