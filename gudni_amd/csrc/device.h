@@ -592,7 +592,6 @@ enum Cross { kCrossRight, kCrossLeft, kCrossDown };
 struct Stack {
     uint64_t s0, s1;
     uint32_t dirty;   // groups of 32 bits touched since their composite was cached
-    uint32_t known;   // picture/gradient colours of this pixel held in the lane's cache, by rank
     __device__ __forceinline__ void flip(uint32_t h)
     {
         const uint32_t b = h & kLabelMask;
@@ -869,8 +868,7 @@ static_assert(sizeof(TileTable::gc) >= sizeof(uint16_t) * kWave * (1 << kRadixMa
 // alpha is exactly 1 every later layer adds its colour times 0, so the walk
 // stops there with the same result.
 __device__ __forceinline__ float4 group_composite(const SceneDesc &s, const TileTable *tt,
-                                                  uint32_t word, int g, int ax, int ay, int32_t *cache,
-                                                  uint32_t &known)
+                                                  uint32_t word, int g, int ax, int ay)
 {
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
     while (word) {
@@ -882,29 +880,7 @@ __device__ __forceinline__ float4 group_composite(const SceneDesc &s, const Tile
         if ((tag & kTagTypeMask) == kTagSolid) {
             pm = tt->color[n];
         } else {
-            // a picture or gradient colour depends on the shape and the
-            // pixel only: the tile's first 32 such shapes (by rank) keep
-            // this pixel's colour in the lane's cache
-            const uint64_t pm0 = tt->pic0, pm1 = tt->pic1;
-            const uint32_t rank =
-                n < 64 ? (uint32_t)__popcll(pm0 & ((1ull << n) - 1ull))
-                       : (uint32_t)(__popcll(pm0) + __popcll(pm1 & ((1ull << (n - 64)) - 1ull)));
-            float4 c;
-            if (rank < 32u && ((known >> rank) & 1u)) {
-                c = make_float4(__int_as_float(cache[(4 * rank + 0) * kWave]),
-                                __int_as_float(cache[(4 * rank + 1) * kWave]),
-                                __int_as_float(cache[(4 * rank + 2) * kWave]),
-                                __int_as_float(cache[(4 * rank + 3) * kWave]));
-            } else {
-                c = read_color_sub(s, tag, tt->color[n], ax, ay);
-                if (rank < 32u) {
-                    cache[(4 * rank + 0) * kWave] = __float_as_int(c.x);
-                    cache[(4 * rank + 1) * kWave] = __float_as_int(c.y);
-                    cache[(4 * rank + 2) * kWave] = __float_as_int(c.z);
-                    cache[(4 * rank + 3) * kWave] = __float_as_int(c.w);
-                    known |= 1u << rank;
-                }
-            }
+            const float4 c = read_color_sub(s, tag, tt->color[n], ax, ay);
             pm = make_float4(c.x * c.w, c.y * c.w, c.z * c.w, c.w);
         }
         const float t = 1.0f - acc.w;
@@ -920,11 +896,9 @@ __device__ __forceinline__ float4 group_composite(const SceneDesc &s, const Tile
 // Accurate-mode colour of a grouped tile: over(group 3 .. group 0,
 // background), recomputing only the dirty groups' cached composites.  Groups
 // under an alpha of exactly 1 add nothing (their colour times 0): they are
-// neither read nor recomputed, and stay dirty.  cache: the lane's shape
-// scratch, whose first 128 words (unused by a compact tile) hold this
-// pixel's picture and gradient colours (st.known: which of them).
+// neither read nor recomputed, and stay dirty.
 __device__ float4 grouped_color(const SceneDesc &s, TileTable *tt, Stack &st, int lane, int ax,
-                                int ay, int32_t *cache)
+                                int ay)
 {
     const uint32_t words[4] = {(uint32_t)st.s0, (uint32_t)(st.s0 >> 32), (uint32_t)st.s1,
                                (uint32_t)(st.s1 >> 32)};
@@ -935,7 +909,7 @@ __device__ float4 grouped_color(const SceneDesc &s, TileTable *tt, Stack &st, in
         if (A == 1.0f) break;
         float4 gcol;
         if (st.dirty & (1u << g)) {
-            gcol = group_composite(s, tt, words[g], g, ax, ay, cache, st.known);
+            gcol = group_composite(s, tt, words[g], g, ax, ay);
             tt->gc[g][0][lane] = gcol.x;
             tt->gc[g][1][lane] = gcol.y;
             tt->gc[g][2][lane] = gcol.z;
@@ -1094,7 +1068,6 @@ __device__ uint32_t render(const Frame &f, const SceneDesc &s, Queue &q, const C
     if (!(p.pixel_y <= c.float_height)) return 0;
     float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float area_sum = 0.0f;
-    st.known = 0u;
     if (grouped) {
         // picture colours depend on the pixel: their groups are stale
         const uint64_t p0 = st.s0 & tt->pic0, p1 = st.s1 & tt->pic1;
@@ -1131,7 +1104,7 @@ __device__ uint32_t render(const Frame &f, const SceneDesc &s, Queue &q, const C
             p.sx1 = nx;
             float4 col;
             if (grouped) {
-                col = grouped_color(s, tt, st, lane, c.dx, ay, sidx);
+                col = grouped_color(s, tt, st, lane, c.dx, ay);
             } else {
                 bool positional = false;
                 col = determine_color<kCompact>(f, s, st, sidx, tt, c.dx, ay, positional);
@@ -1165,7 +1138,6 @@ __device__ uint32_t render(const Frame &f, const SceneDesc &s, Queue &q, const C
             if (!(p.pixel_y <= c.float_height)) break;
             acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             area_sum = 0.0f;
-            st.known = 0u;
             if (grouped) {
                 const uint64_t p0 = st.s0 & tt->pic0, p1 = st.s1 & tt->pic1;
                 st.dirty |= ((uint32_t)p0 ? 1u : 0u) | ((p0 >> 32) ? 2u : 0u) |
