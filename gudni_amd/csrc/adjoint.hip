@@ -93,6 +93,11 @@ struct AdjFrame {
     const AdjDesc *scenes;
     float4 *q_orig;               // [slots][2 cap][64]: per generated line k, row 2k holds
                                   // (A.y, B.y, sum w, sum w u), row 2k+1 (dL/dA.y, dL/dB.y, 0, 0)
+    // rings the forward sweep runs on ([slots][64][cap], as the threshold
+    // rings), so that the sorted records stay for the backward sweep; null:
+    // the backward sweep regenerates them
+    float4 *copy_thr;
+    uint32_t *copy_hdr;
 };
 
 enum Mode { kRecord = 0, kApply = 1 };
@@ -790,9 +795,9 @@ __device__ __forceinline__ void wave_sync()
 }
 
 template <bool kCompact>
-__device__ void adj_unit(const Frame &f, const AdjDesc &d, int lane, uint32_t unit, Queue &q, int32_t *sidx,
-                         TileTable *tt, float4 *straight, float4 *dcol, float4 *orig, uint32_t &status,
-                         float &loss)
+__device__ void adj_unit(const Frame &f, const AdjDesc &d, int lane, uint32_t unit, Queue &q, Queue &fq,
+                         int32_t *sidx, TileTable *tt, float4 *straight, float4 *dcol, float4 *orig,
+                         uint32_t &status, float &loss)
 {
     const SceneDesc &s = d.s;
     const int tile = (int)(unit >> 2), quarter = (int)(unit & 3u);
@@ -837,17 +842,35 @@ __device__ void adj_unit(const Frame &f, const AdjDesc &d, int lane, uint32_t un
     bool ok = active && !q.spilled && !q.overflow;
     if (ok) radix_sort_queue(q, hist, s.sort_bits, c.float_height);
     stamp(1);
-    if (ok) status = sweep_forward<kCompact>(f, d, q, c, Stack{s0, s1, 0u}, sidx, tt, loss);
+    // the forward sweep splits and pops records: with a ring of its own it
+    // runs on a copy of the sorted queue and leaves q as the backward sweep
+    // needs it (the same records a regeneration would give, at the cost of
+    // one copy)
+    const bool copied = fq.thr != q.thr;
+    if (copied) {
+        fq.head = 0;
+        fq.len = 0;
+        fq.spilled = false;
+        fq.overflow = false;
+        if (ok) {
+            for (int i = 0; i < q.len; ++i) fq.set(i, q.get_h(i), q.get(i));
+            fq.len = q.len;
+        }
+    }
+    Queue &sq = copied ? fq : q;
+    if (ok) status = sweep_forward<kCompact>(f, d, sq, c, Stack{s0, s1, 0u}, sidx, tt, loss);
     stamp(2);
-    if (active && q.overflow) status |= kStatusOverflow;
-    if (__any(q.spilled) || __any(status != 0)) return;
-    // B: recompute the records, backward sweep
-    q.head = 0; q.len = 0;
-    bits = 0; s0 = 0; s1 = 0;
-    generate_o<kCompact, kRecord>(f, d, q, c, active, lane, span0, span1, sidx, bits, s0, s1, orig);
+    if (active && (q.overflow || sq.overflow)) status |= kStatusOverflow;
+    if (__any(q.spilled || sq.spilled) || __any(status != 0)) return;
+    // B: the sorted records again (kept, or recomputed), backward sweep
+    if (!copied) {
+        q.head = 0; q.len = 0;
+        bits = 0; s0 = 0; s1 = 0;
+        generate_o<kCompact, kRecord>(f, d, q, c, active, lane, span0, span1, sidx, bits, s0, s1, orig);
+    }
     stamp(3);
     ok = active && !q.spilled && !q.overflow;
-    if (ok) radix_sort_queue(q, hist, s.sort_bits, c.float_height);
+    if (ok && !copied) radix_sort_queue(q, hist, s.sort_bits, c.float_height);
     stamp(4);
     if (ok) status = sweep_backward<kCompact>(f, d, q, c, Stack{s0, s1, 0u}, sidx, tt, straight, dcol, orig);
     stamp(5);
@@ -890,6 +913,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) a
     float4 *q_thr = f.q_thr + ((size_t)slot * kWave + lane) * cap;
     uint32_t *q_hdr = f.q_hdr + ((size_t)slot * kWave + lane) * cap;
     float4 *orig = a.q_orig + (size_t)slot * 2 * cap * kWave + lane;
+    float4 *c_thr = a.copy_thr ? a.copy_thr + ((size_t)slot * kWave + lane) * cap : q_thr;
+    uint32_t *c_hdr = a.copy_hdr ? a.copy_hdr + ((size_t)slot * kWave + lane) * cap : q_hdr;
     int32_t *sidx = f.shape_idx + (size_t)slot * kSidxWords * kWave + lane;
     TileTable *tt = &tables[wave];
     float4 *dcol = dcols[wave];
@@ -919,8 +944,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) a
         q.spilled = false; q.overflow = false;
         uint32_t status = 0;
         float loss = 0.0f;
-        if (count <= f.max_shapes) adj_unit<true>(f, d, lane, unit, q, sidx, tt, straight, dcol, orig, status, loss);
-        else adj_unit<false>(f, d, lane, unit, q, sidx, tt, straight, dcol, orig, status, loss);
+        Queue fq = q;
+        fq.thr = c_thr; fq.hdr = c_hdr;
+        if (count <= f.max_shapes)
+            adj_unit<true>(f, d, lane, unit, q, fq, sidx, tt, straight, dcol, orig, status, loss);
+        else
+            adj_unit<false>(f, d, lane, unit, q, fq, sidx, tt, straight, dcol, orig, status, loss);
         if (status) atomicOr(f.status, status);
         const bool spilled = __any(q.spilled);
         if (spilled) {
@@ -1003,7 +1032,7 @@ struct gudni_adjoint_batch {
     int device = 0;
     std::vector<Scene> scenes;
     DevBuf geo, subs, pict, uses, shapes, bounds, curves, points, target, image, gpix, d_points, d_rgba,
-        loss, tiles, tile_job, tile_scene, work, descs, orig_fast, orig_slow, grads;
+        loss, tiles, tile_job, tile_scene, work, descs, orig_fast, orig_slow, grads, copy_thr, copy_hdr;
     std::vector<DevScene> dev;              // empty until the first device rebuild
     std::vector<size_t> geo_off, refs_off, curves_off, pict_off, uses_off, grads_off;
     std::vector<size_t> points_off, subs_off, pix_off;
@@ -1150,6 +1179,8 @@ int build(gudni_ctx *ctx, gudni_adjoint_batch *a)
     size_t scap = 1;
     while ((int)scap < ctx->spec.max_thresholds) scap <<= 1;
     CHECK_HIP(ctx, a->orig_fast.ensure(fast_slots * ((size_t)1 << dev::kFastCapLog2) * 2 * 64 * 16));
+    CHECK_HIP(ctx, a->copy_thr.ensure(fast_slots * ((size_t)1 << dev::kFastCapLog2) * 64 * 16));
+    CHECK_HIP(ctx, a->copy_hdr.ensure(fast_slots * ((size_t)1 << dev::kFastCapLog2) * 64 * 4));
     CHECK_HIP(ctx, a->orig_slow.ensure(slow_slots * scap * 2 * 64 * 16));
     CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return GUDNI_OK;
@@ -1438,6 +1469,8 @@ int launch(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t work_begin, int32_t n
     af.f.shape_idx = (int32_t *)ctx->fast_sidx.p;
     af.f.cap_log2 = dev::kFastCapLog2;
     af.q_orig = (float4 *)a->orig_fast.p;
+    af.copy_thr = (float4 *)a->copy_thr.p;
+    af.copy_hdr = (uint32_t *)a->copy_hdr.p;
     const int grid = std::max(1, std::min(a->blocks, (n_work + 3) / 4));
     hipLaunchKernelGGL(adj::adjoint_pass, dim3(grid), dim3(256), 0, st, af, 0);
     CHECK_HIP(ctx, hipGetLastError());
@@ -1450,6 +1483,8 @@ int launch(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t work_begin, int32_t n
     af.f.shape_idx = (int32_t *)ctx->slow_sidx.p;
     af.f.cap_log2 = slog;
     af.q_orig = (float4 *)a->orig_slow.p;
+    af.copy_thr = nullptr;   // the retry pass's rings are large: it regenerates instead
+    af.copy_hdr = nullptr;
     hipLaunchKernelGGL(adj::adjoint_pass, dim3(ctx->slow_blocks), dim3(256), 0, st, af, 1);
     CHECK_HIP(ctx, hipGetLastError());
     return GUDNI_OK;
@@ -1639,7 +1674,7 @@ void gudni_adjoint_free(gudni_adjoint_batch *a)
     DevBuf *bufs[] = {&a->geo, &a->subs, &a->pict, &a->uses, &a->shapes, &a->bounds, &a->curves,
                       &a->points, &a->target, &a->image, &a->gpix, &a->d_points, &a->d_rgba, &a->loss,
                       &a->tiles, &a->tile_job, &a->tile_scene, &a->work, &a->descs, &a->orig_fast,
-                      &a->orig_slow, &a->grads};
+                      &a->orig_slow, &a->grads, &a->copy_thr, &a->copy_hdr};
     for (DevBuf *b : bufs) b->release();
     a->drop_device_scenes();
     delete a;
