@@ -98,6 +98,10 @@ struct AdjFrame {
     // the backward sweep regenerates them
     float4 *copy_thr;
     uint32_t *copy_hdr;
+    // [slots][4 cap][64]: per generated line k, rows 4k..4k+3 record what
+    // maps its weights to outline points (kk, and per end the curve, kind,
+    // ta, tb and the chord's slope and position); null: phase C regenerates
+    float4 *q_ends;
 };
 
 enum Mode { kRecord = 0, kApply = 1 };
@@ -240,9 +244,19 @@ __device__ void apply_end(const AdjDesc &d, const EndSrc &e, const Trav &tv, flo
 // addLineSegment + addThreshold with the line's origin: kRecord pushes the
 // record with origin bits and notes (A.y, 1/(B.y-A.y)); kApply pushes
 // nothing and maps the line's accumulated weights to its end points.
+// The chord of a kind-1 end (a column-border intercept of P(ta)P(tb)):
+// its slope dy/dx and the border's position v along it.
+__device__ __forceinline__ float2 chord_terms(const EndSrc &e, const Trav &tv)
+{
+    if (e.kind != 1) return make_float2(0.0f, 0.0f);
+    const float2 pa = bez(tv, e.ta), pb = bez(tv, e.tb);
+    const float dxab = pb.x - pa.x;
+    return make_float2((pb.y - pa.y) / dxab, (e.X - pa.x) / dxab);   // (sl, v)
+}
+
 template <int kMode>
 __device__ void add_line_o(Queue &q, const Column &c, float2 l, float2 r, uint32_t bit, bool &added,
-                           bool &enclosed, float4 *orig, uint32_t &n_orig, const AdjDesc &d,
+                           bool &enclosed, float4 *orig, float4 *ends, uint32_t &n_orig, const AdjDesc &d,
                            const EndSrc &el, const Trav &tl, const EndSrc &er, const Trav &tr)
 {
     Thr t{cl_min(l.y, r.y), cl_max(l.y, r.y), l.x, r.x};
@@ -270,6 +284,14 @@ __device__ void add_line_o(Queue &q, const Column &c, float2 l, float2 r, uint32
             q.set(0, h | (k << kOriginShift), t);
             orig[2 * k * kWave] = make_float4(l.y, r.y, 0.0f, 0.0f);
             orig[(2 * k + 1) * kWave] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (ends) {
+                const float kk = r.y != l.y ? (r.x - l.x) / (r.y - l.y) : 0.0f;
+                const float2 cl = chord_terms(el, tl), cr = chord_terms(er, tr);
+                ends[4 * k * kWave] = make_float4(kk, __int_as_float(el.curve), __int_as_float(el.kind), el.ta);
+                ends[(4 * k + 1) * kWave] = make_float4(el.tb, cl.x, cl.y, __int_as_float(er.curve));
+                ends[(4 * k + 2) * kWave] = make_float4(__int_as_float(er.kind), er.ta, er.tb, cr.x);
+                ends[(4 * k + 3) * kWave] = make_float4(cr.y, 0.0f, 0.0f, 0.0f);
+            }
         }
     } else {
         const float4 o = orig[2 * k * kWave];
@@ -285,7 +307,7 @@ __device__ void add_line_o(Queue &q, const Column &c, float2 l, float2 r, uint32
 // spawnThresholds with end point origins.
 template <int kMode>
 __device__ void spawn_o(Queue &q, const Column &c, uint32_t bit, const Trav &l, const Trav &r, int lcur,
-                        int rcur, bool &added, bool &enclosed, float4 *orig, uint32_t &n_orig,
+                        int rcur, bool &added, bool &enclosed, float4 *orig, float4 *ends, uint32_t &n_orig,
                         const AdjDesc &d)
 {
     float yl;
@@ -304,7 +326,7 @@ __device__ void spawn_o(Queue &q, const Column &c, uint32_t bit, const Trav &l, 
     EndSrc e4{rcur, 0, 1.0f, 0.0f, 0.0f};
     bool lw = false, rw = false;
     if ((l.right.x < 1.0f) && (l.right.x > 0.0f)) {
-        add_line_o<kMode>(q, c, make_float2(l.xpos, yl), l.right, bit, added, enclosed, orig, n_orig, d,
+        add_line_o<kMode>(q, c, make_float2(l.xpos, yl), l.right, bit, added, enclosed, orig, ends, n_orig, d,
                           e1, l, e2, l);
         lw = true;
     }
@@ -317,7 +339,7 @@ __device__ void spawn_o(Queue &q, const Column &c, uint32_t bit, const Trav &l, 
         else e4 = EndSrc{rcur, 1, k.ta, k.tb, r.xpos};
     }
     if ((r.left.x > 0.0f) && (r.left.x < 1.0f) && (l.index != r.index)) {
-        add_line_o<kMode>(q, c, r.left, make_float2(r.xpos, yr), bit, added, enclosed, orig, n_orig, d,
+        add_line_o<kMode>(q, c, r.left, make_float2(r.xpos, yr), bit, added, enclosed, orig, ends, n_orig, d,
                           e3, r, e4, r);
         rw = true;
     }
@@ -326,7 +348,7 @@ __device__ void spawn_o(Queue &q, const Column &c, uint32_t bit, const Trav &l, 
         const bool br_left = rw || (r.left.x == r.right.x);
         const float2 bl = bl_right ? l.right : make_float2(l.xpos, yl);
         const float2 br = br_left ? r.left : make_float2(r.xpos, yr);
-        add_line_o<kMode>(q, c, bl, br, bit, added, enclosed, orig, n_orig, d, bl_right ? e2 : e1, l,
+        add_line_o<kMode>(q, c, bl, br, bit, added, enclosed, orig, ends, n_orig, d, bl_right ? e2 : e1, l,
                           br_left ? e3 : e4, r);
     }
 }
@@ -336,7 +358,8 @@ __device__ void spawn_o(Queue &q, const Column &c, uint32_t bit, const Trav &l, 
 template <bool kCompact, int kMode>
 __device__ void generate_o(const Frame &f, const AdjDesc &d, Queue &q, const Column &c, bool active,
                            int lane, float span0, float span1, int32_t *sidx, uint32_t &shape_bits,
-                           uint64_t &stack0, uint64_t &stack1, float4 *orig)
+                           uint64_t &stack0, uint64_t &stack1, float4 *orig, float4 *ends = nullptr,
+                           uint32_t *lines = nullptr)
 {
     const SceneDesc &s = d.s;
     const float dx = (float)c.dx, dy = (float)c.dy;
@@ -393,7 +416,8 @@ __device__ void generate_o(const Frame &f, const AdjDesc &d, Queue &q, const Col
                     const int tree_size = ((int)size - 4) / 2;
                     const int lcur = curve_base + search_curve(l, strand + 32, tree_size, dx, dy, true);
                     const int rcur = curve_base + search_curve(r, strand + 32, tree_size, dx, dy, false);
-                    spawn_o<kMode>(q, c, label, l, r, lcur, rcur, added, enclosed_strand, orig, n_orig, d);
+                    spawn_o<kMode>(q, c, label, l, r, lcur, rcur, added, enclosed_strand, orig, ends, n_orig,
+                                   d);
                 }
                 offset += 8u * size;
                 enclosed_shape = enclosed_shape != enclosed_strand;
@@ -407,6 +431,42 @@ __device__ void generate_o(const Frame &f, const AdjDesc &d, Queue &q, const Col
                 shape_bits += 1u;
             }
         }
+    }
+    if (lines) *lines = n_orig;
+}
+
+// Gradient (gx, gy) on a recorded line end -> outline points (apply_end with
+// the chord terms recorded at generation).
+__device__ __forceinline__ void apply_recorded_end(const AdjDesc &d, int curve, int kind, float ta, float tb,
+                                                   float sl, float v, float gx, float gy)
+{
+    if (kind == 0) {
+        chain_point(d, curve, ta, make_float2(gx, gy));
+    } else if (kind == 2) {
+        chain_point(d, curve, ta, make_float2(0.0f, gy));
+    } else {
+        chain_point(d, curve, ta, make_float2(-gy * sl * (1.0f - v), gy * (1.0f - v)));
+        chain_point(d, curve, tb, make_float2(-gy * sl * v, gy * v));
+    }
+}
+
+// Phase C from the recorded ends: every generated line's accumulated
+// weights -> its end points -> outline points, in generation order, as the
+// regenerating walk (generate_o<kApply>) applies them.
+__device__ void apply_lines(const AdjDesc &d, const float4 *orig, const float4 *ends, uint32_t n_lines)
+{
+    for (uint32_t k = 0; k < n_lines; ++k) {
+        const float4 o = orig[2 * k * kWave];
+        const float4 e = orig[(2 * k + 1) * kWave];
+        if (o.z == 0.0f && o.w == 0.0f && e.x == 0.0f && e.y == 0.0f) continue;
+        const float4 r0 = ends[4 * k * kWave], r1 = ends[(4 * k + 1) * kWave];
+        const float4 r2 = ends[(4 * k + 2) * kWave], r3 = ends[(4 * k + 3) * kWave];
+        const float kk = r0.x;
+        const float gbx = o.w, gax = o.z - o.w;
+        apply_recorded_end(d, __float_as_int(r0.y), __float_as_int(r0.z), r0.w, r1.x, r1.y, r1.z, gax,
+                           -kk * gax + e.x);
+        apply_recorded_end(d, __float_as_int(r1.w), __float_as_int(r2.x), r2.y, r2.z, r2.w, r3.x, gbx,
+                           -kk * gbx + e.y);
     }
 }
 
@@ -797,7 +857,7 @@ __device__ __forceinline__ void wave_sync()
 template <bool kCompact>
 __device__ void adj_unit(const Frame &f, const AdjDesc &d, int lane, uint32_t unit, Queue &q, Queue &fq,
                          int32_t *sidx, TileTable *tt, float4 *straight, float4 *dcol, float4 *orig,
-                         uint32_t &status, float &loss)
+                         float4 *ends, uint32_t &status, float &loss)
 {
     const SceneDesc &s = d.s;
     const int tile = (int)(unit >> 2), quarter = (int)(unit & 3u);
@@ -835,9 +895,10 @@ __device__ void adj_unit(const Frame &f, const AdjDesc &d, int lane, uint32_t un
     };
     // A: forward
     // every generate_o call is made by the whole wave (it ballots over shapes)
-    uint32_t bits = 0;
+    uint32_t bits = 0, n_lines = 0;
     uint64_t s0 = 0, s1 = 0;
-    generate_o<kCompact, kRecord>(f, d, q, c, active, lane, span0, span1, sidx, bits, s0, s1, orig);
+    generate_o<kCompact, kRecord>(f, d, q, c, active, lane, span0, span1, sidx, bits, s0, s1, orig, ends,
+                                  &n_lines);
     stamp(0);
     bool ok = active && !q.spilled && !q.overflow;
     if (ok) radix_sort_queue(q, hist, s.sort_bits, c.float_height);
@@ -866,7 +927,8 @@ __device__ void adj_unit(const Frame &f, const AdjDesc &d, int lane, uint32_t un
     if (!copied) {
         q.head = 0; q.len = 0;
         bits = 0; s0 = 0; s1 = 0;
-        generate_o<kCompact, kRecord>(f, d, q, c, active, lane, span0, span1, sidx, bits, s0, s1, orig);
+        generate_o<kCompact, kRecord>(f, d, q, c, active, lane, span0, span1, sidx, bits, s0, s1, orig, ends,
+                                      &n_lines);
     }
     stamp(3);
     ok = active && !q.spilled && !q.overflow;
@@ -874,8 +936,11 @@ __device__ void adj_unit(const Frame &f, const AdjDesc &d, int lane, uint32_t un
     stamp(4);
     if (ok) status = sweep_backward<kCompact>(f, d, q, c, Stack{s0, s1, 0u}, sidx, tt, straight, dcol, orig);
     stamp(5);
-    // C: line weights -> outline points
-    {
+    // C: line weights -> outline points, from the ends recorded in A (or
+    // by walking the generation once more)
+    if (ends) {
+        if (ok && status == 0) apply_lines(d, orig, ends, n_lines);
+    } else {
         uint32_t bits2 = 0;
         uint64_t a0 = 0, a1 = 0;
         generate_o<kCompact, kApply>(f, d, q, c, ok && status == 0, lane, span0, span1, sidx, bits2, a0,
@@ -913,6 +978,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) a
     float4 *q_thr = f.q_thr + ((size_t)slot * kWave + lane) * cap;
     uint32_t *q_hdr = f.q_hdr + ((size_t)slot * kWave + lane) * cap;
     float4 *orig = a.q_orig + (size_t)slot * 2 * cap * kWave + lane;
+    float4 *ends = a.q_ends ? a.q_ends + (size_t)slot * 4 * cap * kWave + lane : nullptr;
     float4 *c_thr = a.copy_thr ? a.copy_thr + ((size_t)slot * kWave + lane) * cap : q_thr;
     uint32_t *c_hdr = a.copy_hdr ? a.copy_hdr + ((size_t)slot * kWave + lane) * cap : q_hdr;
     int32_t *sidx = f.shape_idx + (size_t)slot * kSidxWords * kWave + lane;
@@ -947,9 +1013,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) a
         Queue fq = q;
         fq.thr = c_thr; fq.hdr = c_hdr;
         if (count <= f.max_shapes)
-            adj_unit<true>(f, d, lane, unit, q, fq, sidx, tt, straight, dcol, orig, status, loss);
+            adj_unit<true>(f, d, lane, unit, q, fq, sidx, tt, straight, dcol, orig, ends, status, loss);
         else
-            adj_unit<false>(f, d, lane, unit, q, fq, sidx, tt, straight, dcol, orig, status, loss);
+            adj_unit<false>(f, d, lane, unit, q, fq, sidx, tt, straight, dcol, orig, ends, status, loss);
         if (status) atomicOr(f.status, status);
         const bool spilled = __any(q.spilled);
         if (spilled) {
@@ -1032,7 +1098,8 @@ struct gudni_adjoint_batch {
     int device = 0;
     std::vector<Scene> scenes;
     DevBuf geo, subs, pict, uses, shapes, bounds, curves, points, target, image, gpix, d_points, d_rgba,
-        loss, tiles, tile_job, tile_scene, work, descs, orig_fast, orig_slow, grads, copy_thr, copy_hdr;
+        loss, tiles, tile_job, tile_scene, work, descs, orig_fast, orig_slow, grads, copy_thr, copy_hdr,
+        ends_fast;
     std::vector<DevScene> dev;              // empty until the first device rebuild
     std::vector<size_t> geo_off, refs_off, curves_off, pict_off, uses_off, grads_off;
     std::vector<size_t> points_off, subs_off, pix_off;
@@ -1181,6 +1248,7 @@ int build(gudni_ctx *ctx, gudni_adjoint_batch *a)
     CHECK_HIP(ctx, a->orig_fast.ensure(fast_slots * ((size_t)1 << dev::kFastCapLog2) * 2 * 64 * 16));
     CHECK_HIP(ctx, a->copy_thr.ensure(fast_slots * ((size_t)1 << dev::kFastCapLog2) * 64 * 16));
     CHECK_HIP(ctx, a->copy_hdr.ensure(fast_slots * ((size_t)1 << dev::kFastCapLog2) * 64 * 4));
+    CHECK_HIP(ctx, a->ends_fast.ensure(fast_slots * ((size_t)1 << dev::kFastCapLog2) * 4 * 64 * 16));
     CHECK_HIP(ctx, a->orig_slow.ensure(slow_slots * scap * 2 * 64 * 16));
     CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return GUDNI_OK;
@@ -1471,6 +1539,7 @@ int launch(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t work_begin, int32_t n
     af.q_orig = (float4 *)a->orig_fast.p;
     af.copy_thr = (float4 *)a->copy_thr.p;
     af.copy_hdr = (uint32_t *)a->copy_hdr.p;
+    af.q_ends = (float4 *)a->ends_fast.p;
     const int grid = std::max(1, std::min(a->blocks, (n_work + 3) / 4));
     hipLaunchKernelGGL(adj::adjoint_pass, dim3(grid), dim3(256), 0, st, af, 0);
     CHECK_HIP(ctx, hipGetLastError());
@@ -1485,6 +1554,7 @@ int launch(gudni_ctx *ctx, gudni_adjoint_batch *a, int32_t work_begin, int32_t n
     af.q_orig = (float4 *)a->orig_slow.p;
     af.copy_thr = nullptr;   // the retry pass's rings are large: it regenerates instead
     af.copy_hdr = nullptr;
+    af.q_ends = nullptr;
     hipLaunchKernelGGL(adj::adjoint_pass, dim3(ctx->slow_blocks), dim3(256), 0, st, af, 1);
     CHECK_HIP(ctx, hipGetLastError());
     return GUDNI_OK;
@@ -1674,7 +1744,7 @@ void gudni_adjoint_free(gudni_adjoint_batch *a)
     DevBuf *bufs[] = {&a->geo, &a->subs, &a->pict, &a->uses, &a->shapes, &a->bounds, &a->curves,
                       &a->points, &a->target, &a->image, &a->gpix, &a->d_points, &a->d_rgba, &a->loss,
                       &a->tiles, &a->tile_job, &a->tile_scene, &a->work, &a->descs, &a->orig_fast,
-                      &a->orig_slow, &a->grads, &a->copy_thr, &a->copy_hdr};
+                      &a->orig_slow, &a->grads, &a->copy_thr, &a->copy_hdr, &a->ends_fast};
     for (DevBuf *b : bufs) b->release();
     a->drop_device_scenes();
     delete a;
