@@ -475,11 +475,13 @@ __device__ void apply_lines(const AdjDesc &d, const float4 *orig, const float4 *
 // the tile's LDS copy.
 template <bool kCompact>
 __device__ __forceinline__ float4 label_color(const SceneDesc &s, const int32_t *sidx, const TileTable *tt,
-                                              const float4 *straight, int b, int ax, int ay, uint64_t &tag)
+                                              const float4 *straight, int b, int ax, int ay, uint64_t &tag,
+                                              LayerCache &lc)
 {
     if (kCompact) {
         tag = tt->tag[b];
-        return read_color_sub(s, tag, straight[b], ax, ay);
+        if ((tag & kTagTypeMask) == kTagSolid) return straight[b];
+        return lc.color(s, tt, b, tag, straight[b], ax, ay);
     }
     tag = s.shapes[sidx[b * kWave]].tag;
     return read_color(s, tag, ax, ay);
@@ -528,7 +530,7 @@ __device__ __forceinline__ uint64_t label_tag(const SceneDesc &s, const int32_t 
 template <bool kCompact>
 __device__ void color_grad(const SceneDesc &s, const Stack &st, const int32_t *sidx, const TileTable *tt,
                            const float4 *straight, float4 *dcol, float *d_rgba, int ax, int ay, float4 S,
-                           float4 C, float4 gc)
+                           float4 C, float4 gc, LayerCache &lc)
 {
     if (!(S.w > 0.0f)) return;
     const float gPx = gc.x / S.w, gPy = gc.y / S.w, gPz = gc.z / S.w;
@@ -543,7 +545,7 @@ __device__ void color_grad(const SceneDesc &s, const Stack &st, const int32_t *s
         else break;
         if (!mw.visit(label_tag(s, sidx, tt, kCompact, b))) continue;
         uint64_t tag;
-        const float4 col = label_color<kCompact>(s, sidx, tt, straight, b, ax, ay, tag);
+        const float4 col = label_color<kCompact>(s, sidx, tt, straight, b, ax, ay, tag, lc);
         const float T = 1.0f - F.w;
         const float4 pm = make_float4(col.x * col.w, col.y * col.w, col.z * col.w, col.w);
         float4 Fn;
@@ -569,7 +571,7 @@ __device__ void color_grad(const SceneDesc &s, const Stack &st, const int32_t *s
                     else { bb = 63 - __clzll((long long)u0); u0 &= ~(1ull << bb); }
                     if (!mu.visit(label_tag(s, sidx, tt, kCompact, bb))) continue;
                     uint64_t tag2;
-                    const float4 c2 = label_color<kCompact>(s, sidx, tt, straight, bb, ax, ay, tag2);
+                    const float4 c2 = label_color<kCompact>(s, sidx, tt, straight, bb, ax, ay, tag2, lc);
                     const float t2 = 1.0f - U.w;
                     U.x = U.x + c2.x * c2.w * t2;
                     U.y = U.y + c2.y * c2.w * t2;
@@ -662,6 +664,8 @@ __device__ uint32_t sweep_forward(const Frame &f, const AdjDesc &d, Queue &q, co
     p.rnd = 0;
     long guard = kSectionGuard;
     int ay = c.dy;
+    // picture and gradient colours of the current pixel (compact tiles)
+    LayerCache lc{const_cast<int32_t *>(sidx), 0u};
     // one loop over sections and pixel ends, as render's
     p.pixel_y = 1.0f;
     if (!(p.pixel_y <= c.float_height)) return 0;
@@ -680,7 +684,8 @@ __device__ uint32_t sweep_forward(const Frame &f, const AdjDesc &d, Queue &q, co
             p.sx0 = p.sx1;
             p.sx1 = nx;
             bool positional = false;
-            const float4 col = determine_color<kCompact>(f, s, st, sidx, tt, c.dx, ay, positional);
+            const float4 col = determine_color<kCompact>(f, s, st, sidx, tt, c.dx, ay, positional,
+                                                         kCompact ? &lc : nullptr);
             const float area = (p.sx1 - p.sx0) * (p.sy1 - p.sy0);
             acc.x += col.x * area;
             acc.y += col.y * area;
@@ -706,6 +711,7 @@ __device__ uint32_t sweep_forward(const Frame &f, const AdjDesc &d, Queue &q, co
             if (!(p.pixel_y <= c.float_height)) break;
             acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             area_sum = 0.0f;
+            lc.known = 0u;
         }
     }
     return 0;
@@ -724,6 +730,7 @@ __device__ uint32_t sweep_backward(const Frame &f, const AdjDesc &d, Queue &q, c
     p.rnd = 0;
     long guard = kSectionGuard;
     int ay = c.dy;
+    LayerCache lc{const_cast<int32_t *>(sidx), 0u};
     // one loop over sections and pixel ends, as render's; the span state
     // below is reset at each pixel's start
     p.pixel_y = 1.0f;
@@ -770,7 +777,8 @@ __device__ uint32_t sweep_backward(const Frame &f, const AdjDesc &d, Queue &q, c
             p.sx0 = p.sx1;
             p.sx1 = nx;
             bool positional = false;
-            const float4 col = determine_color<kCompact>(f, s, st, sidx, tt, c.dx, ay, positional);
+            const float4 col = determine_color<kCompact>(f, s, st, sidx, tt, c.dx, ay, positional,
+                                                         kCompact ? &lc : nullptr);
             const float dy = p.sy1 - p.sy0;
             const float area = (p.sx1 - p.sx0) * dy;
             {
@@ -807,7 +815,7 @@ __device__ uint32_t sweep_backward(const Frame &f, const AdjDesc &d, Queue &q, c
             const float4 gc = make_float4(G.x * area, G.y * area, G.z * area, G.w * area);
             if (gc.x != 0.0f || gc.y != 0.0f || gc.z != 0.0f || gc.w != 0.0f) {
                 const float4 S = make_float4(col.x * col.w, col.y * col.w, col.z * col.w, col.w);
-                color_grad<kCompact>(s, st, sidx, tt, straight, dcol, d.d_rgba, c.dx, ay, S, col, gc);
+                color_grad<kCompact>(s, st, sidx, tt, straight, dcol, d.d_rgba, c.dx, ay, S, col, gc, lc);
             }
             if (p.current < p.num_active) {
                 const uint32_t h = q.get_h(p.current);
@@ -828,6 +836,7 @@ __device__ uint32_t sweep_backward(const Frame &f, const AdjDesc &d, Queue &q, c
             p.pixel_y += 1.0f;
             if (!(p.pixel_y <= c.float_height)) break;
             G = d.gpix[(long long)ay * s.width + c.dx];
+            lc.known = 0u;
             have_prev = false;
             prev_inside = false;
             prev_origin = 0;
@@ -876,6 +885,21 @@ __device__ void adj_unit(const Frame &f, const AdjDesc &d, int lane, uint32_t un
                 col = make_float4(col.x * col.w, col.y * col.w, col.z * col.w, col.w);
             tt->color[n] = col;
             dcol[n] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        // the tile-local shapes whose substance is a picture or gradient
+        uint64_t pic_lo = 0, pic_hi = 0;
+        for (int n = lane; n < c.num_shapes; n += kWave) {
+            if ((s.shapes[c.shape_start + n].tag & kTagTypeMask) != kTagSolid) {
+                if (n < 64) pic_lo |= 1ull << n; else pic_hi |= 1ull << (n - 64);
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            pic_lo |= __shfl_xor(pic_lo, o);
+            pic_hi |= __shfl_xor(pic_hi, o);
+        }
+        if (lane == 0) {
+            tt->pic0 = pic_lo;
+            tt->pic1 = pic_hi;
         }
         wave_sync();
     }

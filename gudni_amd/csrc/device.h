@@ -938,11 +938,48 @@ __device__ float4 grouped_color(const SceneDesc &s, TileTable *tt, Stack &st, in
     return make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+// A lane's colours of the current pixel for picture and gradient layers of a
+// compact tile, which depend on the shape and the pixel only: the tile's
+// first 32 such shapes (by rank among them, tt->pic0/pic1) keep theirs in
+// the first 128 words of the lane's shape scratch, which a compact tile
+// does not use.  known: which ranks hold the current pixel's colour (clear
+// it at every new pixel).
+struct LayerCache {
+    int32_t *words;
+    uint32_t known;
+
+    __device__ __forceinline__ float4 color(const SceneDesc &s, const TileTable *tt, int n, uint64_t tag,
+                                            float4 sub, int ax, int ay)
+    {
+        const uint64_t pm0 = tt->pic0, pm1 = tt->pic1;
+        const uint32_t rank =
+            n < 64 ? (uint32_t)__popcll(pm0 & ((1ull << n) - 1ull))
+                   : (uint32_t)(__popcll(pm0) + __popcll(pm1 & ((1ull << (n - 64)) - 1ull)));
+        if (rank < 32u && ((known >> rank) & 1u))
+            return make_float4(__int_as_float(words[(4 * rank + 0) * kWave]),
+                               __int_as_float(words[(4 * rank + 1) * kWave]),
+                               __int_as_float(words[(4 * rank + 2) * kWave]),
+                               __int_as_float(words[(4 * rank + 3) * kWave]));
+        const float4 c = read_color_sub(s, tag, sub, ax, ay);
+        if (rank < 32u) {
+            words[(4 * rank + 0) * kWave] = __float_as_int(c.x);
+            words[(4 * rank + 1) * kWave] = __float_as_int(c.y);
+            words[(4 * rank + 2) * kWave] = __float_as_int(c.z);
+            words[(4 * rank + 3) * kWave] = __float_as_int(c.w);
+            known |= 1u << rank;
+        }
+        return c;
+    }
+};
+
 // determineColor (Kernels.cl:1447-1513).  kCompact: the stack is labelled by
-// tile-local shape index and tags/colours come from the LDS table.
+// tile-local shape index and tags/colours come from the LDS table.  lc (a
+// compact tile's, or null): picture and gradient colours through the lane's
+// cache of the current pixel.
 template <bool kCompact>
 __device__ float4 determine_color(const Frame &f, const SceneDesc &s, const Stack &st, const int32_t *sidx,
-                                  const TileTable *tt, int ax, int ay, bool &positional)
+                                  const TileTable *tt, int ax, int ay, bool &positional,
+                                  LayerCache *lc = nullptr)
 {
     // The walk visits set stack bits from the top (findTop with a falling
     // ignoreAbove, Kernels.cl:281-292): pop the highest bit of a working copy.
@@ -983,8 +1020,11 @@ __device__ float4 determine_color(const Frame &f, const SceneDesc &s, const Stac
                 if (kCompact && solid) {
                     next = tt->color[top_bit];          // already in the mode's form
                 } else {
-                    next = kCompact ? read_color_sub(s, tag, tt->color[top_bit], ax, ay)
-                                    : read_color(s, tag, ax, ay);
+                    if (kCompact)
+                        next = lc ? lc->color(s, tt, top_bit, tag, tt->color[top_bit], ax, ay)
+                                  : read_color_sub(s, tag, tt->color[top_bit], ax, ay);
+                    else
+                        next = read_color(s, tag, ax, ay);
                     positional = positional || !solid;
                     if (!ref_mode)
                         next = make_float4(next.x * next.w, next.y * next.w, next.z * next.w, next.w);
