@@ -226,6 +226,13 @@ def render_f32(r, scene, spec):
     return f32.cpu().numpy().reshape(scene.height, scene.width, 4)
 
 
+# What the adjoint entry points take (gudni_adjoint_upload): the workloads
+# timed below lie inside it.
+ADJOINT_INPUTS = ("even-odd shapes with solid, gradient and picture substances, subtract compounds "
+                  "and shared substances; nonzero-fill shapes are refused; colour gradients for "
+                  "solid substances only")
+
+
 def adjoint_bytes(scene, spec):
     """Algorithmic bytes of one adjoint step: three passes over the tiles'
     shape records and strands (forward, backward, point pass: recomputation,
@@ -351,7 +358,7 @@ def adjoint_leg(spec, steps, log_fn, fixed=None):
             "ms_per_step_with_device_update": dev_loop_ms,
             "steps_per_s_with_device_update": 1e3 / dev_loop_ms,
             "loss_first": loss0, "loss_after_steps": loss, "n_points": int(params.pairs.size // 2),
-            "n_substances": int(len(params.substances))}
+            "n_substances": int(len(params.substances)), "inputs_accepted": ADJOINT_INPUTS}
 
 
 def batched_adjoint_leg(spec, steps, log_fn, fixed=None):
@@ -430,7 +437,7 @@ def batched_adjoint_leg(spec, steps, log_fn, fixed=None):
             "sequential_ms_per_step": seq_ms, "sequential_steps_per_s": 1e3 / seq_ms,
             "ms_per_step_with_device_update": loop_ms, "steps_per_s_with_device_update": 1e3 / loop_ms,
             "speedup_vs_sequential": seq_ms / one_ms,
-            "shapes": int(sum(s.n_shapes for s, _ in items))}
+            "shapes": int(sum(s.n_shapes for s, _ in items)), "inputs_accepted": ADJOINT_INPUTS}
 
 
 def batched_leg(spec, steps, log_fn, fixed=None):

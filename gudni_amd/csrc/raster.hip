@@ -987,6 +987,10 @@ static int run_job(gudni_ctx *ctx, const void *shapes, int32_t n_shapes, const v
     const size_t bw = window ? (size_t)(bx1 - bx0) : 0, bh = window ? (size_t)(by1 - by0) : 0;
     uint32_t *box_dev = (uint32_t *)ctx->out.p + (size_t)by0 * (size_t)w + (size_t)bx0;
     if (window) {
+        // a call that failed after queueing its copy from the staging buffer
+        // returned without waiting for it: no copy may still read the buffer
+        // that is refilled (or freed) here
+        CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
         if (ctx->l1_stage_bytes < 4 * bw * bh) {
             if (ctx->l1_stage) CHECK_HIP(ctx, hipHostFree(ctx->l1_stage));
             ctx->l1_stage = nullptr;

@@ -317,15 +317,16 @@ int   gudni_raster_scene(gudni_ctx *ctx, const gudni_scene *scene, uint32_t *hos
  * pre-quantisation RGBA.  Loss: L = sum over pixels and RGBA channels of
  * (image - target)^2.  Gradients: dL/d(outline points) in the layout of
  * gudni_outlines.pairs (on.x on.y off.x off.y per pair) and dL/d(substance
- * RGBA) for solid substances (zero for pictures).  Discrete decisions (knob
+ * RGBA) for solid substances (zero for pictures and gradients).  Discrete decisions (knob
  * splits, strand runs, tiles, curve subdivision, record drops, section
  * structure) are held; the knob-split parameter follows the analytic vertical
  * tangent.  Shapes may share substances and be subtract compounds: the
  * colour walk merges them as determineColor does (Kernels.cl:1447-1513) and
  * a substance's gradient sums over every layer it colours.  Nonzero-fill
  * shapes (GUDNI_TAG_NONZERO) are refused.  One persistent launch runs the
- * forward and, recomputing each tile's records, the backward, for all
- * scenes of the handle. */
+ * forward and the backward, for all scenes of the handle; the backward
+ * sweeps the records the forward left sorted (a unit that overflows its
+ * rings regenerates them in the retry pass). */
 typedef struct gudni_adjoint_scene {
     gudni_outlines outlines;            /* the geometry parameters           */
     const float *rgba;  int32_t n_substances;   /* 16 B substance records    */
