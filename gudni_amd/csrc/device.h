@@ -860,35 +860,49 @@ struct TileTable {
     float gc[4][4][64];
     uint64_t pic0, pic1;   // tile-local shapes whose substance is a picture
     int grouped;           // the tile qualifies for grouped colours
+    // per group of 32: the tile-local shapes some lane of the current unit
+    // holds a record or an enclosure of, i.e. every stack bit it can set
+    uint32_t cand[4];
 };
 static_assert(sizeof(TileTable::gc) >= sizeof(uint16_t) * kWave * (1 << kRadixMaxBits),
               "the sort histogram lives in the colour cache");
 
-// Composite of one group's set shapes, top-down, premultiplied.  Once the
-// alpha is exactly 1 every later layer adds its colour times 0, so the walk
-// stops there with the same result.
+// Composite of one group's set shapes (word: this lane's bits, 0 for a lane
+// that takes no part), top-down, premultiplied.  Once the alpha is exactly 1
+// every later layer adds its colour times 0, so a lane stops there with the
+// same result.  The wave walks the group's shapes together, from the top of
+// `cand` (wave-uniform, a superset of every lane's bits): each shape is
+// composited by the lanes that hold it.  A lane meets its own shapes in its
+// own order, as a walk of its bits alone would, but the wave evaluates one
+// substance at a time, whose tag and record every lane reads alike.
 __device__ __forceinline__ float4 group_composite(const SceneDesc &s, const TileTable *tt,
-                                                  uint32_t word, int g, int ax, int ay)
+                                                  uint32_t word, int g, int ax, int ay, uint32_t cand)
 {
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    while (word) {
-        const int b = 31 - __clz((int)word);
-        word &= ~(1u << b);
-        const int n = 32 * g + b;
-        float4 pm;
-        const uint64_t tag = tt->tag[n];
-        if ((tag & kTagTypeMask) == kTagSolid) {
-            pm = tt->color[n];
-        } else {
-            const float4 c = read_color_sub(s, tag, tt->color[n], ax, ay);
-            pm = make_float4(c.x * c.w, c.y * c.w, c.z * c.w, c.w);
+    cand = __builtin_amdgcn_readfirstlane(cand);
+    while (cand) {
+        if (!__any(word != 0u)) break;
+        const int b = 31 - __clz((int)cand);
+        cand &= ~(1u << b);
+        if ((word >> b) & 1u) {
+            word &= ~(1u << b);
+            const int n = 32 * g + b;
+            const uint64_t tv = tt->tag[n];
+            const uint64_t tag = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(tv >> 32)) << 32) |
+                                 (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)tv);
+            float4 pm = tt->color[n];
+            if ((tag & kTagTypeMask) != kTagSolid) {
+                pm.x = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(pm.x)));
+                const float4 c = read_color_sub(s, tag, pm, ax, ay);
+                pm = make_float4(c.x * c.w, c.y * c.w, c.z * c.w, c.w);
+            }
+            const float t = 1.0f - acc.w;
+            acc.x = acc.x + pm.x * t;
+            acc.y = acc.y + pm.y * t;
+            acc.z = acc.z + pm.z * t;
+            acc.w = acc.w + pm.w * t;
+            if (acc.w == 1.0f) word = 0u;
         }
-        const float t = 1.0f - acc.w;
-        acc.x = acc.x + pm.x * t;
-        acc.y = acc.y + pm.y * t;
-        acc.z = acc.z + pm.z * t;
-        acc.w = acc.w + pm.w * t;
-        if (acc.w == 1.0f) break;
     }
     return acc;
 }
@@ -904,26 +918,32 @@ __device__ float4 grouped_color(const SceneDesc &s, TileTable *tt, Stack &st, in
                                (uint32_t)(st.s1 >> 32)};
     float P0 = 0.f, P1 = 0.f, P2 = 0.f, A = 0.f;
     uint32_t fresh = 0;
+    bool open = true;   // no group so far has brought the alpha to 1
 #pragma unroll
     for (int g = 3; g >= 0; --g) {
-        if (A == 1.0f) break;
-        float4 gcol;
-        if (st.dirty & (1u << g)) {
-            gcol = group_composite(s, tt, words[g], g, ax, ay);
+        open = open && A != 1.0f;
+        const bool stale = open && (st.dirty & (1u << g)) != 0u;
+        float4 gcol = make_float4(0.f, 0.f, 0.f, 0.f);
+        // every lane of the sweep step takes part in the walk (with no bits
+        // unless its group is stale)
+        if (__any(stale)) gcol = group_composite(s, tt, stale ? words[g] : 0u, g, ax, ay, tt->cand[g]);
+        if (stale) {
             tt->gc[g][0][lane] = gcol.x;
             tt->gc[g][1][lane] = gcol.y;
             tt->gc[g][2][lane] = gcol.z;
             tt->gc[g][3][lane] = gcol.w;
             fresh |= 1u << g;
-        } else {
+        } else if (open) {
             gcol = make_float4(tt->gc[g][0][lane], tt->gc[g][1][lane], tt->gc[g][2][lane],
                                tt->gc[g][3][lane]);
         }
-        const float t = 1.0f - A;
-        P0 = P0 + gcol.x * t;
-        P1 = P1 + gcol.y * t;
-        P2 = P2 + gcol.z * t;
-        A = A + gcol.w * t;
+        if (open) {
+            const float t = 1.0f - A;
+            P0 = P0 + gcol.x * t;
+            P1 = P1 + gcol.y * t;
+            P2 = P2 + gcol.z * t;
+            A = A + gcol.w * t;
+        }
     }
     st.dirty &= ~fresh;
     {
@@ -1307,6 +1327,24 @@ __device__ void run_unit(const Frame &f, const SceneDesc &s, int lane, uint32_t 
         const int x1 = x0 + unit_width(t.hdepth, f.compute_depth);
         generate<kCompact>(f, s, q, c, active, lane, (float)x0 - 1.0f, (float)x1 + 1.0f, sidx, bits,
                            s0, s1, c0, c1, any_nonzero);
+    }
+    if (kCompact) {
+        // a lane's stack only ever holds bits of shapes it took a record or
+        // an enclosure of: their union over the wave bounds the colour walks
+        uint64_t u0 = c0, u1 = c1;
+        for (int o = 32; o > 0; o >>= 1) {
+            u0 |= __shfl_xor(u0, o);
+            u1 |= __shfl_xor(u1, o);
+        }
+        if (lane == 0) {
+            tt->cand[0] = (uint32_t)u0;
+            tt->cand[1] = (uint32_t)(u0 >> 32);
+            tt->cand[2] = (uint32_t)u1;
+            tt->cand[3] = (uint32_t)(u1 >> 32);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     const bool ok = active && !q.spilled && !q.overflow;
     const long long t1 = timed ? clock64() : 0;
