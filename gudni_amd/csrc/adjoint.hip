@@ -356,10 +356,10 @@ __device__ void spawn_o(Queue &q, const Column &c, uint32_t bit, const Trav &l, 
 // buildThresholdArray with origins: the same shape walk, prefilter and
 // labels as the raster path's generate, so records and their order match it.
 template <bool kCompact, int kMode>
-__device__ void generate_o(const Frame &f, const AdjDesc &d, Queue &q, const Column &c, bool active,
-                           int lane, float span0, float span1, int32_t *sidx, uint32_t &shape_bits,
-                           uint64_t &stack0, uint64_t &stack1, float4 *orig, float4 *ends = nullptr,
-                           uint32_t *lines = nullptr)
+__device__ __forceinline__ void generate_o(const Frame &f, const AdjDesc &d, Queue &q, const Column &c,
+                                           bool active, int lane, float span0, float span1, int32_t *sidx,
+                                           uint32_t &shape_bits, uint64_t &stack0, uint64_t &stack1,
+                                           float4 *orig, float4 *ends = nullptr, uint32_t *lines = nullptr)
 {
     const SceneDesc &s = d.s;
     const float dx = (float)c.dx, dy = (float)c.dy;
@@ -453,7 +453,8 @@ __device__ __forceinline__ void apply_recorded_end(const AdjDesc &d, int curve, 
 // Phase C from the recorded ends: every generated line's accumulated
 // weights -> its end points -> outline points, in generation order, as the
 // regenerating walk (generate_o<kApply>) applies them.
-__device__ void apply_lines(const AdjDesc &d, const float4 *orig, const float4 *ends, uint32_t n_lines)
+__device__ __forceinline__ void apply_lines(const AdjDesc &d, const float4 *orig, const float4 *ends,
+                                            uint32_t n_lines)
 {
     for (uint32_t k = 0; k < n_lines; ++k) {
         const float4 o = orig[2 * k * kWave];
@@ -654,8 +655,8 @@ __device__ __forceinline__ void span_term(float4 *orig, int org, int end, const 
 
 // Forward sweep: the image, per-pixel dL/d(accumulated colour), the loss.
 template <bool kCompact>
-__device__ uint32_t sweep_forward(const Frame &f, const AdjDesc &d, Queue &q, const Column &c, Stack st,
-                                  const int32_t *sidx, const TileTable *tt, float &loss)
+__device__ __forceinline__ uint32_t sweep_forward(const Frame &f, const AdjDesc &d, Queue &q, const Column &c,
+                                                  Stack st, const int32_t *sidx, const TileTable *tt, float &loss)
 {
     const SceneDesc &s = d.s;
     Sweep p;
@@ -719,9 +720,9 @@ __device__ uint32_t sweep_forward(const Frame &f, const AdjDesc &d, Queue &q, co
 
 // Backward sweep over the same sections.
 template <bool kCompact>
-__device__ uint32_t sweep_backward(const Frame &f, const AdjDesc &d, Queue &q, const Column &c, Stack st,
-                                   const int32_t *sidx, const TileTable *tt, const float4 *straight,
-                                   float4 *dcol, float4 *orig)
+__device__ __forceinline__ uint32_t sweep_backward(const Frame &f, const AdjDesc &d, Queue &q, const Column &c,
+                                                   Stack st, const int32_t *sidx, const TileTable *tt,
+                                                   const float4 *straight, float4 *dcol, float4 *orig)
 {
     const SceneDesc &s = d.s;
     Sweep p;
@@ -863,10 +864,16 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// One work unit, phases A-D.  It and the stages it calls (generation, both
+// sweeps, the line map) are inlined into the kernel: a call would take the
+// frame, the scene descriptor and the queues by address, which keeps them in
+// per-lane scratch memory for the whole unit (848 bytes a lane with 46
+// spilled VGPRs, against 80 bytes and none inlined).
 template <bool kCompact>
-__device__ void adj_unit(const Frame &f, const AdjDesc &d, int lane, uint32_t unit, Queue &q, Queue &fq,
-                         int32_t *sidx, TileTable *tt, float4 *straight, float4 *dcol, float4 *orig,
-                         float4 *ends, uint32_t &status, float &loss)
+__device__ __forceinline__ void adj_unit(const Frame &f, const AdjDesc &d, int lane, uint32_t unit, Queue &q,
+                                         Queue &fq, int32_t *sidx, TileTable *tt, float4 *straight,
+                                         float4 *dcol, float4 *orig, float4 *ends, uint32_t &status,
+                                         float &loss)
 {
     const SceneDesc &s = d.s;
     const int tile = (int)(unit >> 2), quarter = (int)(unit & 3u);
@@ -927,26 +934,32 @@ __device__ void adj_unit(const Frame &f, const AdjDesc &d, int lane, uint32_t un
     bool ok = active && !q.spilled && !q.overflow;
     if (ok) radix_sort_queue(q, hist, s.sort_bits, c.float_height);
     stamp(1);
-    // the forward sweep splits and pops records: with a ring of its own it
-    // runs on a copy of the sorted queue and leaves q as the backward sweep
-    // needs it (the same records a regeneration would give, at the cost of
-    // one copy)
+    // the forward sweep splits and pops records, and runs on fq: with a ring
+    // of its own, fq holds a copy of the sorted queue and q stays as the
+    // backward sweep needs it (the same records a regeneration would give,
+    // at the cost of one copy); without one, fq is q's ring and the backward
+    // sweep regenerates.  (One queue object per role keeps both in
+    // registers.)  A forward sweep that outgrows its ring hands the unit to
+    // the retry pass like a full generation: its flag goes to q.
     const bool copied = fq.thr != q.thr;
+    fq.spilled = false;
+    fq.overflow = false;
     if (copied) {
         fq.head = 0;
         fq.len = 0;
-        fq.spilled = false;
-        fq.overflow = false;
         if (ok) {
             for (int i = 0; i < q.len; ++i) fq.set(i, q.get_h(i), q.get(i));
             fq.len = q.len;
         }
+    } else {
+        fq.head = q.head;
+        fq.len = q.len;
     }
-    Queue &sq = copied ? fq : q;
-    if (ok) status = sweep_forward<kCompact>(f, d, sq, c, Stack{s0, s1, 0u}, sidx, tt, loss);
+    if (ok) status = sweep_forward<kCompact>(f, d, fq, c, Stack{s0, s1, 0u}, sidx, tt, loss);
     stamp(2);
-    if (active && (q.overflow || sq.overflow)) status |= kStatusOverflow;
-    if (__any(q.spilled || sq.spilled) || __any(status != 0)) return;
+    q.spilled = q.spilled || fq.spilled;
+    if (active && (q.overflow || fq.overflow)) status |= kStatusOverflow;
+    if (__any(q.spilled) || __any(status != 0)) return;
     // B: the sorted records again (kept, or recomputed), backward sweep
     if (!copied) {
         q.head = 0; q.len = 0;

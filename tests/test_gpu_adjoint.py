@@ -136,10 +136,14 @@ def test_torch_autograd_wrapper_descends(rasterizer):
 
 
 def _same_result(a, b, what):
-    """Equal up to the order of atomic accumulation (the image bit for bit)."""
+    """Equal up to the order of atomic accumulation (the image bit for bit).
+    A gradient is an FP32 sum of per-tile partial sums added atomically in
+    whatever order the tiles finish; with cancellation among ~10^4 tiles two
+    orders differ by ~1e-4 relative (a substance's colour gradient of 7.7e5
+    on a 2048^2 scene: 1.09e-4), so the bound is 1e-3."""
     assert abs(a[0] - b[0]) <= 1e-6 * max(1.0, abs(a[0])), f"{what}: loss {a[0]} vs {b[0]}"
-    np.testing.assert_allclose(a[1], b[1], rtol=1e-4, atol=1e-6 * max(1e-30, np.abs(a[1]).max()), err_msg=what)
-    np.testing.assert_allclose(a[2], b[2], rtol=1e-4, atol=1e-6 * max(1e-30, np.abs(a[2]).max()), err_msg=what)
+    np.testing.assert_allclose(a[1], b[1], rtol=1e-3, atol=1e-5 * max(1e-30, np.abs(a[1]).max()), err_msg=what)
+    np.testing.assert_allclose(a[2], b[2], rtol=1e-3, atol=1e-5 * max(1e-30, np.abs(a[2]).max()), err_msg=what)
     np.testing.assert_array_equal(a[3], b[3], err_msg=what)
 
 
