@@ -69,6 +69,42 @@ def test_adjoint_forward_matches_oracle(rasterizer, adjoint_small, tile):
     assert np.all(np.isfinite(dp)) and np.all(np.isfinite(dc))
 
 
+def fan_scene(k=50):
+    """One polygon zig-zagging 2k times between x = 10.2 and 10.8 inside
+    column 10 of a 64^2 canvas, each edge its own strand, the edges' tops
+    staggered down the first 16 rows and their bottoms at row 63.5.  At 64-px
+    tiles the column's top 16-row segment generates one record per edge,
+    within the fast pass's 128-record rings, but its sweep splits every
+    started record at each later top, so the live queue grows to about twice
+    that before the split-off tops are popped."""
+    from gudni_amd import scenes as S
+    pts = []
+    for i in range(k):
+        pts.append((10.2, 0.5 + 0.3 * i))
+        pts.append((10.8, 63.5))
+    return S.build_scene(64, 64, [(S.solid_tag(0), [S.straight_pairs(pts)])], [[0.3, 0.6, 0.2, 0.8]],
+                         name="fan")
+
+
+def test_forward_sweep_outgrowing_its_ring_is_retried():
+    """A unit whose records fit the fast pass's ring but whose forward sweep
+    outgrows it (the sweep runs on a copy of the sorted records) goes to the
+    full-capacity pass like one whose generation overflows: its pixels come
+    out as the oracle's."""
+    sc = fan_scene()
+    target = np.zeros((sc.height, sc.width, 4), np.float32)
+    say("adjoint forward of a sweep that outgrows its ring")
+    r = G.Rasterizer(G.default_spec(max_tile_size=64))
+    try:
+        loss, dp, dc, img = gpu_adjoint(r, sc, target)
+    finally:
+        r.close()
+    ref = H.oracle_render(sc, G.default_spec(max_tile_size=64), want_f64=True)
+    err = np.abs(img.astype(np.float64) - ref["f64"].reshape(img.shape)).max()
+    assert err <= COVERAGE_TOL, f"forward image differs from the oracle by {err}"
+    assert np.all(np.isfinite(dp)) and np.all(np.isfinite(dc))
+
+
 @pytest.mark.parametrize("tile", [256, 64, 32, 16])
 def test_adjoint_gradients_match_finite_differences(rasterizer, adjoint_small, tile):
     sc, target = adjoint_small
