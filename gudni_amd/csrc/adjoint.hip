@@ -973,6 +973,11 @@ __device__ __forceinline__ void adj_unit(const Frame &f, const AdjDesc &d, int l
     stamp(4);
     if (ok) status = sweep_backward<kCompact>(f, d, q, c, Stack{s0, s1, 0u}, sidx, tt, straight, dcol, orig);
     stamp(5);
+    // B repeats A's queue operations on the same records, so it cannot
+    // outgrow a ring the forward sweep fit in; should it, the unit goes to
+    // the retry pass before C and D flush anything, so that nothing is
+    // counted twice
+    if (__any(q.spilled)) return;
     // C: line weights -> outline points, from the ends recorded in A (or
     // by walking the generation once more)
     if (ends) {
