@@ -591,13 +591,16 @@ enum Cross { kCrossRight, kCrossLeft, kCrossDown };
 
 struct Stack {
     uint64_t s0, s1;
-    uint32_t dirty;   // groups of 32 bits touched since their composite was cached
+    uint32_t dirty;   // colour groups touched since their composite was cached
+    // a bit's colour group is (b * gm) >> 16, gm = ceil(2^16 / the group
+    // width) (TileTable::gm; exact for b < 128); 2048: groups of 32
+    uint32_t gm = 2048u;
     __device__ __forceinline__ void flip(uint32_t h)
     {
         const uint32_t b = h & kLabelMask;
         if (b < 64u) s0 ^= 1ull << b;
         else s1 ^= 1ull << (b - 64u);
-        dirty |= 1u << (b >> 5);
+        dirty |= 1u << ((b * gm) >> 16);
     }
     // Crossing record h: an even-odd shape flips its bit (passHeader,
     // Kernels.cl:298-304).  A nonzero-fill shape adds the crossing's winding
@@ -627,7 +630,7 @@ struct Stack {
         const uint64_t m = 1ull << (b & 63u);
         uint64_t &word = b < 64u ? s0 : s1;
         word = v != 0 ? (word | m) : (word & ~m);
-        dirty |= 1u << (b >> 5);
+        dirty |= 1u << ((b * gm) >> 16);
     }
 };
 
@@ -860,8 +863,11 @@ struct TileTable {
     float gc[4][4][64];
     uint64_t pic0, pic1;   // tile-local shapes whose substance is a picture
     int grouped;           // the tile qualifies for grouped colours
-    // per group of 32: the tile-local shapes some lane of the current unit
-    // holds a record or an enclosure of, i.e. every stack bit it can set
+    // colour group g holds tile-local shapes [g gq, (g+1) gq), gq = ceil(n / 4)
+    // for a tile of n shapes; gm = ceil(2^16 / gq) maps a shape to its group
+    uint32_t gq, gm;
+    // per group: the tile-local shapes some lane of the current unit holds a
+    // record or an enclosure of, i.e. every stack bit it can set
     uint32_t cand[4];
 };
 static_assert(sizeof(TileTable::gc) >= sizeof(uint16_t) * kWave * (1 << kRadixMaxBits),
@@ -875,8 +881,19 @@ static_assert(sizeof(TileTable::gc) >= sizeof(uint16_t) * kWave * (1 << kRadixMa
 // composited by the lanes that hold it.  A lane meets its own shapes in its
 // own order, as a walk of its bits alone would, but the wave evaluates one
 // substance at a time, whose tag and record every lane reads alike.
+// Bits [start, start + q) of the 128-bit set (s1:s0), q <= 32, start < 128;
+// bits past 127 read as 0.
+__device__ __forceinline__ uint32_t group_bits(uint64_t s0, uint64_t s1, uint32_t start, uint32_t q)
+{
+    uint64_t v;
+    if (start >= 64u) v = s1 >> (start - 64u);
+    else if (start == 0u) v = s0;
+    else v = (s0 >> start) | (s1 << (64u - start));
+    return (uint32_t)v & (q >= 32u ? 0xFFFFFFFFu : ((1u << q) - 1u));
+}
+
 __device__ __forceinline__ float4 group_composite(const SceneDesc &s, const TileTable *tt,
-                                                  uint32_t word, int g, int ax, int ay, uint32_t cand)
+                                                  uint32_t word, int n0, int ax, int ay, uint32_t cand)
 {
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
     cand = __builtin_amdgcn_readfirstlane(cand);
@@ -886,7 +903,7 @@ __device__ __forceinline__ float4 group_composite(const SceneDesc &s, const Tile
         cand &= ~(1u << b);
         if ((word >> b) & 1u) {
             word &= ~(1u << b);
-            const int n = 32 * g + b;
+            const int n = n0 + b;
             const uint64_t tv = tt->tag[n];
             const uint64_t tag = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(tv >> 32)) << 32) |
                                  (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)tv);
@@ -914,8 +931,9 @@ __device__ __forceinline__ float4 group_composite(const SceneDesc &s, const Tile
 __device__ float4 grouped_color(const SceneDesc &s, TileTable *tt, Stack &st, int lane, int ax,
                                 int ay)
 {
-    const uint32_t words[4] = {(uint32_t)st.s0, (uint32_t)(st.s0 >> 32), (uint32_t)st.s1,
-                               (uint32_t)(st.s1 >> 32)};
+    const uint32_t q = __builtin_amdgcn_readfirstlane(tt->gq);
+    const uint32_t words[4] = {group_bits(st.s0, st.s1, 0u, q), group_bits(st.s0, st.s1, q, q),
+                               group_bits(st.s0, st.s1, 2u * q, q), group_bits(st.s0, st.s1, 3u * q, q)};
     float P0 = 0.f, P1 = 0.f, P2 = 0.f, A = 0.f;
     uint32_t fresh = 0;
     bool open = true;   // no group so far has brought the alpha to 1
@@ -926,7 +944,8 @@ __device__ float4 grouped_color(const SceneDesc &s, TileTable *tt, Stack &st, in
         float4 gcol = make_float4(0.f, 0.f, 0.f, 0.f);
         // every lane of the sweep step takes part in the walk (with no bits
         // unless its group is stale)
-        if (__any(stale)) gcol = group_composite(s, tt, stale ? words[g] : 0u, g, ax, ay, tt->cand[g]);
+        if (__any(stale))
+            gcol = group_composite(s, tt, stale ? words[g] : 0u, (int)(g * q), ax, ay, tt->cand[g]);
         if (stale) {
             tt->gc[g][0][lane] = gcol.x;
             tt->gc[g][1][lane] = gcol.y;
@@ -1124,6 +1143,8 @@ __device__ uint32_t render(const Frame &f, const SceneDesc &s, Queue &q, const C
     const bool grouped = kCompact && tt->grouped;
     const int lane = threadIdx.x & (kWave - 1);
     st.dirty = 0xFu;
+    if (grouped) st.gm = __builtin_amdgcn_readfirstlane(tt->gm);
+    const uint32_t gq = grouped ? __builtin_amdgcn_readfirstlane(tt->gq) : 32u;
     p.pixel_y = 1.0f;
     if (!(p.pixel_y <= c.float_height)) return 0;
     float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -1131,8 +1152,8 @@ __device__ uint32_t render(const Frame &f, const SceneDesc &s, Queue &q, const C
     if (grouped) {
         // picture colours depend on the pixel: their groups are stale
         const uint64_t p0 = st.s0 & tt->pic0, p1 = st.s1 & tt->pic1;
-        st.dirty |= ((uint32_t)p0 ? 1u : 0u) | ((p0 >> 32) ? 2u : 0u) |
-                    ((uint32_t)p1 ? 4u : 0u) | ((p1 >> 32) ? 8u : 0u);
+        st.dirty |= (group_bits(p0, p1, 0u, gq) ? 1u : 0u) | (group_bits(p0, p1, gq, gq) ? 2u : 0u) |
+                    (group_bits(p0, p1, 2u * gq, gq) ? 4u : 0u) | (group_bits(p0, p1, 3u * gq, gq) ? 8u : 0u);
     }
     for (;;) {
         if ((p.sx1 < 1.0f) || (p.sy1 < p.pixel_y)) {
@@ -1200,8 +1221,9 @@ __device__ uint32_t render(const Frame &f, const SceneDesc &s, Queue &q, const C
             area_sum = 0.0f;
             if (grouped) {
                 const uint64_t p0 = st.s0 & tt->pic0, p1 = st.s1 & tt->pic1;
-                st.dirty |= ((uint32_t)p0 ? 1u : 0u) | ((p0 >> 32) ? 2u : 0u) |
-                            ((uint32_t)p1 ? 4u : 0u) | ((p1 >> 32) ? 8u : 0u);
+                st.dirty |= (group_bits(p0, p1, 0u, gq) ? 1u : 0u) | (group_bits(p0, p1, gq, gq) ? 2u : 0u) |
+                            (group_bits(p0, p1, 2u * gq, gq) ? 4u : 0u) |
+                            (group_bits(p0, p1, 3u * gq, gq) ? 8u : 0u);
             }
         }
     }
@@ -1313,6 +1335,9 @@ __device__ void run_unit(const Frame &f, const SceneDesc &s, int lane, uint32_t 
             tt->pic0 = pic_lo;
             tt->pic1 = pic_hi;
             tt->grouped = grouped_tile ? 1 : 0;
+            const uint32_t gq = c.num_shapes > 4 ? (uint32_t)(c.num_shapes + 3) / 4u : 1u;
+            tt->gq = gq;
+            tt->gm = (65536u + gq - 1u) / gq;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -1337,10 +1362,8 @@ __device__ void run_unit(const Frame &f, const SceneDesc &s, int lane, uint32_t 
             u1 |= __shfl_xor(u1, o);
         }
         if (lane == 0) {
-            tt->cand[0] = (uint32_t)u0;
-            tt->cand[1] = (uint32_t)(u0 >> 32);
-            tt->cand[2] = (uint32_t)u1;
-            tt->cand[3] = (uint32_t)(u1 >> 32);
+            const uint32_t q = tt->gq;
+            for (uint32_t g = 0; g < 4u; ++g) tt->cand[g] = group_bits(u0, u1, g * q, q);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();

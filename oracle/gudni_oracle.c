@@ -1085,17 +1085,18 @@ static void read_color(const colorctx_t *cs, uint64_t tag, float *out)
 /* Accurate-mode colour of a grouped tile (see run_column): every visited
  * shape composites its own colour, so the walk is a plain premultiplied
  * "over" of the set shapes, evaluated as over(group 3, group 2, group 1,
- * group 0, background) where group g holds tile-local shapes
- * [32g, 32g+32) composited top-down. */
+ * group 0, background) where, in a tile of n shapes, group g holds
+ * tile-local shapes [g q, (g+1) q), q = ceil(n / 4), composited top-down. */
 static void grouped_color(col_t *c, const colorctx_t *cs, float *base)
 {
     float P[3] = {0, 0, 0}, A = 0.0f;
+    const int q = c->num_shapes > 4 ? (c->num_shapes + 3) / 4 : 1;
     for (int g = 3; g >= 0; g--) {
         float gp[3] = {0, 0, 0}, ga = 0.0f;
         for (int b = c->spec->max_shapes - 1; b >= 0; b--) {
             if (!((c->stack[b >> 6] >> (b & 63)) & 1ull)) continue;
             int n = c->shape_indices[b] - c->shape_start;
-            if ((n >> 5) != g) continue;
+            if (n / q != g) continue;
             uint64_t tag; memcpy(&tag, cs->shapes + 16L * c->shape_indices[b], 8);
             float col[4];
             read_color(cs, tag, col);
