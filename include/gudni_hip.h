@@ -115,8 +115,11 @@ typedef struct gudni_gradient {
  * un-premultiplies after every shape (three divisions per step, composite,
  * Kernels.cl:878-887); ACCURATE keeps the accumulated colour premultiplied
  * (P += rgb*a*(1-A), A += a*(1-A) with the reference's alpha expression, so
- * the walk stops at the same shape) and divides once per walk.  Frames of the
- * two modes differ by at most one 8-bit step. */
+ * the walk stops at the same shape) and divides once per walk; on a tile of
+ * n shapes, all added with distinct substances, it composites four groups of
+ * ceil(n/4) consecutive shapes and then the groups (the same "over", rounded
+ * as that grouping rounds it).  Frames of the two modes differ by at most one
+ * 8-bit step. */
 #define GUDNI_COVERAGE_ACCURATE  0
 #define GUDNI_COVERAGE_REFERENCE 1
 
