@@ -141,26 +141,48 @@ __device__ Cut intersect_track(Trav t)
     return c;
 }
 
-// searchTree, also returning the in-order index of the curve found.
-__device__ __forceinline__ int search_curve(Trav &tr, const GUDNI_CONST uint8_t *tree, int tree_size,
-                                            float dx, float dy, bool is_left)
+
+// searchTree (Kernels.cl:1337-1357), also giving the in-order index of the
+// curve found, for the column's left border (l, which also takes a node at
+// its x) and right border (r) in one loop, their node reads issued together;
+// the curves' indices go to lcur and rcur.
+__device__ __forceinline__ void search_curve_pair(Trav &l, Trav &r, const GUDNI_CONST uint8_t *tree, int tree_size,
+                                                  float dx, float dy, int &lcur, int &rcur)
 {
-    tr.index = 0;
-    int last = -1;
-    while (tr.index < tree_size) {
-        const float4 n = ld4(tree + 16 * tr.index);
-        const float px = n.x - dx, py = n.y - dy, cx = n.z - dx, cy = n.w - dy;
-        if ((tr.xpos < px) || (is_left && tr.xpos == px)) {
-            tr.right = make_float2(px, py);
-            tr.index = (tr.index << 1) + 1;
-        } else {
-            tr.left = make_float2(px, py);
-            tr.ctl = make_float2(cx, cy);
-            last = tr.index;
-            tr.index = (tr.index << 1) + 2;
+    l.index = 0;
+    r.index = 0;
+    int llast = -1, rlast = -1;
+    while (l.index < tree_size || r.index < tree_size) {
+        const bool ml = l.index < tree_size, mr = r.index < tree_size;
+        const float4 nl = ld4(tree + 16 * (ml ? l.index : 0));
+        const float4 nr = ld4(tree + 16 * (mr ? r.index : 0));
+        if (ml) {
+            const float px = nl.x - dx, py = nl.y - dy, cx = nl.z - dx, cy = nl.w - dy;
+            if ((l.xpos < px) || (l.xpos == px)) {
+                l.right = make_float2(px, py);
+                l.index = (l.index << 1) + 1;
+            } else {
+                l.left = make_float2(px, py);
+                l.ctl = make_float2(cx, cy);
+                llast = l.index;
+                l.index = (l.index << 1) + 2;
+            }
+        }
+        if (mr) {
+            const float px = nr.x - dx, py = nr.y - dy, cx = nr.z - dx, cy = nr.w - dy;
+            if (r.xpos < px) {
+                r.right = make_float2(px, py);
+                r.index = (r.index << 1) + 1;
+            } else {
+                r.left = make_float2(px, py);
+                r.ctl = make_float2(cx, cy);
+                rlast = r.index;
+                r.index = (r.index << 1) + 2;
+            }
         }
     }
-    return last < 0 ? 0 : 1 + (int)kInOrder.v[tree_size + 1][last];
+    lcur = llast < 0 ? 0 : 1 + (int)kInOrder.v[tree_size + 1][llast];
+    rcur = rlast < 0 ? 0 : 1 + (int)kInOrder.v[tree_size + 1][rlast];
 }
 
 __device__ __forceinline__ float2 bez(const Trav &t, float s)
@@ -414,8 +436,10 @@ __device__ __forceinline__ void generate_o(const Frame &f, const AdjDesc &d, Que
                     l.xpos = cl_max(0.0f, l.left.x);
                     r.xpos = cl_min(1.0f, l.right.x);
                     const int tree_size = ((int)size - 4) / 2;
-                    const int lcur = curve_base + search_curve(l, strand + 32, tree_size, dx, dy, true);
-                    const int rcur = curve_base + search_curve(r, strand + 32, tree_size, dx, dy, false);
+                    int lcur, rcur;
+                    search_curve_pair(l, r, strand + 32, tree_size, dx, dy, lcur, rcur);
+                    lcur += curve_base;
+                    rcur += curve_base;
                     spawn_o<kMode>(q, c, label, l, r, lcur, rcur, added, enclosed_strand, orig, ends, n_orig,
                                    d);
                 }

