@@ -892,14 +892,6 @@ struct TileTable {
 static_assert(sizeof(TileTable::gc) >= sizeof(uint16_t) * kWave * (1 << kRadixMaxBits),
               "the sort histogram lives in the colour cache");
 
-// Composite of one group's set shapes (word: this lane's bits, 0 for a lane
-// that takes no part), top-down, premultiplied.  Once the alpha is exactly 1
-// every later layer adds its colour times 0, so a lane stops there with the
-// same result.  The wave walks the group's shapes together, from the top of
-// `cand` (wave-uniform, a superset of every lane's bits): each shape is
-// composited by the lanes that hold it.  A lane meets its own shapes in its
-// own order, as a walk of its bits alone would, but the wave evaluates one
-// substance at a time, whose tag and record every lane reads alike.
 // Bits [start, start + q) of the 128-bit set (s1:s0), q <= 32, start < 128;
 // bits past 127 read as 0.
 __device__ __forceinline__ uint32_t group_bits(uint64_t s0, uint64_t s1, uint32_t start, uint32_t q)
@@ -911,6 +903,14 @@ __device__ __forceinline__ uint32_t group_bits(uint64_t s0, uint64_t s1, uint32_
     return (uint32_t)v & (q >= 32u ? 0xFFFFFFFFu : ((1u << q) - 1u));
 }
 
+// Composite of one group's set shapes (word: this lane's bits, 0 for a lane
+// that takes no part), top-down, premultiplied.  Once the alpha is exactly 1
+// every later layer adds its colour times 0, so a lane stops there with the
+// same result.  The wave walks the group's shapes together, from the top of
+// `cand` (wave-uniform, a superset of every lane's bits): each shape is
+// composited by the lanes that hold it.  A lane meets its own shapes in its
+// own order, as a walk of its bits alone would, but the wave evaluates one
+// substance at a time, whose tag and record every lane reads alike.
 __device__ __forceinline__ float4 group_composite(const SceneDesc &s, const TileTable *tt,
                                                   uint32_t word, int n0, int ax, int ay, uint32_t cand)
 {
